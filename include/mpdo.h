@@ -1,0 +1,149 @@
+/*
+ * libmpdo_gfx950 — C ABI of the MI355X (gfx950) implementation of the MPDO noisy-circuit
+ * update path of WeiguoMa/Tomography-assisted-MPDO-QCircuit (package MPDOSimulator).
+ *
+ * Data model: one site tensor per qubit, complex64 (interleaved re, im floats), row-major
+ * T[l][2][a][r] in device memory: l = bond to the left neighbour (bond_{i-1}_{i}), 2 = physics_i,
+ * a = inner/Kraus index (I_i), r = bond to the right neighbour. A missing edge is a dimension 1.
+ * Output tensors are written into caller-allocated device buffers whose capacity the matching
+ * *_dims / *_capacity query gives; the entry points return the actual dimensions in the
+ * output descriptors. Every entry point is stream-ordered on `stream` (a hipStream_t, NULL =
+ * default stream) and returns after the results are complete when it has to read a
+ * data-dependent rank back (documented per call). Workspace is a library-internal,
+ * grow-only device buffer per device; calls from several host threads must not overlap.
+ */
+#ifndef MPDO_H
+#define MPDO_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  MPDO_OK = 0,
+  MPDO_EINVAL = 1,   /* bad shape/argument (maps to ValueError/TypeError)                    */
+  MPDO_ENOCONV = 2,  /* eigensolver sweep cap hit (maps to torch.linalg.LinAlgError)          */
+  MPDO_ENOMEM = 3,   /* device allocation of the workspace failed                            */
+  MPDO_EHIP = 4      /* HIP runtime error                                                    */
+} mpdo_status;
+
+typedef struct {
+  void* data; /* device pointer, complex64 T[l][2][a][r] */
+  int l;
+  int a;
+  int r;
+} mpdo_site;
+
+int mpdo_version(void);
+/* Thread-local text of the last failure. */
+const char* mpdo_last_error(void);
+/* Frees the internal workspace of the current device. */
+void mpdo_release_workspace(void);
+/*
+ * Accumulation precision of the complex64 tensor contractions (no counterpart in the
+ * reference, which computes in the dtype it is given): 0 = fp64 matrix-core accumulation
+ * (default; complex64 storage), 1 = fp32 matrix-core accumulation (twice the peak rate).
+ * Gram matrices and eigensolves are always fp64. Returns the previous mode.
+ */
+int mpdo_set_math_mode(int mode);
+
+/*
+ * Single-qubit gate / Kraus absorption, batched over sites.
+ * Replaces Circuit.py:138-178 (TensorCircuit._apply_single_qubit_gate) for each target qubit:
+ * T'[l, s', a*K + k, r] = sum_s E_k[s', s] T[l, s, a, r] (old inner index major, the
+ * tn.flatten_edges order of Circuit.py:174-178). A plain gate is K = 1.
+ * kraus: host array, per site K[i] matrices E_k[s'][s] (complex64), concatenated.
+ * out[i].data must hold l*2*(a*K[i])*r elements and must not alias in[i].data.
+ */
+mpdo_status mpdo_apply_1q(int nsites, const mpdo_site* in, const mpdo_site* out,
+                          const float* kraus, const int* K, void* stream);
+
+/*
+ * Two-qubit gate (+ Kraus channel) on adjacent sites (min, max) and the SVD split.
+ * Replaces Circuit.py:74-136 (TensorCircuit._apply_two_qubits_gate) with tn.split_node's
+ * U*sqrt(S) | sqrt(S)*V^H convention: theta[(l, s_min, a_min), (s_max, a_max*K + k, r)] =
+ * sum G[s_min', s_max', s_min, s_max, k] T_min T_max, split keeping the singular values of
+ * decompositions.svd's rule with absolute error `eps_abs` (the reference passes
+ * GLOBAL_MINIMUM = 2.718281828459045e-8, Circuit.py:22,123) and no count cap.
+ * gate: host (2,2,2,2,K) complex64 ordered (o_min, o_max, i_min, i_max, k).
+ * out_min->data / out_max->data: capacity mpdo_apply_2q_capacity(...) for the new bond.
+ * On return out_min = (l, a_min, k) and out_max = (k, a_max*K, r); *discarded = sum s_rest^2.
+ * Synchronises `stream` once (the kept rank is data dependent).
+ */
+int mpdo_apply_2q_capacity(const mpdo_site* tmin, const mpdo_site* tmax, int K);
+mpdo_status mpdo_apply_2q(const mpdo_site* tmin, const mpdo_site* tmax, const float* gate, int K,
+                          double eps_abs, mpdo_site* out_min, mpdo_site* out_max, double* discarded,
+                          void* stream);
+
+/*
+ * Left-to-right QR sweep. Replaces TNNOptimizer.py:87-108 (qr_left2right) with
+ * decompositions.py:360-406 (qr): for i = 0..n-2, T_i as [(l s a) x r] = Q R with
+ * min(rows, cols) columns, T_i <- Q, T_{i+1} <- R T_{i+1}. mpdo_qr_left2right_dims fills the
+ * output dims (deterministic); out[i].data must hold them.
+ */
+void mpdo_qr_left2right_dims(int n, const mpdo_site* in, mpdo_site* out);
+mpdo_status mpdo_qr_left2right(int n, const mpdo_site* in, mpdo_site* out, void* stream);
+
+/*
+ * Truncating SVD sweep with the sqrt(S) split.
+ * direction 0 replaces TNNOptimizer.py:111-134 (svd_right2left), direction 1 replaces
+ * TNNOptimizer.py:137-161 (svd_left2right); keep rule decompositions.py:328-345 with
+ * relative=True; max_singular_values <= 0 means None, max_truncation_err < 0 means None.
+ * left_canonical != 0 asserts that sites 0..n-2 are left-orthonormal (true right after
+ * mpdo_qr_left2right, the bondTruncate pairing of TNNOptimizer.py:72-84), which lets each step
+ * skip the factorisation of the left tensor. mpdo_svd_sweep_dims gives capacity dims.
+ * On return out[i] holds actual dims; discarded[j] (n-1 entries, bond order of the sweep)
+ * receives sum s_rest^2 of each step. Synchronises once per step.
+ */
+void mpdo_svd_sweep_dims(int n, const mpdo_site* in, int direction, int max_singular_values, mpdo_site* out);
+mpdo_status mpdo_svd_sweep(int n, const mpdo_site* in, mpdo_site* out, int direction,
+                           int max_singular_values, double max_truncation_err, int left_canonical,
+                           double* discarded, void* stream);
+
+/*
+ * Inner-index (kappa) truncation of independent sites. Replaces the loop body of
+ * TNNOptimizer.py:164-197 (svdKappa_left2right): SVD of T as [(l s r) x a], keep
+ * min(kappa, k_err) (relative=True), T <- U*S. out[i].data capacity: l*2*min(kappa, a)*r
+ * (or a when kappa <= 0). discarded[i] = sum s_rest^2. Synchronises once.
+ */
+mpdo_status mpdo_svd_kappa(int nsites, const mpdo_site* in, mpdo_site* out, int max_singular_values,
+                           double max_truncation_err, double* discarded, void* stream);
+
+/*
+ * Unnormalised Re Tr(O rho~) for products of single-site operators, rho~ = sum_a T_a T_a^H.
+ * Replaces dmOperations.py:174-201 (pauli_expect) and :40-48 (trace_rho).
+ * codes: nstrings x n, per site -1 = identity (traced), 0 = X, 1 = Y, 2 = Z (PAULI_DICT,
+ * dmOperations.py:11-15), 3 = |0><0|, 4 = |1><1| (projectors for bitstring probabilities).
+ * out: host, nstrings doubles. Synchronises once.
+ */
+mpdo_status mpdo_pauli_expect(int n, const mpdo_site* sites, int nstrings, const int* codes, double* out,
+                              void* stream);
+
+/*
+ * Reduced density matrix of the qubits with keep[i] != 0 (others traced), rows = ket bits of
+ * the kept qubits in increasing index order, lowest index most significant, unnormalised.
+ * Replaces Circuit.py:226-262 (_create_dmNodes/_contract_dm) + dmOperations.py:18-37.
+ * out: device complex64 (2^k x 2^k), k = number of kept qubits (k <= 12).
+ */
+mpdo_status mpdo_reduced_dm(int n, const mpdo_site* sites, const int* keep, void* out, void* stream);
+
+/*
+ * Re Tr(rho0 rho1) of two MPDOs over the same qubits. Replaces dmOperations.py:51-66
+ * (trace_rho_rho). out: host double. Synchronises once.
+ */
+mpdo_status mpdo_trace_rho_rho(int n, const mpdo_site* a, const mpdo_site* b, double* out, void* stream);
+
+/*
+ * State vector of a pure MPDO (every inner dimension 1), amplitudes in ket-bit order with the
+ * lowest qubit index most significant. Replaces Circuit.py:264-272 (cal_vector).
+ * out: device complex64, 2^n entries (n <= 30).
+ */
+mpdo_status mpdo_contract_state(int n, const mpdo_site* sites, void* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPDO_H */

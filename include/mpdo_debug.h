@@ -1,0 +1,33 @@
+/*
+ * Kernel-level test hooks of libmpdo_gfx950. They have no counterpart in the reference: they
+ * expose the building blocks behind include/mpdo.h (the strided complex GEMM on the matrix
+ * cores and the batched fp64 Jacobi eigensolver) so that tests can check them in isolation.
+ */
+#ifndef MPDO_DEBUG_H
+#define MPDO_DEBUG_H
+
+#include "mpdo.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* C[b] = alpha op(A[b]) op(B[b]) + beta C[b]; strides in complex elements.
+ * prec: 0 = complex64 in / fp32 accumulate / complex64 out, 1 = complex64 in / fp64 / complex128 out,
+ *       2 = complex128 in / fp64 / complex128 out, 3 = complex128 in / fp64 / complex64 out,
+ *       4 = complex64 in / fp64 / complex64 out. */
+mpdo_status mpdo_debug_gemm(int prec, int M, int N, int K, int batch,
+                            const void* A, long long rsA, long long csA, long long bsA, int conjA,
+                            const void* B, long long rsB, long long csB, long long bsB, int conjB,
+                            void* C, long long rsC, long long csC, long long bsC,
+                            double alpha_re, double alpha_im, double beta_re, double beta_im, void* stream);
+
+/* Eigen-decomposition of `batch` Hermitian complex128 n x n matrices (row-major, contiguous,
+ * destroyed): w[b][n] descending, V[b][n][n] with eigenvectors in columns. *sweeps = sweeps used. */
+mpdo_status mpdo_debug_eigh(int n, int batch, void* A, double* w, void* V, int* sweeps, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPDO_DEBUG_H */

@@ -1,0 +1,428 @@
+// Batched Hermitian eigensolver in fp64 for the Gram matrices behind every SVD of the path.
+//
+// n <= 64: one workgroup per matrix holds A and V in LDS (2 x 64x65 complex128 = 130 KiB of
+//   the 160 KiB) and runs parallel cyclic Jacobi (round-robin pairing, n/2 disjoint complex
+//   rotations per round) until a sweep performs no rotation.
+// n > 64: two-sided block Jacobi on the zero-padded N x N matrix (N a multiple of 64) with
+//   32-wide blocks: each round pairs the blocks round-robin, one workgroup per block pair
+//   diagonalises its 64x64 principal submatrix in LDS (one inner sweep) and emits the 64x64
+//   unitary W; A <- J^H A J and V <- V J are then applied tile by tile.
+// Rotations skip |a_pq| <= 1e-15 sqrt(a_pp a_qq) (relative criterion; for the positive
+// semidefinite Gram matrices this yields eigenvalues with small relative error).
+#include "mpdo_internal.h"
+
+#include <vector>
+
+namespace mpdo {
+
+namespace {
+
+constexpr int NS = 64;       // LDS matrix order
+constexpr int LD = NS + 1;   // padded LDS leading dimension
+constexpr double REL_TOL = 1e-15;
+// Off-diagonal entries below ABS_TOL * max_i |a_ii| of the whole matrix are left alone: the
+// rotations themselves leave fp64 rounding of order n * eps * max|a_ii| behind, so a smaller
+// floor would keep a rank-deficient null space churning forever. Eigenvalues are resolved to
+// 1e-14 of the largest, i.e. singular values to 1e-7 of the largest - the rounding level of
+// the complex64 data the Gram matrices come from.
+constexpr double ABS_TOL = 1e-14;
+
+struct Rot { double cs, sn, wr, wi; int on; };
+
+__device__ inline void pair_of(int N, int r, int i, int& p, int& q) {
+  if (i == 0) { p = N - 1; q = r; }
+  else { p = (r + i) % (N - 1); q = (r - i + (N - 1)) % (N - 1); }
+}
+
+// One parallel Jacobi round on the LDS matrix A (order N, even) accumulating into V.
+// Returns through `cnt` (LDS) the number of rotations applied.
+__device__ void jacobi_round(double2 (*A)[LD], double2 (*V)[LD], Rot* rot, int N, int r,
+                             double floor_abs, int* cnt) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int np = N / 2;
+  if (tid < np) {
+    int p, q;
+    pair_of(N, r, tid, p, q);
+    if (p > q) { int t = p; p = q; q = t; }
+    double a = A[p][p].x, b = A[q][q].x;
+    double2 c = A[p][q];
+    double ac = sqrt(c.x * c.x + c.y * c.y);
+    Rot R; R.on = 0; R.cs = 1; R.sn = 0; R.wr = 1; R.wi = 0;
+    if (ac > floor_abs && ac > REL_TOL * sqrt(fabs(a * b))) {
+      double zeta = (b - a) / (2.0 * ac);
+      double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+      R.cs = 1.0 / sqrt(1.0 + t * t);
+      R.sn = t * R.cs;
+      R.wr = c.x / ac; R.wi = c.y / ac;
+      R.on = 1;
+      atomicAdd(cnt, 1);
+    }
+    rot[tid] = R;
+  }
+  __syncthreads();
+  // columns: A <- A J, V <- V J with J = [[cs, sn], [-sn conj(w), cs conj(w)]]
+  for (int task = tid; task < np * N; task += nt) {
+    int i = task / N, k = task % N;
+    Rot R = rot[i];
+    if (!R.on) continue;
+    int p, q;
+    pair_of(N, r, i, p, q);
+    if (p > q) { int t = p; p = q; q = t; }
+    // conj(w) * x
+    double2 x = A[k][p], y = A[k][q];
+    double2 wy = make_double2(R.wr * y.x + R.wi * y.y, R.wr * y.y - R.wi * y.x);
+    A[k][p] = make_double2(R.cs * x.x - R.sn * wy.x, R.cs * x.y - R.sn * wy.y);
+    A[k][q] = make_double2(R.sn * x.x + R.cs * wy.x, R.sn * x.y + R.cs * wy.y);
+    x = V[k][p]; y = V[k][q];
+    wy = make_double2(R.wr * y.x + R.wi * y.y, R.wr * y.y - R.wi * y.x);
+    V[k][p] = make_double2(R.cs * x.x - R.sn * wy.x, R.cs * x.y - R.sn * wy.y);
+    V[k][q] = make_double2(R.sn * x.x + R.cs * wy.x, R.sn * x.y + R.cs * wy.y);
+  }
+  __syncthreads();
+  // rows: A <- J^H A with J^H = [[cs, -sn w], [sn, cs w]]
+  for (int task = tid; task < np * N; task += nt) {
+    int i = task / N, k = task % N;
+    Rot R = rot[i];
+    if (!R.on) continue;
+    int p, q;
+    pair_of(N, r, i, p, q);
+    if (p > q) { int t = p; p = q; q = t; }
+    double2 x = A[p][k], y = A[q][k];
+    double2 wy = make_double2(R.wr * y.x - R.wi * y.y, R.wr * y.y + R.wi * y.x);
+    A[p][k] = make_double2(R.cs * x.x - R.sn * wy.x, R.cs * x.y - R.sn * wy.y);
+    A[q][k] = make_double2(R.sn * x.x + R.cs * wy.x, R.sn * x.y + R.cs * wy.y);
+  }
+  __syncthreads();
+  if (tid < np) {
+    Rot R = rot[tid];
+    if (R.on) {
+      int p, q;
+      pair_of(N, r, tid, p, q);
+      if (p > q) { int t = p; p = q; q = t; }
+      A[p][q] = make_double2(0, 0);
+      A[q][p] = make_double2(0, 0);
+      A[p][p].y = 0; A[q][q].y = 0;
+    }
+  }
+  __syncthreads();
+}
+
+__device__ double max_diag(double2 (*A)[LD], int N, double* red) {
+  const int tid = threadIdx.x;
+  double m = 0;
+  for (int i = tid; i < N; i += blockDim.x) m = fmax(m, fabs(A[i][i].x));
+  red[tid] = m;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (tid < s) red[tid] = fmax(red[tid], red[tid + s]);
+    __syncthreads();
+  }
+  double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// ---------------------------------------------------------------------------------------
+// n <= 64: whole matrix in LDS
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void eigh_small_kernel(int n, double2* Ag, long long lda, long long bsA,
+                                                         double* w, long long ldw, double2* Vg,
+                                                         long long ldv, long long bsV, int max_sweeps,
+                                                         int* sweeps_out) {
+  __shared__ double2 A[NS][LD];
+  __shared__ double2 V[NS][LD];
+  __shared__ Rot rot[NS / 2];
+  __shared__ double red[256];
+  __shared__ int cnt;
+  __shared__ int order[NS];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int N = (n + 1) & ~1;
+  double2* Ab = Ag + (long long)b * bsA;
+  for (int e = tid; e < N * N; e += blockDim.x) {
+    int i = e / N, j = e % N;
+    A[i][j] = (i < n && j < n) ? Ab[(long long)i * lda + j] : make_double2(0, 0);
+    V[i][j] = make_double2(i == j ? 1.0 : 0.0, 0);
+  }
+  __syncthreads();
+  double fl = max_diag(A, N, red) * ABS_TOL;
+  int sw = 0;
+  for (; sw < max_sweeps; ++sw) {
+    if (tid == 0) cnt = 0;
+    __syncthreads();
+    for (int r = 0; r < N - 1; ++r) jacobi_round(A, V, rot, N, r, fl, &cnt);
+    if (cnt == 0) break;
+    __syncthreads();
+  }
+  // sort descending (stable by index) over the n real positions: the zero padding never mixes
+  // with them, and ranking it could displace slightly negative null-space eigenvalues
+  for (int i = tid; i < n; i += blockDim.x) {
+    double li = A[i][i].x;
+    int rk = 0;
+    for (int j = 0; j < n; ++j) {
+      double lj = A[j][j].x;
+      rk += (lj > li) || (lj == li && j < i);
+    }
+    order[rk] = i;
+  }
+  __syncthreads();
+  for (int j = tid; j < n; j += blockDim.x) w[(long long)b * ldw + j] = A[order[j]][order[j]].x;
+  double2* Vb = Vg + (long long)b * bsV;
+  for (int e = tid; e < n * n; e += blockDim.x) {
+    int i = e / n, j = e % n;
+    Vb[(long long)i * ldv + j] = V[i][order[j]];
+  }
+  if (tid == 0 && sweeps_out) atomicMax(sweeps_out, sw + 1);
+}
+
+// ---------------------------------------------------------------------------------------
+// n > 64: block Jacobi on an N x N padded copy (N % 64 == 0), block width 32
+// ---------------------------------------------------------------------------------------
+struct BJ {
+  int n, N, batch;
+  double2* A;   // [batch][N][N]
+  double2* V;   // [batch][N][N]
+  double2* W;   // [batch][N/64][64][64]
+  int* flag;    // [batch]
+  double* floor_abs;   // [batch]: ABS_TOL * max diagonal of the whole matrix
+};
+
+__global__ __launch_bounds__(256) void bj_floor(BJ s) {
+  __shared__ double red[256];
+  const int b = blockIdx.x;
+  const double2* Ab = s.A + (long long)b * s.N * s.N;
+  double m = 0;
+  for (int i = threadIdx.x; i < s.N; i += blockDim.x) m = fmax(m, fabs(Ab[(long long)i * s.N + i].x));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + st]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) s.floor_abs[b] = red[0] * ABS_TOL;
+}
+
+__global__ void bj_init(BJ s, const double2* Ag, long long lda, long long bsA) {
+  long long tot = (long long)s.N * s.N;
+  int b = blockIdx.y;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot; e += (long long)gridDim.x * blockDim.x) {
+    int i = (int)(e / s.N), j = (int)(e % s.N);
+    double2 v = make_double2(0, 0);
+    if (i < s.n && j < s.n) v = Ag[(long long)b * bsA + (long long)i * lda + j];
+    s.A[(long long)b * tot + e] = v;
+    s.V[(long long)b * tot + e] = make_double2(i == j ? 1.0 : 0.0, 0);
+  }
+}
+
+__device__ inline void block_pair(int P, int r, int i, int& I, int& J) {
+  pair_of(P, r, i, I, J);
+  if (I > J) { int t = I; I = J; J = t; }
+}
+
+// one workgroup per (block pair, matrix): diagonalise the 64x64 principal submatrix
+__global__ __launch_bounds__(256) void bj_solve(BJ s, int r, int inner_sweeps) {
+  __shared__ double2 A[NS][LD];
+  __shared__ double2 V[NS][LD];
+  __shared__ Rot rot[NS / 2];
+  __shared__ double red[256];
+  __shared__ int cnt;
+  const int P = s.N / 32, pi = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+  int I, J;
+  block_pair(P, r, pi, I, J);
+  const double2* Ab = s.A + (long long)b * s.N * s.N;
+  for (int e = tid; e < NS * NS; e += blockDim.x) {
+    int i = e / NS, j = e % NS;
+    int gi = (i < 32 ? I * 32 + i : J * 32 + i - 32);
+    int gj = (j < 32 ? I * 32 + j : J * 32 + j - 32);
+    A[i][j] = Ab[(long long)gi * s.N + gj];
+    V[i][j] = make_double2(i == j ? 1.0 : 0.0, 0);
+  }
+  if (tid == 0) cnt = 0;
+  __syncthreads();
+  const double fl = s.floor_abs[b];
+  (void)red;
+  for (int sw = 0; sw < inner_sweeps; ++sw) {
+    for (int rr = 0; rr < NS - 1; ++rr) jacobi_round(A, V, rot, NS, rr, fl, &cnt);
+  }
+  double2* Wb = s.W + ((long long)b * (P / 2) + pi) * NS * NS;
+  for (int e = tid; e < NS * NS; e += blockDim.x) Wb[e] = V[e / NS][e % NS];
+  if (tid == 0 && cnt > 0) atomicOr(&s.flag[b], 1);
+}
+
+// X[rows, K u L] <- X[rows, K u L] * W_KL   (cols=1)   or
+// X[K u L, cols] <- W_KL^H * X[K u L, cols] (cols=0), in place, 64x64 tile per workgroup.
+// blockIdx.z selects the matrix: z < batch -> A[z], else V[z - batch] (columns only).
+__global__ __launch_bounds__(256) void bj_rotate(BJ s, int r, int cols) {
+  __shared__ double2 X[NS][LD];
+  __shared__ double2 W[NS][LD];
+  const int P = s.N / 32, pi = blockIdx.y, t0 = blockIdx.x * NS, tid = threadIdx.x;
+  int z = blockIdx.z;
+  bool isV = z >= s.batch;
+  int b = isV ? z - s.batch : z;
+  int I, J;
+  block_pair(P, r, pi, I, J);
+  double2* M = (isV ? s.V : s.A) + (long long)b * s.N * s.N;
+  const double2* Wb = s.W + ((long long)b * (P / 2) + pi) * NS * NS;
+  for (int e = tid; e < NS * NS; e += blockDim.x) {
+    int i = e / NS, j = e % NS;
+    W[i][j] = Wb[e];
+    if (cols) {
+      int gj = (j < 32 ? I * 32 + j : J * 32 + j - 32);
+      X[i][j] = M[(long long)(t0 + i) * s.N + gj];
+    } else {
+      int gi = (i < 32 ? I * 32 + i : J * 32 + i - 32);
+      X[i][j] = M[(long long)gi * s.N + t0 + j];
+    }
+  }
+  __syncthreads();
+  // each thread: 4x4 outputs
+  const int ti = (tid / 16) * 4, tj = (tid % 16) * 4;
+  double2 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[a][c] = make_double2(0, 0);
+  if (cols) {
+    // out[i][j] = sum_k X[i][k] W[k][j]
+    for (int k = 0; k < NS; ++k) {
+      double2 x[4], y[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) x[a] = X[ti + a][k];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) y[c] = W[k][tj + c];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          acc[a][c].x += x[a].x * y[c].x - x[a].y * y[c].y;
+          acc[a][c].y += x[a].x * y[c].y + x[a].y * y[c].x;
+        }
+    }
+  } else {
+    // out[i][j] = sum_k conj(W[k][i]) X[k][j]
+    for (int k = 0; k < NS; ++k) {
+      double2 x[4], y[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) x[a] = W[k][ti + a];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) y[c] = X[k][tj + c];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          acc[a][c].x += x[a].x * y[c].x + x[a].y * y[c].y;
+          acc[a][c].y += x[a].x * y[c].y - x[a].y * y[c].x;
+        }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      int i = ti + a, j = tj + c;
+      if (cols) {
+        int gj = (j < 32 ? I * 32 + j : J * 32 + j - 32);
+        M[(long long)(t0 + i) * s.N + gj] = acc[a][c];
+      } else {
+        int gi = (i < 32 ? I * 32 + i : J * 32 + i - 32);
+        M[(long long)gi * s.N + t0 + j] = acc[a][c];
+      }
+    }
+}
+
+// eigenvalues = diag(A), sorted descending (stable), eigenvectors = matching columns of V
+__global__ __launch_bounds__(256) void bj_finish(BJ s, double* w, long long ldw, double2* Vg, long long ldv,
+                                                 long long bsV, int* order_ws) {
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const double2* Ab = s.A + (long long)b * s.N * s.N;
+  const double2* Vb = s.V + (long long)b * s.N * s.N;
+  int* order = order_ws + (long long)b * s.N;
+  for (int i = tid; i < s.n; i += blockDim.x) {
+    double li = Ab[(long long)i * s.N + i].x;
+    int rk = 0;
+    for (int j = 0; j < s.n; ++j) {
+      double lj = Ab[(long long)j * s.N + j].x;
+      rk += (lj > li) || (lj == li && j < i);
+    }
+    order[rk] = i;
+  }
+  __syncthreads();
+  for (int j = tid; j < s.n; j += blockDim.x) {
+    int o = order[j];
+    w[(long long)b * ldw + j] = Ab[(long long)o * s.N + o].x;
+  }
+  double2* Vo = Vg + (long long)b * bsV;
+  for (long long e = tid; e < (long long)s.n * s.n; e += blockDim.x) {
+    int i = (int)(e / s.n), j = (int)(e % s.n);
+    Vo[(long long)i * ldv + j] = Vb[(long long)i * s.N + order[j]];
+  }
+}
+
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+}  // namespace
+
+size_t eigh_workspace_bytes(int n, int batch) {
+  if (n <= NS) return 256;
+  size_t N = (size_t)((n + 63) / 64) * 64;
+  size_t mats = align256(N * N * sizeof(double2) * batch);
+  size_t w = align256((N / 64) * NS * NS * sizeof(double2) * batch);
+  return 2 * mats + w + align256(sizeof(int) * batch) + align256(sizeof(double) * batch) +
+         align256(sizeof(int) * N * batch) + 256;
+}
+
+hipError_t eigh(int n, int batch, void* A, long long lda, long long bsA, double* w, long long ldw, void* V,
+                long long ldv, long long bsV, void* ws, hipStream_t st, int* sweeps_out) {
+  if (n <= 0 || batch <= 0) return hipSuccess;
+  const int max_sweeps = 60;
+  hipError_t err;
+  if (n <= NS) {
+    int* sw_dev = (int*)ws;
+    if ((err = hipMemsetAsync(sw_dev, 0, sizeof(int), st)) != hipSuccess) return err;
+    hipLaunchKernelGGL(eigh_small_kernel, dim3(batch), dim3(256), 0, st, n, (double2*)A, lda, bsA, w, ldw,
+                       (double2*)V, ldv, bsV, max_sweeps, sw_dev);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+    if (sweeps_out) {
+      if ((err = hipMemcpyAsync(sweeps_out, sw_dev, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess) return err;
+      if ((err = hipStreamSynchronize(st)) != hipSuccess) return err;
+    }
+    return hipSuccess;
+  }
+  BJ s;
+  s.n = n;
+  s.N = ((n + 63) / 64) * 64;
+  s.batch = batch;
+  char* p = (char*)ws;
+  size_t mats = align256((size_t)s.N * s.N * sizeof(double2) * batch);
+  s.A = (double2*)p; p += mats;
+  s.V = (double2*)p; p += mats;
+  s.W = (double2*)p; p += align256((size_t)(s.N / 64) * NS * NS * sizeof(double2) * batch);
+  s.flag = (int*)p; p += align256(sizeof(int) * batch);
+  s.floor_abs = (double*)p; p += align256(sizeof(double) * batch);
+  int* order = (int*)p;
+  const int P = s.N / 32;
+  hipLaunchKernelGGL(bj_init, dim3(256, batch), dim3(256), 0, st, s, (const double2*)A, lda, bsA);
+  hipLaunchKernelGGL(bj_floor, dim3(batch), dim3(256), 0, st, s);
+  std::vector<int> hflag(batch);
+  int sw = 0;
+  for (; sw < max_sweeps; ++sw) {
+    if ((err = hipMemsetAsync(s.flag, 0, sizeof(int) * batch, st)) != hipSuccess) return err;
+    for (int r = 0; r < P - 1; ++r) {
+      hipLaunchKernelGGL(bj_solve, dim3(P / 2, batch), dim3(256), 0, st, s, r, 1);
+      hipLaunchKernelGGL(bj_rotate, dim3(s.N / NS, P / 2, 2 * batch), dim3(256), 0, st, s, r, 1);
+      hipLaunchKernelGGL(bj_rotate, dim3(s.N / NS, P / 2, batch), dim3(256), 0, st, s, r, 0);
+    }
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+    if ((err = hipMemcpyAsync(hflag.data(), s.flag, sizeof(int) * batch, hipMemcpyDeviceToHost, st)) != hipSuccess)
+      return err;
+    if ((err = hipStreamSynchronize(st)) != hipSuccess) return err;
+    int any = 0;
+    for (int b = 0; b < batch; ++b) any |= hflag[b];
+    if (!any) break;
+  }
+  hipLaunchKernelGGL(bj_finish, dim3(batch), dim3(256), 0, st, s, w, ldw, (double2*)V, ldv, bsV, order);
+  if (sweeps_out) *sweeps_out = sw + 1;
+  return hipGetLastError();
+}
+
+}  // namespace mpdo

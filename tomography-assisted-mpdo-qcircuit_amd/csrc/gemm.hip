@@ -1,0 +1,216 @@
+// Strided batched complex GEMM on CDNA4 matrix cores (gfx950).
+//
+// Complex products are formed as four real MFMA products per k-step
+//   Cre += Are*Bre - Aim*Bim,  Cim += Are*Bim + Aim*Bre
+// with v_mfma_f32_16x16x4_f32 (complex64, fp32 accumulate) or v_mfma_f64_16x16x4_f64
+// (fp64 accumulate; complex64 inputs are widened on the way into LDS, which makes Gram
+// matrices of complex64 data exact up to fp64 rounding).
+//
+// Tiles: BM x BN outputs per 256-thread workgroup (4 waves in a 2x2 grid), BK complex
+// elements of the reduction per LDS stage, the next stage prefetched into registers while
+// the current one feeds the matrix cores. Operands are read along whichever of their two
+// dimensions is unit-stride, so transposed and conjugated views need no copies.
+#include "mpdo_internal.h"
+
+namespace mpdo {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Acc;
+// Output layout of the 16x16x4 MFMAs (A[i][k] in lane i + 16k, B[k][j] in lane j + 16k):
+// f32: D[i][j] in lane j + 16*(i/4), item i%4;  f64: D[i][j] in lane j + 16*(i%4), item i/4.
+template <> struct Acc<float> {
+  typedef floatx4 V;
+  typedef float2 C;
+  static __device__ inline int row(int lk, int q) { return lk * 4 + q; }
+  static __device__ inline V mfma(float a, float b, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct Acc<double> {
+  typedef doublex4 V;
+  typedef double2 C;
+  static __device__ inline int row(int lk, int q) { return lk + 4 * q; }
+  static __device__ inline V mfma(double a, double b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+
+template <typename TA, typename TB> __device__ inline TB cvt(TA v);
+template <> __device__ inline float2 cvt<float2, float2>(float2 v) { return v; }
+template <> __device__ inline double2 cvt<float2, double2>(float2 v) { return make_double2(v.x, v.y); }
+template <> __device__ inline double2 cvt<double2, double2>(double2 v) { return v; }
+template <> __device__ inline float2 cvt<double2, float2>(double2 v) { return make_float2((float)v.x, (float)v.y); }
+
+// TIn: operand storage type; TA: accumulation scalar; TOut: output storage type.
+template <typename TIn, typename TA, typename TOut, int BM, int BN, int BK>
+__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
+  typedef typename Acc<TA>::V V;
+  typedef typename Acc<TA>::C CA;   // complex in accumulation precision (LDS element)
+  constexpr int WM = BM / 2, WN = BN / 2;      // per-wave tile
+  constexpr int TM = WM / 16, TN = WN / 16;    // 16x16 MFMA tiles per wave
+  constexpr int PA = BM * BK / 256, PB = BK * BN / 256;
+  __shared__ CA As[BK][BM + 1];
+  __shared__ CA Bs[BK][BN + 1];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * WM, wn = (wave & 1) * WN;
+  const int b = blockIdx.z;
+  // tile order: blockIdx.x enumerates (m-tile, n-tile) with n fastest
+  const int ntn = (g.N + BN - 1) / BN;
+  const int m0 = (blockIdx.x / ntn) * BM, n0 = (blockIdx.x % ntn) * BN;
+  const TIn* A = (const TIn*)g.A + (long long)b * g.bsA;
+  const TIn* B = (const TIn*)g.B + (long long)b * g.bsB;
+  const bool aK = (g.csA == 1);   // A contiguous along k
+  const bool bK = (g.rsB == 1);   // B contiguous along k
+  const float sA = g.conjA ? -1.f : 1.f, sB = g.conjB ? -1.f : 1.f;
+
+  V accr[TM][TN], acci[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) { accr[i][j][q] = 0; acci[i][j][q] = 0; }
+
+  TIn ra[PA], rb[PB];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < PA; ++i) {
+      int e = aK ? tid * PA + i : i * 256 + tid;
+      int row = aK ? e / BK : e % BM;
+      int kk = aK ? e % BK : e / BM;
+      int gm = m0 + row, gk = k0 + kk;
+      TIn v;
+      v.x = 0; v.y = 0;
+      if (gm < g.M && gk < g.K) v = A[(long long)gm * g.rsA + (long long)gk * g.csA];
+      ra[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < PB; ++i) {
+      int e = bK ? tid * PB + i : i * 256 + tid;
+      int col = bK ? e / BK : e % BN;
+      int kk = bK ? e % BK : e / BN;
+      int gn = n0 + col, gk = k0 + kk;
+      TIn v;
+      v.x = 0; v.y = 0;
+      if (gn < g.N && gk < g.K) v = B[(long long)gk * g.rsB + (long long)gn * g.csB];
+      rb[i] = v;
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int i = 0; i < PA; ++i) {
+      int e = aK ? tid * PA + i : i * 256 + tid;
+      int row = aK ? e / BK : e % BM;
+      int kk = aK ? e % BK : e / BM;
+      CA v = cvt<TIn, CA>(ra[i]);
+      v.y *= sA;
+      As[kk][row] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < PB; ++i) {
+      int e = bK ? tid * PB + i : i * 256 + tid;
+      int col = bK ? e / BK : e % BN;
+      int kk = bK ? e % BK : e / BN;
+      CA v = cvt<TIn, CA>(rb[i]);
+      v.y *= sB;
+      Bs[kk][col] = v;
+    }
+  };
+
+  const int lr = lane & 15, lk = lane >> 4;
+  load(0);
+  for (int k0 = 0; k0 < g.K; k0 += BK) {
+    __syncthreads();
+    store();
+    __syncthreads();
+    if (k0 + BK < g.K) load(k0 + BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      CA a[TM], bb[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) a[i] = As[kk + lk][wm + i * 16 + lr];
+#pragma unroll
+      for (int j = 0; j < TN; ++j) bb[j] = Bs[kk + lk][wn + j * 16 + lr];
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          accr[i][j] = Acc<TA>::mfma(a[i].x, bb[j].x, accr[i][j]);
+          accr[i][j] = Acc<TA>::mfma(-a[i].y, bb[j].y, accr[i][j]);
+          acci[i][j] = Acc<TA>::mfma(a[i].x, bb[j].y, acci[i][j]);
+          acci[i][j] = Acc<TA>::mfma(a[i].y, bb[j].x, acci[i][j]);
+        }
+    }
+  }
+
+  TOut* C = (TOut*)g.C + (long long)b * g.bsC;
+  const bool beta0 = (g.beta_re == 0.0 && g.beta_im == 0.0);
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        int row = m0 + wm + i * 16 + Acc<TA>::row(lk, q);
+        int col = n0 + wn + j * 16 + lr;
+        if (row < g.M && col < g.N) {
+          double xr = accr[i][j][q], xi = acci[i][j][q];
+          double yr = g.alpha_re * xr - g.alpha_im * xi;
+          double yi = g.alpha_re * xi + g.alpha_im * xr;
+          long long off = (long long)row * g.rsC + (long long)col * g.csC;
+          if (!beta0) {
+            TOut o = C[off];
+            yr += g.beta_re * o.x - g.beta_im * o.y;
+            yi += g.beta_re * o.y + g.beta_im * o.x;
+          }
+          TOut o;
+          o.x = yr;
+          o.y = yi;
+          C[off] = o;
+        }
+      }
+}
+
+template <typename TIn, typename TA, typename TOut, int BM, int BN, int BK>
+static hipError_t launch(const GemmArgs& g, hipStream_t s) {
+  if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return hipSuccess;
+  int tiles = ((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
+  dim3 grid(tiles, 1, g.batch);
+  hipLaunchKernelGGL((gemm_kernel<TIn, TA, TOut, BM, BN, BK>), grid, dim3(256), 0, s, g);
+  return hipGetLastError();
+}
+
+hipError_t gemm(Prec p, const GemmArgs& g, hipStream_t s) {
+  if (g.K <= 0) {
+    // C = beta*C (degenerate reduction): handled by a K=1 zero-operand pass is not needed
+    // by any caller; reject explicitly.
+    return hipErrorInvalidValue;
+  }
+  if (g.batch > 65535) return hipErrorInvalidValue;
+  switch (p) {
+    case C64_F32ACC: return launch<float2, float, float2, 128, 128, 16>(g, s);
+    case C64_F64ACC: return launch<float2, double, double2, 64, 64, 16>(g, s);
+    case C128_F64ACC: return launch<double2, double, double2, 64, 64, 16>(g, s);
+    case C128_IN_C64_OUT: return launch<double2, double, float2, 64, 64, 16>(g, s);
+    case C64_F64ACC_C64OUT: return launch<float2, double, float2, 64, 64, 16>(g, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+GemmArgs gemm_rowmajor(int M, int N, int K,
+                       const void* A, long long lda, bool transA, bool conjA,
+                       const void* B, long long ldb, bool transB, bool conjB,
+                       void* C, long long ldc) {
+  GemmArgs g{};
+  g.M = M; g.N = N; g.K = K; g.batch = 1;
+  g.A = A; g.rsA = transA ? 1 : lda; g.csA = transA ? lda : 1; g.bsA = 0; g.conjA = conjA;
+  g.B = B; g.rsB = transB ? 1 : ldb; g.csB = transB ? ldb : 1; g.bsB = 0; g.conjB = conjB;
+  g.C = C; g.rsC = ldc; g.csC = 1; g.bsC = 0;
+  g.alpha_re = 1; g.alpha_im = 0; g.beta_re = 0; g.beta_im = 0;
+  return g;
+}
+
+}  // namespace mpdo

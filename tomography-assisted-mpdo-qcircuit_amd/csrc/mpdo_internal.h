@@ -1,0 +1,44 @@
+// Internal declarations shared by the gfx950 kernel files of libmpdo.
+// Every complex number is stored interleaved (re, im): float2 for complex64, double2 for
+// complex128. Site tensors use the fixed layout T[l][s=2][a][r] (r fastest).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mpdo {
+
+// General strided batched complex GEMM:  C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b]
+// with op(A)[i,k] = A[b*bsA + i*rsA + k*csA] (conjugated when conjA), similarly for B[k,j]
+// and C[i,j] = C[b*bsC + i*rsC + j*csC]. Strides are in complex elements.
+struct GemmArgs {
+  int M, N, K, batch;
+  const void* A; long long rsA, csA, bsA; int conjA;
+  const void* B; long long rsB, csB, bsB; int conjB;
+  void* C; long long rsC, csC, bsC;
+  double alpha_re, alpha_im, beta_re, beta_im;
+};
+
+enum Prec { C64_F32ACC = 0,   // complex64 in, fp32 MFMA accumulate, complex64 out
+            C64_F64ACC = 1,   // complex64 in, fp64 MFMA accumulate, complex128 out
+            C128_F64ACC = 2,  // complex128 in, fp64 MFMA accumulate, complex128 out
+            C128_IN_C64_OUT = 3,    // complex128 in, fp64 accumulate, complex64 out
+            C64_F64ACC_C64OUT = 4 };  // complex64 in, fp64 accumulate, complex64 out
+
+hipError_t gemm(Prec p, const GemmArgs& g, hipStream_t s);
+
+// Convenience: C = alpha*A*B (+beta C) with row-major operands of the given leading dims.
+GemmArgs gemm_rowmajor(int M, int N, int K,
+                       const void* A, long long lda, bool transA, bool conjA,
+                       const void* B, long long ldb, bool transB, bool conjB,
+                       void* C, long long ldc);
+
+// Batched Hermitian eigensolver (complex128): for each of `batch` matrices
+// A[b] (n x n, row-major, leading dim lda, batch stride bsA) computes eigenvalues in
+// descending order w[b*ldw + j] and eigenvectors V[b*bsV + i*ldv + j] (column j = vector j).
+// A is destroyed. `ws` must hold eigh_workspace_bytes(n, batch) bytes.
+size_t eigh_workspace_bytes(int n, int batch);
+hipError_t eigh(int n, int batch, void* A, long long lda, long long bsA,
+                double* w, long long ldw, void* V, long long ldv, long long bsV,
+                void* ws, hipStream_t s, int* sweeps_out);
+
+}  // namespace mpdo

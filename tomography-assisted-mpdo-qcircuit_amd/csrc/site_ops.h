@@ -1,0 +1,36 @@
+// Declarations of the memory-bound kernels (site_ops.hip).
+#pragma once
+#include "mpdo_internal.h"
+
+namespace mpdo {
+
+struct Site1q {
+  const float2* in;
+  float2* out;
+  int l, a, r, K;
+  long long koff;   // offset of this site's [K][2][2] Kraus block in the Kraus buffer
+};
+
+struct PermDesc {
+  int rank;
+  int odims[8];
+  long long istride[8];
+};
+
+hipError_t apply_1q(int nsites, const Site1q* d_sites, const float2* d_kraus, long long max_elems,
+                    hipStream_t s);
+hipError_t channel2q(const float2* Y, const float2* G, int K, int P1, int P2, float2* C, hipStream_t s);
+hipError_t permute(const float2* in, float2* out, int rank, const int* dims, const int* perm, hipStream_t s,
+                   int batch = 1, long long bstride = 0);
+hipError_t keep_count(const double* lam, long long ldw, int nev, int len_s, int msv, double err, int relative,
+                      int* k_out, double* disc_out, int batch, hipStream_t s);
+hipError_t factor(const double2* V, long long ldv, const double* lam, int rows, int nev, int k, double power,
+                  double floor_rel, int trans, float2* out, long long ldo, hipStream_t s);
+hipError_t factor64(const double2* V, long long ldv, const double* lam, int rows, int nev, int k, double power,
+                    double floor_rel, int trans, double2* out, long long ldo, hipStream_t s);
+hipError_t eye(float2* out, int rows, int cols, hipStream_t s);
+hipError_t phys_op(const float2* X, float2* out, int L, long long rest, const float2* P, hipStream_t s);
+
+hipError_t trace_prod(const float2* A, const float2* B, int n, double2* out, hipStream_t s);
+
+}  // namespace mpdo

@@ -1,0 +1,296 @@
+// Memory-bound kernels of the MPDO update path (gfx950): Kraus absorption into the inner index,
+// the two-qubit channel on the QR-reduced core, tensor permutations, the truncation rule of
+// decompositions.svd, and the conversion of fp64 eigenvectors into complex64 factors.
+#include "site_ops.h"
+
+namespace mpdo {
+
+// ---------------------------------------------------------------------------------------
+// 1q gate / Kraus absorption (Circuit.py:138-178):
+//   T'[l, s', a*K + k, r] = sum_s E_k[s', s] T[l, s, a, r]
+// One thread per (l, a, r) input column pair; writes 2K outputs, coalesced along r.
+// ---------------------------------------------------------------------------------------
+__global__ void apply_1q_kernel(const Site1q* sites, const float2* kraus) {
+  const Site1q S = sites[blockIdx.y];
+  const long long tot = (long long)S.l * S.a * S.r;
+  const float2* E = kraus + S.koff;          // [k][2][2]
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
+       e += (long long)gridDim.x * blockDim.x) {
+    int r = (int)(e % S.r);
+    long long t = e / S.r;
+    int a = (int)(t % S.a);
+    int l = (int)(t / S.a);
+    const float2* in = S.in + ((long long)l * 2 * S.a + a) * S.r + r;
+    float2 x0 = in[0], x1 = in[(long long)S.a * S.r];
+    const long long aK = (long long)S.a * S.K;
+    float2* out = S.out + ((long long)l * 2 * aK + (long long)a * S.K) * S.r + r;
+    for (int k = 0; k < S.K; ++k) {
+      float2 e00 = E[k * 4 + 0], e01 = E[k * 4 + 1], e10 = E[k * 4 + 2], e11 = E[k * 4 + 3];
+      float2 y0, y1;
+      y0.x = e00.x * x0.x - e00.y * x0.y + e01.x * x1.x - e01.y * x1.y;
+      y0.y = e00.x * x0.y + e00.y * x0.x + e01.x * x1.y + e01.y * x1.x;
+      y1.x = e10.x * x0.x - e10.y * x0.y + e11.x * x1.x - e11.y * x1.y;
+      y1.y = e10.x * x0.y + e10.y * x0.x + e11.x * x1.y + e11.y * x1.x;
+      out[(long long)k * S.r] = y0;
+      out[(aK + k) * (long long)S.r] = y1;
+    }
+  }
+}
+
+hipError_t apply_1q(int nsites, const Site1q* d_sites, const float2* d_kraus, long long max_elems,
+                    hipStream_t s) {
+  if (nsites <= 0) return hipSuccess;
+  int blocks = (int)((max_elems + 255) / 256);
+  if (blocks > 4096) blocks = 4096;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(apply_1q_kernel, dim3(blocks, nsites), dim3(256), 0, s, d_sites, d_kraus);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------
+// Two-qubit channel on the reduced core (Circuit.py:87-106):
+//   C[(p*2 + s1'), ((s2'*K + k)*P2 + q)] = sum_{s1,s2} G[s1', s2', s1, s2, k] Y[s1][s2][p][q]
+// ---------------------------------------------------------------------------------------
+__global__ void channel2q_kernel(const float2* Y, const float2* G, int K, int P1, int P2, float2* C) {
+  const long long tot = (long long)P1 * P2;
+  const long long ldc = 2LL * K * P2;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
+       e += (long long)gridDim.x * blockDim.x) {
+    int p = (int)(e / P2), q = (int)(e % P2);
+    float2 y[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) y[i] = Y[(long long)i * tot + e];
+    for (int k = 0; k < K; ++k) {
+#pragma unroll
+      for (int o = 0; o < 4; ++o) {   // o = s1'*2 + s2'
+        float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {   // i = s1*2 + s2
+          float2 g = G[((o * 4 + i) * K) + k];
+          acc.x += g.x * y[i].x - g.y * y[i].y;
+          acc.y += g.x * y[i].y + g.y * y[i].x;
+        }
+        int s1 = o >> 1, s2 = o & 1;
+        C[((long long)p * 2 + s1) * ldc + ((long long)s2 * K + k) * P2 + q] = acc;
+      }
+    }
+  }
+}
+
+hipError_t channel2q(const float2* Y, const float2* G, int K, int P1, int P2, float2* C, hipStream_t s) {
+  long long tot = (long long)P1 * P2;
+  int blocks = (int)((tot + 255) / 256);
+  if (blocks > 8192) blocks = 8192;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(channel2q_kernel, dim3(blocks), dim3(256), 0, s, Y, G, K, P1, P2, C);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------
+// Permutation of a row-major tensor of rank <= 6 (complex64):
+//   out[idx permuted by perm] = in[idx]; out dims are dims[perm[0]], dims[perm[1]], ...
+// ---------------------------------------------------------------------------------------
+__global__ void permute_kernel(const float2* in, float2* out, PermDesc d, long long tot) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
+       e += (long long)gridDim.x * blockDim.x) {
+    // decompose the OUTPUT index, gather from the input (coalesced writes)
+    long long t = e, src = 0;
+    for (int i = d.rank - 1; i >= 0; --i) {
+      int c = (int)(t % d.odims[i]);
+      t /= d.odims[i];
+      src += (long long)c * d.istride[i];
+    }
+    out[e] = in[src];
+  }
+}
+
+hipError_t permute(const float2* in, float2* out, int rank, const int* dims, const int* perm, hipStream_t s,
+                   int batch, long long bstride) {
+  PermDesc d;
+  d.rank = rank;
+  long long st[8];
+  st[rank - 1] = 1;
+  for (int i = rank - 2; i >= 0; --i) st[i] = st[i + 1] * dims[i + 1];
+  long long tot = 1;
+  for (int i = 0; i < rank; ++i) {
+    d.odims[i] = dims[perm[i]];
+    d.istride[i] = st[perm[i]];
+    tot *= dims[i];
+  }
+  int blocks = (int)((tot + 255) / 256);
+  if (blocks > 8192) blocks = 8192;
+  if (blocks < 1) blocks = 1;
+  for (int b = 0; b < batch; ++b)
+    hipLaunchKernelGGL(permute_kernel, dim3(blocks), dim3(256), 0, s, in + b * bstride, out + b * bstride, d, tot);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------
+// decompositions.svd keep-count (decompositions.py:328-345) on s = sqrt(max(lambda, 0)),
+// lambda the descending Gram eigenvalues; entries past `nev` are zero singular values.
+// Writes k and the discarded weight sum_{j >= k} s_j^2 (the reference computes and drops it).
+// ---------------------------------------------------------------------------------------
+__global__ void keep_count_kernel(const double* lam, long long ldw, int nev, int len_s, int msv,
+                                  double err, int relative, int* k_out, double* disc_out, int batch) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  const double* L = lam + (long long)b * ldw;
+  int n = len_s;
+  int maxk = (msv <= 0 || msv > n) ? n : msv;   // msv <= 0: no cap (None)
+  int kerr = maxk;
+  if (err >= 0 && n > 0) {
+    double tot = 0;
+    for (int j = 0; j < nev && j < n; ++j) tot += fmax(L[j], 0.0);
+    double te_last = sqrt(tot);
+    double s0 = sqrt(fmax(L[0], 0.0));
+    double bound = relative ? err * s0 : err;
+    double cum = 0;
+    for (int i = 0; i < n - 1; ++i) {
+      cum += (i < nev) ? fmax(L[i], 0.0) : 0.0;
+      if (te_last - sqrt(cum) <= bound) { kerr = i + 1; break; }
+    }
+  }
+  int k = kerr < maxk ? kerr : maxk;
+  double d = 0;
+  for (int j = k; j < nev && j < n; ++j) d += fmax(L[j], 0.0);
+  k_out[b] = k;
+  disc_out[b] = d;
+}
+
+hipError_t keep_count(const double* lam, long long ldw, int nev, int len_s, int msv, double err, int relative,
+                      int* k_out, double* disc_out, int batch, hipStream_t s) {
+  hipLaunchKernelGGL(keep_count_kernel, dim3((batch + 63) / 64), dim3(64), 0, s, lam, ldw, nev, len_s, msv, err,
+                     relative, k_out, disc_out, batch);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------
+// Factor from an eigendecomposition (V columns = eigenvectors, lambda descending):
+//   trans = 0:  out[i*ldo + j] = V[i][j] * f(lambda_j)           i < rows, j < k
+//   trans = 1:  out[j*ldo + i] = conj(V[i][j]) * f(lambda_j)
+// with f(lambda) = lambda^power when lambda > floor_rel * lambda_0 (else 0), and zero for
+// j >= nev (padded zero singular values).
+// ---------------------------------------------------------------------------------------
+template <typename TO>
+__global__ void factor_kernel(const double2* V, long long ldv, const double* lam, int rows, int nev, int k,
+                              double power, double floor_rel, int trans, TO* out, long long ldo) {
+  const long long tot = (long long)rows * k;
+  const double l0 = lam[0] > 0 ? lam[0] : 0.0;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
+       e += (long long)gridDim.x * blockDim.x) {
+    int i, j;
+    if (trans) { j = (int)(e / rows); i = (int)(e % rows); }
+    else { i = (int)(e / k); j = (int)(e % k); }
+    TO o;
+    o.x = 0; o.y = 0;
+    if (j < nev) {
+      double lj = lam[j];
+      if (lj > floor_rel * l0 && lj > 0) {
+        double f = (power == 0.0) ? 1.0 : pow(lj, power);
+        double2 v = V[(long long)i * ldv + j];
+        o.x = v.x * f;
+        o.y = (trans ? -v.y : v.y) * f;
+      }
+    }
+    if (trans) out[(long long)j * ldo + i] = o;
+    else out[(long long)i * ldo + j] = o;
+  }
+}
+
+hipError_t factor(const double2* V, long long ldv, const double* lam, int rows, int nev, int k, double power,
+                  double floor_rel, int trans, float2* out, long long ldo, hipStream_t s) {
+  long long tot = (long long)rows * k;
+  if (tot <= 0) return hipSuccess;
+  int blocks = (int)((tot + 255) / 256);
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(factor_kernel<float2>, dim3(blocks), dim3(256), 0, s, V, ldv, lam, rows, nev, k, power,
+                     floor_rel, trans, out, ldo);
+  return hipGetLastError();
+}
+
+hipError_t factor64(const double2* V, long long ldv, const double* lam, int rows, int nev, int k, double power,
+                    double floor_rel, int trans, double2* out, long long ldo, hipStream_t s) {
+  long long tot = (long long)rows * k;
+  if (tot <= 0) return hipSuccess;
+  int blocks = (int)((tot + 255) / 256);
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(factor_kernel<double2>, dim3(blocks), dim3(256), 0, s, V, ldv, lam, rows, nev, k, power,
+                     floor_rel, trans, out, ldo);
+  return hipGetLastError();
+}
+
+// identity into a (rows x cols) complex64 row-major matrix
+__global__ void eye_kernel(float2* out, int rows, int cols) {
+  long long tot = (long long)rows * cols;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
+       e += (long long)gridDim.x * blockDim.x) {
+    int i = (int)(e / cols), j = (int)(e % cols);
+    out[e] = make_float2(i == j ? 1.f : 0.f, 0.f);
+  }
+}
+
+hipError_t eye(float2* out, int rows, int cols, hipStream_t s) {
+  long long tot = (long long)rows * cols;
+  int blocks = (int)((tot + 255) / 256);
+  if (blocks > 8192) blocks = 8192;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(eye_kernel, dim3(blocks), dim3(256), 0, s, out, rows, cols);
+  return hipGetLastError();
+}
+
+// P (2x2) applied to the physical index of X[l][s][rest]: out[l][s'][rest] = sum_s P[s'][s] X[l][s][rest]
+__global__ void phys_op_kernel(const float2* X, float2* out, int L, long long rest, float2 p00, float2 p01,
+                               float2 p10, float2 p11) {
+  long long tot = (long long)L * rest;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
+       e += (long long)gridDim.x * blockDim.x) {
+    long long l = e / rest, c = e % rest;
+    float2 x0 = X[(l * 2) * rest + c], x1 = X[(l * 2 + 1) * rest + c];
+    float2 y0, y1;
+    y0.x = p00.x * x0.x - p00.y * x0.y + p01.x * x1.x - p01.y * x1.y;
+    y0.y = p00.x * x0.y + p00.y * x0.x + p01.x * x1.y + p01.y * x1.x;
+    y1.x = p10.x * x0.x - p10.y * x0.y + p11.x * x1.x - p11.y * x1.y;
+    y1.y = p10.x * x0.y + p10.y * x0.x + p11.x * x1.y + p11.y * x1.x;
+    out[(l * 2) * rest + c] = y0;
+    out[(l * 2 + 1) * rest + c] = y1;
+  }
+}
+
+hipError_t phys_op(const float2* X, float2* out, int L, long long rest, const float2* P, hipStream_t s) {
+  long long tot = (long long)L * rest;
+  int blocks = (int)((tot + 255) / 256);
+  if (blocks > 8192) blocks = 8192;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(phys_op_kernel, dim3(blocks), dim3(256), 0, s, X, out, L, rest, P[0], P[1], P[2], P[3]);
+  return hipGetLastError();
+}
+
+// sum_{i,j} A[i][j] B[j][i] (complex, fp64 accumulation), one workgroup
+__global__ __launch_bounds__(1024) void trace_prod_kernel(const float2* A, const float2* B, int n, double2* out) {
+  __shared__ double2 red[1024];
+  double2 acc = make_double2(0, 0);
+  const long long tot = (long long)n * n;
+  for (long long e = threadIdx.x; e < tot; e += blockDim.x) {
+    int i = (int)(e / n), j = (int)(e % n);
+    float2 x = A[e], y = B[(long long)j * n + i];
+    acc.x += (double)x.x * y.x - (double)x.y * y.y;
+    acc.y += (double)x.x * y.y + (double)x.y * y.x;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      red[threadIdx.x].x += red[threadIdx.x + st].x;
+      red[threadIdx.x].y += red[threadIdx.x + st].y;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+hipError_t trace_prod(const float2* A, const float2* B, int n, double2* out, hipStream_t s) {
+  hipLaunchKernelGGL(trace_prod_kernel, dim3(1), dim3(1024), 0, s, A, B, n, out);
+  return hipGetLastError();
+}
+
+}  // namespace mpdo

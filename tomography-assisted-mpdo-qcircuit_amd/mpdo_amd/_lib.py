@@ -1,0 +1,309 @@
+"""ctypes binding of libmpdo_gfx950.so (the C ABI in include/mpdo.h).
+
+This is the only route from the Python host to the device: there is no fallback. Loading fails
+loudly when the shared library is missing, and every compute call requires complex64 tensors
+that live on a ROCm device.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import List, Optional, Sequence, Tuple
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libmpdo_gfx950.so')
+
+MPDO_OK, MPDO_EINVAL, MPDO_ENOCONV, MPDO_ENOMEM, MPDO_EHIP = range(5)
+
+
+class Site(ctypes.Structure):
+    _fields_ = [('data', ctypes.c_void_p), ('l', ctypes.c_int), ('a', ctypes.c_int), ('r', ctypes.c_int)]
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load the HIP library (once). Raises ImportError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f'{LIB_PATH} is missing: build it with `python -m mpdo_amd._build` '
+                          '(hipcc --offload-arch=gfx950); there is no CPU fallback')
+    lib = ctypes.CDLL(LIB_PATH)
+    P, I, D, VP = ctypes.POINTER, ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+    SP = P(Site)
+    sig = {
+        'mpdo_version': (I, []),
+        'mpdo_last_error': (ctypes.c_char_p, []),
+        'mpdo_release_workspace': (None, []),
+        'mpdo_set_math_mode': (I, [I]),
+        'mpdo_apply_1q': (I, [I, SP, SP, P(ctypes.c_float), P(I), VP]),
+        'mpdo_apply_2q_capacity': (I, [SP, SP, I]),
+        'mpdo_apply_2q': (I, [SP, SP, P(ctypes.c_float), I, D, SP, SP, P(D), VP]),
+        'mpdo_qr_left2right_dims': (None, [I, SP, SP]),
+        'mpdo_qr_left2right': (I, [I, SP, SP, VP]),
+        'mpdo_svd_sweep_dims': (None, [I, SP, I, I, SP]),
+        'mpdo_svd_sweep': (I, [I, SP, SP, I, I, D, I, P(D), VP]),
+        'mpdo_svd_kappa': (I, [I, SP, SP, I, D, P(D), VP]),
+        'mpdo_pauli_expect': (I, [I, SP, I, P(I), P(D), VP]),
+        'mpdo_reduced_dm': (I, [I, SP, P(I), VP, VP]),
+        'mpdo_trace_rho_rho': (I, [I, SP, SP, P(D), VP]),
+        'mpdo_contract_state': (I, [I, SP, VP, VP]),
+        'mpdo_debug_gemm': (I, [I, I, I, I, I, VP, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, I,
+                                VP, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, I,
+                                VP, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong,
+                                D, D, D, D, VP]),
+        'mpdo_debug_eigh': (I, [I, I, VP, P(D), VP, P(I), VP]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+class MPDOError(RuntimeError):
+    pass
+
+
+def check(status: int, what: str) -> None:
+    if status == MPDO_OK:
+        return
+    msg = f'{what}: {load().mpdo_last_error().decode(errors="replace")}'
+    if status == MPDO_ENOCONV:
+        raise torch.linalg.LinAlgError(msg)
+    if status == MPDO_EINVAL:
+        raise ValueError(msg)
+    if status == MPDO_ENOMEM:
+        raise MemoryError(msg)
+    raise MPDOError(msg)
+
+
+def _stream(device: torch.device) -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _require_device(t: torch.Tensor) -> None:
+    if t.device.type != 'cuda':
+        raise RuntimeError('the MPDO kernels run on a ROCm GPU (gfx950); got a tensor on '
+                           f'{t.device} - there is no CPU path')
+    if t.dtype != torch.complex64:
+        raise TypeError(f'site tensors must be complex64, got {t.dtype}')
+
+
+def site_desc(t: torch.Tensor) -> Site:
+    """Descriptor of a (l, 2, a, r) complex64 device tensor (made contiguous by the caller)."""
+    _require_device(t)
+    if t.dim() != 4 or t.shape[1] != 2:
+        raise ValueError(f'site tensor must have shape (l, 2, a, r), got {tuple(t.shape)}')
+    if not t.is_contiguous():
+        raise ValueError('site tensor must be contiguous')
+    return Site(t.data_ptr(), t.shape[0], t.shape[2], t.shape[3])
+
+
+def _sites(ts: Sequence[torch.Tensor]):
+    arr = (Site * len(ts))()
+    for i, t in enumerate(ts):
+        arr[i] = site_desc(t)
+    return arr
+
+
+def _empty(n_elems: int, device) -> torch.Tensor:
+    return torch.empty(max(int(n_elems), 1), dtype=torch.complex64, device=device)
+
+
+def _view(buf: torch.Tensor, s: Site) -> torch.Tensor:
+    n = s.l * 2 * s.a * s.r
+    return buf[:n].view(s.l, 2, s.a, s.r)
+
+
+def set_math_mode(mode: str) -> str:
+    """'fp64acc' (default) or 'fp32acc': accumulation precision of the tensor contractions."""
+    modes = {'fp64acc': 0, 'fp32acc': 1}
+    old = load().mpdo_set_math_mode(modes[mode])
+    return {0: 'fp64acc', 1: 'fp32acc'}[old]
+
+
+# ---------------------------------------------------------------------------------------------
+def apply_1q(sites: Sequence[torch.Tensor], kraus: Sequence[torch.Tensor]) -> List[torch.Tensor]:
+    """Kraus/gate absorption (Circuit.py:138-178). kraus[i]: (2, 2) gate or (2, 2, K) tensor."""
+    lib = load()
+    n = len(sites)
+    if n == 0:
+        return []
+    dev = sites[0].device
+    mats, Ks = [], []
+    for E in kraus:
+        E = E.detach().to(torch.complex64).cpu()
+        if E.dim() == 2:
+            E = E.unsqueeze(-1)
+        if E.shape[:2] != (2, 2):
+            raise ValueError(f'Kraus tensor must be (2, 2[, K]), got {tuple(E.shape)}')
+        mats.append(E.permute(2, 0, 1).contiguous().view(torch.float32).reshape(-1))
+        Ks.append(E.shape[2])
+    outs = [torch.empty(t.shape[0], 2, t.shape[2] * K, t.shape[3], dtype=torch.complex64, device=dev)
+            for t, K in zip(sites, Ks)]
+    flat = torch.cat(mats).contiguous()
+    Karr = (ctypes.c_int * n)(*Ks)
+    st = lib.mpdo_apply_1q(n, _sites(sites), _sites(outs),
+                           ctypes.cast(flat.data_ptr(), ctypes.POINTER(ctypes.c_float)), Karr, _stream(dev))
+    check(st, 'mpdo_apply_1q')
+    return outs
+
+
+def apply_2q(tmin: torch.Tensor, tmax: torch.Tensor, gate: torch.Tensor, eps_abs: float):
+    """Two-qubit gate/channel + sqrt(S) split (Circuit.py:74-136). gate: (2,2,2,2[,K]) in
+    (o_min, o_max, i_min, i_max[, k]) order. Returns (new_min, new_max, discarded)."""
+    lib = load()
+    dev = tmin.device
+    G = gate.detach().to(torch.complex64).cpu()
+    if G.dim() == 4:
+        G = G.unsqueeze(-1)
+    if tuple(G.shape[:4]) != (2, 2, 2, 2):
+        raise ValueError(f'two-qubit gate must be (2,2,2,2[,K]), got {tuple(G.shape)}')
+    K = G.shape[4]
+    flat = G.contiguous().view(torch.float32).reshape(-1).contiguous()
+    a, b = site_desc(tmin), site_desc(tmax)
+    cap = lib.mpdo_apply_2q_capacity(ctypes.byref(a), ctypes.byref(b), K)
+    bmin = _empty(tmin.shape[0] * 2 * tmin.shape[2] * cap, dev)
+    bmax = _empty(cap * 2 * tmax.shape[2] * K * tmax.shape[3], dev)
+    om, oM = Site(bmin.data_ptr(), 0, 0, 0), Site(bmax.data_ptr(), 0, 0, 0)
+    disc = ctypes.c_double(0.0)
+    st = lib.mpdo_apply_2q(ctypes.byref(a), ctypes.byref(b),
+                           ctypes.cast(flat.data_ptr(), ctypes.POINTER(ctypes.c_float)), K, float(eps_abs),
+                           ctypes.byref(om), ctypes.byref(oM), ctypes.byref(disc), _stream(dev))
+    check(st, 'mpdo_apply_2q')
+    return _view(bmin, om).clone(), _view(bmax, oM).clone(), disc.value
+
+
+def qr_left2right(sites: Sequence[torch.Tensor]) -> List[torch.Tensor]:
+    lib = load()
+    n = len(sites)
+    dev = sites[0].device
+    ins = _sites(sites)
+    dims = (Site * n)()
+    lib.mpdo_qr_left2right_dims(n, ins, dims)
+    bufs = [_empty(d.l * 2 * d.a * d.r, dev) for d in dims]
+    outs = (Site * n)()
+    for i in range(n):
+        outs[i] = Site(bufs[i].data_ptr(), dims[i].l, dims[i].a, dims[i].r)
+    check(lib.mpdo_qr_left2right(n, ins, outs, _stream(dev)), 'mpdo_qr_left2right')
+    return [_view(bufs[i], outs[i]) for i in range(n)]
+
+
+def svd_sweep(sites: Sequence[torch.Tensor], direction: int, chi: Optional[int], err: Optional[float],
+              left_canonical: bool) -> Tuple[List[torch.Tensor], List[float]]:
+    lib = load()
+    n = len(sites)
+    dev = sites[0].device
+    ins = _sites(sites)
+    caps = (Site * n)()
+    msv = int(chi) if chi is not None else 0
+    lib.mpdo_svd_sweep_dims(n, ins, direction, msv, caps)
+    bufs = [_empty(c.l * 2 * c.a * c.r, dev) for c in caps]
+    outs = (Site * n)()
+    for i in range(n):
+        outs[i] = Site(bufs[i].data_ptr(), 0, 0, 0)
+    disc = (ctypes.c_double * max(n - 1, 1))()
+    st = lib.mpdo_svd_sweep(n, ins, outs, direction, msv, float(err) if err is not None else -1.0,
+                            1 if left_canonical else 0, disc, _stream(dev))
+    check(st, 'mpdo_svd_sweep')
+    res = [_view(bufs[i], outs[i]) for i in range(n)]
+    return res, [disc[i] for i in range(n - 1)]
+
+
+def svd_kappa(sites: Sequence[torch.Tensor], kappa: Optional[int], err: Optional[float]):
+    lib = load()
+    n = len(sites)
+    if n == 0:
+        return [], []
+    dev = sites[0].device
+    ins = _sites(sites)
+    bufs = []
+    outs = (Site * n)()
+    for i, t in enumerate(sites):
+        a = t.shape[2]
+        cap = min(kappa, a) if kappa is not None else a
+        bufs.append(_empty(t.shape[0] * 2 * cap * t.shape[3], dev))
+        outs[i] = Site(bufs[i].data_ptr(), 0, 0, 0)
+    disc = (ctypes.c_double * n)()
+    st = lib.mpdo_svd_kappa(n, ins, outs, int(kappa) if kappa is not None else 0,
+                            float(err) if err is not None else -1.0, disc, _stream(dev))
+    check(st, 'mpdo_svd_kappa')
+    return [_view(bufs[i], outs[i]) for i in range(n)], [disc[i] for i in range(n)]
+
+
+def pauli_expect(sites: Sequence[torch.Tensor], codes: Sequence[Sequence[int]]) -> List[float]:
+    lib = load()
+    n = len(sites)
+    ns = len(codes)
+    flat = (ctypes.c_int * (n * ns))()
+    for t, c in enumerate(codes):
+        if len(c) != n:
+            raise ValueError('operator code string must cover every qubit')
+        for i, v in enumerate(c):
+            flat[t * n + i] = int(v)
+    out = (ctypes.c_double * ns)()
+    check(lib.mpdo_pauli_expect(n, _sites(sites), ns, flat, out, _stream(sites[0].device)), 'mpdo_pauli_expect')
+    return [out[i] for i in range(ns)]
+
+
+def reduced_dm(sites: Sequence[torch.Tensor], keep: Sequence[bool]) -> torch.Tensor:
+    lib = load()
+    n = len(sites)
+    k = sum(1 for x in keep if x)
+    dev = sites[0].device
+    out = torch.empty(2 ** k, 2 ** k, dtype=torch.complex64, device=dev)
+    kf = (ctypes.c_int * n)(*[1 if x else 0 for x in keep])
+    check(lib.mpdo_reduced_dm(n, _sites(sites), kf, ctypes.c_void_p(out.data_ptr()), _stream(dev)),
+          'mpdo_reduced_dm')
+    return out
+
+
+def trace_rho_rho(a: Sequence[torch.Tensor], b: Sequence[torch.Tensor]) -> float:
+    lib = load()
+    out = ctypes.c_double(0.0)
+    check(lib.mpdo_trace_rho_rho(len(a), _sites(a), _sites(b), ctypes.byref(out), _stream(a[0].device)),
+          'mpdo_trace_rho_rho')
+    return out.value
+
+
+def contract_state(sites: Sequence[torch.Tensor]) -> torch.Tensor:
+    lib = load()
+    n = len(sites)
+    out = torch.empty(2 ** n, dtype=torch.complex64, device=sites[0].device)
+    check(lib.mpdo_contract_state(n, _sites(sites), ctypes.c_void_p(out.data_ptr()), _stream(sites[0].device)),
+          'mpdo_contract_state')
+    return out
+
+
+# --------------------------------------------------------------------------------- test hooks
+def debug_gemm(prec: int, A: torch.Tensor, B: torch.Tensor, C: torch.Tensor, M: int, N: int, K: int,
+               sa: Tuple[int, int, int], sb: Tuple[int, int, int], sc: Tuple[int, int, int],
+               conjA=False, conjB=False, alpha=1.0, beta=0.0, batch=1):
+    lib = load()
+    st = lib.mpdo_debug_gemm(prec, M, N, K, batch, ctypes.c_void_p(A.data_ptr()), sa[0], sa[1], sa[2], int(conjA),
+                             ctypes.c_void_p(B.data_ptr()), sb[0], sb[1], sb[2], int(conjB),
+                             ctypes.c_void_p(C.data_ptr()), sc[0], sc[1], sc[2],
+                             complex(alpha).real, complex(alpha).imag, complex(beta).real, complex(beta).imag,
+                             _stream(A.device))
+    check(st, 'mpdo_debug_gemm')
+
+
+def debug_eigh(A: torch.Tensor):
+    """A: (batch, n, n) complex128 Hermitian on the device (destroyed). Returns (w, V, sweeps)."""
+    lib = load()
+    b, n, _ = A.shape
+    w = torch.empty(b, n, dtype=torch.float64, device=A.device)
+    V = torch.empty(b, n, n, dtype=torch.complex128, device=A.device)
+    sw = ctypes.c_int(0)
+    check(lib.mpdo_debug_eigh(n, b, ctypes.c_void_p(A.data_ptr()),
+                              ctypes.cast(w.data_ptr(), ctypes.POINTER(ctypes.c_double)),
+                              ctypes.c_void_p(V.data_ptr()), ctypes.byref(sw), _stream(A.device)),
+          'mpdo_debug_eigh')
+    return w, V, sw.value

@@ -1,0 +1,159 @@
+"""Seeded synthetic circuits of the benchmark configurations.
+
+A circuit is a list of ops ``(name, qubits, params)`` plus ``('TRUNCATE', [], [])`` markers,
+replayed onto a ``TensorCircuit`` with ``apply_ops``. Angles come from
+``numpy.random.Generator(PCG64(seed))`` drawn in (layer, qubit, [theta, phi]) order.
+One circuit-layer = single-qubit sublayer (with noise) + two-qubit sublayer (with noise) +
+one truncation.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+Op = Tuple[str, List[int], List[float]]
+
+
+@dataclass
+class Workload:
+    name: str
+    n: int
+    ops: List[Op]
+    noise_type: str
+    ideal: bool
+    chi: Optional[int]
+    kappa: Optional[int]
+    chip: Optional[dict] = None
+    single_qubit_noise: str = 'decay_dephasing'
+    p1: Optional[float] = None
+    depth: int = 0
+    extra: Dict = field(default_factory=dict)
+
+    def circuit_kwargs(self) -> dict:
+        return dict(qn=self.n, ideal=self.ideal, noiseType=self.noise_type, chi=self.chi, kappa=self.kappa,
+                    chip=self.chip, single_qubit_noise=self.single_qubit_noise, p1=self.p1)
+
+
+def brickwork(n: int, depth: int, rng: np.random.Generator, two: Sequence[str] = ('CNOT',),
+              angle_lo: float = 0.0, angle_hi: float = 2 * math.pi, noisy_identity_after_2q: bool = False,
+              truncate_each_layer: bool = True, delta: Optional[np.ndarray] = None) -> List[Op]:
+    """RY(theta) RZ(phi) on every qubit, then a two-qubit gate on (0,1),(2,3),... in even
+    layers and (1,2),(3,4),... in odd layers (``two[layer % len(two)]``), then a truncation."""
+    angles = rng.uniform(angle_lo, angle_hi, size=(depth, n, 2))
+    if delta is not None:
+        angles = angles + delta
+    ops: List[Op] = []
+    for d in range(depth):
+        for q in range(n):
+            ops.append(('RY', [q], [float(angles[d, q, 0])]))
+            ops.append(('RZ', [q], [float(angles[d, q, 1])]))
+        g = two[d % len(two)]
+        for a in range(d % 2, n - 1, 2):
+            ops.append((g, [a, a + 1], []))
+            if noisy_identity_after_2q:
+                ops.append(('I', [a], []))
+                ops.append(('I', [a + 1], []))
+        if truncate_each_layer:
+            ops.append(('TRUNCATE', [], []))
+    return ops
+
+
+def config0(seed: int = 101) -> Workload:
+    """8 qubits, 4 layers, CNOT brickwork, dep1 p=1e-3 (1q), dep2 p=1e-2 (2q), chi=32, kappa=4."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return Workload('config0', 8, brickwork(8, 4, rng), 'idealNoise', False, 32, 4,
+                    chip=dict(dpc_error_rate=0.01), single_qubit_noise='depolarizing', p1=1e-3, depth=4)
+
+
+def config1(seed: int = 202, n: int = 20, depth: int = 20) -> Workload:
+    """20 qubits, depth 20, CNOT (even) / CZ (odd) brickwork, amplitude damping 0.005 plus
+    dephasing 0.005 after every gate (1q: the reference composite; 2q: noisy identities on
+    both qubits), chi=64, kappa=4."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ops = brickwork(n, depth, rng, two=('CNOT', 'CZ'), noisy_identity_after_2q=True)
+    return Workload('config1', n, ops, 'unified', False, 64, 4,
+                    chip=dict(decay_param=0.005, dephasing_param=0.005), depth=depth)
+
+
+def config2(seed: int = 303) -> Workload:
+    """16-qubit hardware-efficient ansatz: 8 layers RY, RZ (angles U[-pi, pi)) + CNOT chain,
+    depolarizing p=0.005 per gate, chi=64, kappa=4."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n, depth = 16, 8
+    angles = rng.uniform(-math.pi, math.pi, size=(depth, n, 2))
+    ops: List[Op] = []
+    for d in range(depth):
+        for q in range(n):
+            ops.append(('RY', [q], [float(angles[d, q, 0])]))
+            ops.append(('RZ', [q], [float(angles[d, q, 1])]))
+        for a in range(n - 1):
+            ops.append(('CNOT', [a, a + 1], []))
+        ops.append(('TRUNCATE', [], []))
+    return Workload('config2', n, ops, 'idealNoise', False, 64, 4, chip=dict(dpc_error_rate=0.005),
+                    single_qubit_noise='depolarizing', p1=0.005, depth=depth)
+
+
+def config3(seed: int = 404, count: int = 256) -> List[Workload]:
+    """Independent random brickwork circuits: n ~ U{8..24}, depth ~ U{4..16},
+    chi ~ {16,32,64,128}, kappa ~ {2,4}, depolarizing or amplitude damping (Bernoulli 1/2)
+    with p ~ U[1e-4, 1e-2]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for c in range(count):
+        n = int(rng.integers(8, 25))
+        depth = int(rng.integers(4, 17))
+        chi = int(rng.choice([16, 32, 64, 128]))
+        kappa = int(rng.choice([2, 4]))
+        dep = bool(rng.integers(0, 2))
+        p = float(rng.uniform(1e-4, 1e-2))
+        ops = brickwork(n, depth, rng, noisy_identity_after_2q=not dep)
+        if dep:
+            w = Workload(f'config3_{c}', n, ops, 'idealNoise', False, chi, kappa, chip=dict(dpc_error_rate=p),
+                         single_qubit_noise='depolarizing', p1=p, depth=depth)
+        else:
+            w = Workload(f'config3_{c}', n, ops, 'unified', False, chi, kappa,
+                         chip=dict(decay_param=p, dephasing_param=0.0), depth=depth)
+        out.append(w)
+    return out
+
+
+def config4(seed: int = 505, copies: int = 32, n: int = 32, depth: int = 30) -> List[Workload]:
+    """32 qubits, depth 30 RY/RZ + CNOT brickwork, dep1/dep2 p=0.002, chi=256, kappa=8; copies
+    with the same base angles plus delta ~ U[-0.01, 0.01]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    base_state = rng.bit_generator.state
+    out = []
+    for c in range(copies):
+        r = np.random.Generator(np.random.PCG64(seed))
+        r.bit_generator.state = base_state
+        delta = None
+        if c > 0:
+            dr = np.random.Generator(np.random.PCG64(seed * 1000 + c))
+            delta = dr.uniform(-0.01, 0.01, size=(depth, n, 2))
+        ops = brickwork(n, depth, r, delta=delta)
+        out.append(Workload(f'config4_{c}', n, ops, 'idealNoise', False, 256, 8, chip=dict(dpc_error_rate=0.002),
+                            single_qubit_noise='depolarizing', p1=0.002, depth=depth))
+    return out
+
+
+def apply_ops(circuit, ops: Sequence[Op]) -> None:
+    """Replay ops onto a TensorCircuit through its builder methods."""
+    for name, q, p in ops:
+        n = name.upper()
+        if n == 'TRUNCATE':
+            circuit.truncate()
+        elif n == 'BARRIER':
+            circuit.barrier()
+        elif n in ('RX', 'RY', 'RZ', 'P', 'U1'):
+            getattr(circuit, n.lower())(p[0], q)
+        elif n in ('I', 'H', 'X', 'Y', 'Z', 'S', 'SDG', 'T'):
+            getattr(circuit, n.lower())(q)
+        elif n in ('CNOT', 'CX', 'CZ', 'CY', 'SWAP', 'ISWAP', 'II'):
+            getattr(circuit, n.lower())(q[0], q[1])
+        elif n in ('CP', 'RXX', 'RYY', 'RZZ', 'PSWAP'):
+            getattr(circuit, n.lower())(p[0], q[0], q[1])
+        else:
+            raise ValueError(f'unknown op {name}')

@@ -136,6 +136,28 @@ mpdo_status mpdo_reduced_dm(int n, const mpdo_site* sites, const int* keep, void
 mpdo_status mpdo_trace_rho_rho(int n, const mpdo_site* a, const mpdo_site* b, double* out, void* stream);
 
 /*
+ * One brickwork circuit-layer for `batch` chains with identical shapes: the single-qubit
+ * channels of every site, the two-qubit channel of every pair (i, i+1), i = parity, parity+2, ...,
+ * then truncate() (QR sweep, chi sweep, kappa cut). Replaces the sequence
+ * Circuit.py:138-178 (per qubit) + Circuit.py:74-136 (per pair) + Circuit.py:476-481
+ * (TNNOptimizer.py:72-197) of one layer of the reference's evolve loop, with the same
+ * truncation decisions and output shapes as calling mpdo_apply_1q, mpdo_apply_2q,
+ * mpdo_qr_left2right, mpdo_svd_sweep(left_canonical=1) and mpdo_svd_kappa in that order (no
+ * error threshold: only count caps). The two-site tensors of the pairs are never materialised.
+ * in[i].data / out[i].data: complex64 [batch][l][2][a][r]; out dims from mpdo_layer_dims, which
+ * returns 0 (and sets the error text) when the layer does not qualify (then use the per-gate
+ * entry points). kraus1: host [batch][n][K1][2][2]; gates2: host [npairs][2][2][2][2][K2] in
+ * (o_min, o_max, i_min, i_max, k) order. discarded: host [batch][3n]: n chi-step slots by bond
+ * index, n kappa slots by site, n gate-time split slots by pair min site (the weight the
+ * reference's split of each pair drops, Circuit.py:120-124), or NULL. Synchronises once per pair
+ * (the kept count of its gate-time split) and at the end.
+ */
+int mpdo_layer_dims(int n, const mpdo_site* in, int K1, int parity, int K2, int chi, int kappa, mpdo_site* out);
+mpdo_status mpdo_layer_batched(int batch, int n, const mpdo_site* in, const float* kraus1, int K1, int parity,
+                               const float* gates2, int K2, int chi, int kappa, const mpdo_site* out,
+                               double* discarded, void* stream);
+
+/*
  * State vector of a pure MPDO (every inner dimension 1), amplitudes in ket-bit order with the
  * lowest qubit index most significant. Replaces Circuit.py:264-272 (cal_vector).
  * out: device complex64, 2^n entries (n <= 30).

@@ -26,6 +26,13 @@ mpdo_status mpdo_debug_gemm(int prec, int M, int N, int K, int batch,
  * destroyed): w[b][n] descending, V[b][n][n] with eigenvectors in columns. *sweeps = sweeps used. */
 mpdo_status mpdo_debug_eigh(int n, int batch, void* A, double* w, void* V, int* sweeps, void* stream);
 
+/* Host-only rehearsal of mpdo_layer_batched: plans the layer and walks every launch with fake
+ * device addresses, checking each operand's extent against the buffer it falls in, without
+ * touching a GPU. `in[i].data`/`out[i].data` may be any distinct non-overlapping addresses.
+ * *workspace_bytes receives the device workspace the layer needs. */
+mpdo_status mpdo_debug_layer_dry_run(int batch, int n, const mpdo_site* in, int K1, int parity, int K2, int chi,
+                                     int kappa, const mpdo_site* out, unsigned long long* workspace_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -21,6 +21,13 @@
 
 using namespace mpdo;
 
+namespace mpdo {
+// Accumulation precision of the complex64 tensor contractions (mpdo_set_math_mode).
+Prec g_tensor_prec = C64_F64ACC_C64OUT;
+mpdo_status layer_fail(mpdo_status s, const char* msg);
+mpdo_status layer_ws(size_t bytes, hipStream_t s, char** base);
+}  // namespace mpdo
+
 namespace {
 
 thread_local std::string g_err;
@@ -117,8 +124,6 @@ hipError_t gram_rows(const float2* X, int n, int cols, long long ldx, double2* G
   g.alpha_re = 1;
   return gemm(C64_F64ACC, g, s);
 }
-// Accumulation precision of the complex64 tensor contractions (mpdo_set_math_mode).
-Prec g_tensor_prec = C64_F64ACC_C64OUT;
 
 // C (M x N) = A (M x K) * B (K x N), all row-major complex64 with the given leading dims
 hipError_t mm(int M, int N, int K, const float2* A, long long lda, const float2* B, long long ldb, float2* C,
@@ -157,6 +162,11 @@ mpdo_status read_k(const int* dk, const double* dd, int* k, double* disc, hipStr
 }
 
 constexpr double QR_FLOOR = 1e-13;   // relative eigenvalue floor of the QR-type factorisations
+// Relative Gram-eigenvalue floor of the sqrt(S) splits: directions with s_j < 1e-6 s_0 are below
+// what complex64 data resolves (a float32 SVD returns them as noise of that size); their rows
+// S^{-1/2} U^H B would be rounding noise amplified by lambda^{-1/4}, so they are set to zero
+// (the kept count is unchanged: those singular values are zero-filled).
+constexpr double SPLIT_FLOOR = 1e-12;
 
 // ---------------------------------------------------------------- truncating SVD step
 // theta = A (rowsA x D) * B (D x colsB). Writes T_left = U sqrt(S) (rowsA x k) to outL and
@@ -203,14 +213,14 @@ mpdo_status chi_step(const float2* A, int rowsA, int D, const float2* B, int col
   int k = 0;
   ST(read_k(w.dk, w.dd, &k, disc, s));
   if (canonical) {
-    CK(factor(w.eZ.V, D, w.eZ.w, D, D, k, 0.25, 0.0, 0, w.FL, k, s));
-    CK(factor(w.eZ.V, D, w.eZ.w, D, D, k, -0.25, 0.0, 1, w.FR, D, s));
+    CK(factor(w.eZ.V, D, w.eZ.w, D, D, k, 0.25, SPLIT_FLOOR, 0, w.FL, k, s));
+    CK(factor(w.eZ.V, D, w.eZ.w, D, D, k, -0.25, SPLIT_FLOOR, 1, w.FR, D, s));
   } else {
     // FL = (W Lambda^{-1/2}) (U' S^{1/2});  FR = (S^{-1/2} U'^H) (Lambda^{1/2} W^H)
     CK(factor64(w.eA.V, D, w.eA.w, D, D, D, -0.5, QR_FLOOR, 0, w.P, D, s));
-    CK(factor64(w.eZ.V, D, w.eZ.w, D, D, k, 0.25, 0.0, 0, w.Q, k, s));
+    CK(factor64(w.eZ.V, D, w.eZ.w, D, D, k, 0.25, SPLIT_FLOOR, 0, w.Q, k, s));
     CK(mm64(C128_IN_C64_OUT, D, k, D, w.P, D, false, false, w.Q, k, w.FL, k, s));
-    CK(factor64(w.eZ.V, D, w.eZ.w, D, D, k, -0.25, 0.0, 1, w.Q, D, s));
+    CK(factor64(w.eZ.V, D, w.eZ.w, D, D, k, -0.25, SPLIT_FLOOR, 1, w.Q, D, s));
     CK(factor64(w.eA.V, D, w.eA.w, D, D, D, 0.5, QR_FLOOR, 1, w.P, D, s));
     CK(mm64(C128_IN_C64_OUT, k, D, D, w.Q, D, false, false, w.P, D, w.FR, D, s));
   }
@@ -223,6 +233,11 @@ mpdo_status chi_step(const float2* A, int rowsA, int D, const float2* B, int col
 bool bad_site(const mpdo_site& t) { return !t.data || t.l <= 0 || t.a <= 0 || t.r <= 0; }
 
 }  // namespace
+
+namespace mpdo {
+mpdo_status layer_fail(mpdo_status s, const char* msg) { return fail(s, "%s", msg); }
+mpdo_status layer_ws(size_t bytes, hipStream_t s, char** base) { return ws_reserve(bytes, s, base); }
+}  // namespace mpdo
 
 // ==========================================================================================
 extern "C" {
@@ -401,7 +416,7 @@ mpdo_status mpdo_apply_2q(const mpdo_site* tmin, const mpdo_site* tmax, const fl
   ST(read_k(dk, dd, &k, &disc, s));
   if (k > cap) return fail(MPDO_EINVAL, "internal: rank %d above capacity %d", k, cap);
   // T_min' = (Q1 (x) I) U sqrt(S)  -> (l, a1, s1', k) -> (l, s1', a1, k)
-  CK(factor(eC.V, nC, eC.w, nC, nC, k, 0.25, 0.0, 0, FL, k, s));
+  CK(factor(eC.V, nC, eC.w, nC, nC, k, 0.25, SPLIT_FLOOR, 0, FL, k, s));
   const float2* Zs = FL;
   if (!q1id) {
     CK(mm(rows1, 2 * k, p1, Q1, p1, FL, 2 * k, Z, 2 * k, s));
@@ -412,7 +427,7 @@ mpdo_status mpdo_apply_2q(const mpdo_site* tmin, const mpdo_site* tmax, const fl
     CK(permute(Zs, (float2*)out_min->data, 4, dims, perm, s));
   }
   // T_max' = S^{-1/2} U^H C (I (x) Q2) -> (k, s2', kk, a2, r) -> (k, s2', a2, kk, r)
-  CK(factor(eC.V, nC, eC.w, nC, nC, k, -0.25, 0.0, 1, FR, nC, s));
+  CK(factor(eC.V, nC, eC.w, nC, nC, k, -0.25, SPLIT_FLOOR, 1, FR, nC, s));
   CK(mm(k, (int)colsC, nC, FR, nC, C, colsC, Wm, colsC, s));
   const float2* Ts = Wm;
   if (!q2id) {

@@ -24,7 +24,10 @@ constexpr double REL_TOL = 1e-15;
 // rotations themselves leave fp64 rounding of order n * eps * max|a_ii| behind, so a smaller
 // floor would keep a rank-deficient null space churning forever. Eigenvalues are resolved to
 // 1e-14 of the largest, i.e. singular values to 1e-7 of the largest - the rounding level of
-// the complex64 data the Gram matrices come from.
+// the complex64 data the Gram matrices come from. The blocked solver (n > 64) applies each
+// 64x64 block rotation as 64-term fp64 sums, once per round, so its residual grows with the
+// number of block columns: its floor is ABS_TOL * N / 64 (1.6e-13 at N = 1024, singular
+// values still resolved to 4e-7 of the largest).
 constexpr double ABS_TOL = 1e-14;
 
 struct Rot { double cs, sn, wr, wi; int on; };
@@ -198,7 +201,7 @@ __global__ __launch_bounds__(256) void bj_floor(BJ s) {
     if (threadIdx.x < st) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + st]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) s.floor_abs[b] = red[0] * ABS_TOL;
+  if (threadIdx.x == 0) s.floor_abs[b] = red[0] * ABS_TOL * (s.N / NS);
 }
 
 __global__ void bj_init(BJ s, const double2* Ag, long long lda, long long bsA) {

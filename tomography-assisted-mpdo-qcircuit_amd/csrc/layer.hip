@@ -1,0 +1,1138 @@
+// One brickwork circuit-layer for a batch of MPDO chains with identical shapes, without ever
+// forming the (up to 4*chi)-bond tensor that the reference's gate-time split gives each pair.
+//
+// Reference schedule being reproduced (Circuit.py:74-178, :469-489; TNNOptimizer.py:72-197):
+//   1q channels on every site -> 2q channel + SVD split on each pair (i, i+1) -> truncate():
+//   QR sweep left to right, sqrt(S)-split SVD sweep right to left with the chi rule, kappa SVD.
+// Equivalent computation (exact in exact arithmetic; shapes identical to the per-gate path):
+//   The gate-time split of the reference (Circuit.py:110-124: SVD of the two-site tensor theta,
+//   absolute tail 2.7e-8, no chi cap) is reproduced in compressed form: T_min = Q1t R1t (QR),
+//   T_max = L2 Q2 (LQ), theta = (Q1t (x) I) C_theta (I (x) Q2) with C_theta = G(R1t (x)_m L2); the
+//   Gram of C_theta gives theta's singular values, the kept count m' and, when a direction is
+//   dropped, the projector P onto the kept left singular vectors. The split tensors themselves
+//   (bond up to 4 chi) are never formed: the QR sweep needs from the pair only
+//   X1r = (R (x) I) Q1t = Q1 Ra and R_max, a factor of the right-bond Gram of R theta', which is
+//   sum_a Q2_a^H (Cu^H Cu summed over the channel/physical index) Q2_a with the core
+//   Cu = (Ra (x) I) P C_theta. The chi step at the pair bond is the truncated SVD of
+//   Theta = R theta' R_max^{-1} P_right = (Q1 (x) I) C'' (I (x) Q3) (Q3 from the LQ of Q2 P_right),
+//   read off the Gram of the core C'' = Cu L3: U sqrt(S) gives the new min site, sqrt(S) V^H the
+//   new max site. The kappa cut of every site runs as soon as the site is final, so the large
+//   max-site tensor exists for one pair at a time.
+// Every stage is batched over the chains (same shapes; rank rules deterministic because no
+// error threshold is used).
+#include "mpdo.h"
+#include "mpdo_internal.h"
+#include "site_ops.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+namespace mpdo {
+// shared with api.hip
+mpdo_status layer_fail(mpdo_status s, const char* msg);
+mpdo_status layer_ws(size_t bytes, hipStream_t s, char** base);
+extern Prec g_tensor_prec;
+}  // namespace mpdo
+
+using namespace mpdo;
+
+namespace {
+
+#define LCK(x)                                                                              \
+  do {                                                                                      \
+    hipError_t e_ = (x);                                                                    \
+    if (e_ != hipSuccess) {                                                                 \
+      char b_[512];                                                                         \
+      snprintf(b_, sizeof b_, "%s:%d %s -> %s %s", __FILE__, __LINE__, #x, hipGetErrorString(e_),       \
+               t_oob.c_str());                                                                      \
+      return layer_fail(MPDO_EHIP, b_);                                                     \
+    }                                                                                       \
+  } while (0)
+#define LST(x)                     \
+  do {                             \
+    mpdo_status s_ = (x);          \
+    if (s_ != MPDO_OK) return s_;  \
+  } while (0)
+
+constexpr double FLOOR = 1e-13;         // relative eigenvalue floor of the QR-type factorisations
+constexpr double SPLIT_FLOOR = 1e-12;   // the same floor as the per-gate sqrt(S) splits (s_j < 1e-6 s_0)
+// absolute tail of the gate-time split (Circuit.py:22,120-124): the smallest k with
+// sqrt(sum_{j >= k} s_j^2) <= GATE_EPS singular values of the two-site tensor are kept
+constexpr double GATE_EPS = 2.718281828459045e-8;
+
+// ------------------------------------------------------------------ host-side bounds checks
+// Every device buffer the layer touches (each carved workspace buffer, each input and output
+// site) is registered as a span; every launch below checks the extent of each operand it
+// reads or writes against the span its pointer falls in and refuses to launch on overflow.
+struct Span {
+  uintptr_t lo, hi;
+};
+thread_local std::vector<Span>* t_spans = nullptr;
+thread_local std::string t_oob;
+// dry run: every check is made, nothing is launched or copied (host-only validation of a plan)
+thread_local bool t_dry = false;
+thread_local size_t t_ws_bytes = 0;   // workspace size of the last planned layer
+
+void add_span(const void* p, size_t bytes) {
+  if (t_spans && p && bytes) t_spans->push_back(Span{(uintptr_t)p, (uintptr_t)p + bytes});
+}
+// operand of `elems` elements of `esz` bytes starting at p lies inside one registered span
+bool fits(const void* p, long long elems, size_t esz, const char* what, int line) {
+  if (!t_spans || elems <= 0) return true;
+  const uintptr_t lo = (uintptr_t)p, hi = lo + (uintptr_t)elems * esz;
+  for (const Span& sp : *t_spans)
+    if (lo >= sp.lo && lo < sp.hi) {
+      if (hi <= sp.hi) return true;
+      char b[256];
+      snprintf(b, sizeof b, "layer.hip:%d %s overruns its buffer by %llu bytes", line, what,
+               (unsigned long long)(hi - sp.hi));
+      t_oob = b;
+      return false;
+    }
+  char b[256];
+  snprintf(b, sizeof b, "layer.hip:%d %s points outside every registered buffer", line, what);
+  t_oob = b;
+  return false;
+}
+// elements spanned by a strided batch of rows x cols matrices (non-negative strides)
+long long ext(int batch, long long bs, long long rows, long long rs, long long cols, long long cs) {
+  if (batch <= 0 || rows <= 0 || cols <= 0) return 0;
+  return (batch - 1) * bs + (rows - 1) * rs + (cols - 1) * cs + 1;
+}
+#define FITS(p, n, T, what) \
+  if (!fits((p), (n), sizeof(T), (what), __LINE__)) return hipErrorInvalidValue
+
+size_t in_bytes(Prec p) { return (p == C128_F64ACC || p == C128_IN_C64_OUT) ? 16 : 8; }
+size_t out_bytes(Prec p) { return (p == C64_F64ACC || p == C128_F64ACC) ? 16 : 8; }
+
+hipError_t k_gemm(Prec p, const GemmArgs& g, hipStream_t s, int line) {
+  if (!fits(g.A, ext(g.batch, g.bsA, g.M, g.rsA, g.K, g.csA), in_bytes(p), "GEMM operand A", line) ||
+      !fits(g.B, ext(g.batch, g.bsB, g.K, g.rsB, g.N, g.csB), in_bytes(p), "GEMM operand B", line) ||
+      !fits(g.C, ext(g.batch, g.bsC, g.M, g.rsC, g.N, g.csC), out_bytes(p), "GEMM result C", line))
+    return hipErrorInvalidValue;
+  if (t_dry) return hipSuccess;
+  return gemm(p, g, s);
+}
+#define K_GEMM(p, g, s) k_gemm((p), (g), (s), __LINE__)
+
+hipError_t k_factor(const double2* V, long long ldv, long long bsV, const double* lam, long long bsL, int rows,
+                    int nev, int k, double power, double floor_rel, int trans, void* out, long long ldo,
+                    long long bsO, int batch, hipStream_t s, bool f64, int line) {
+  const int kv = std::min(k, nev);
+  if (kv > 0 && (!fits(V, ext(batch, bsV, rows, ldv, kv, 1), sizeof(double2), "factor V", line) ||
+                 !fits(lam, ext(batch, bsL, 1, 0, kv, 1), sizeof(double), "factor lambda", line)))
+    return hipErrorInvalidValue;
+  const long long eo = trans ? ext(batch, bsO, k, ldo, rows, 1) : ext(batch, bsO, rows, ldo, k, 1);
+  if (!fits(out, eo, f64 ? sizeof(double2) : sizeof(float2), "factor output", line)) return hipErrorInvalidValue;
+  if (t_dry) return hipSuccess;
+  return f64 ? factor64_b(V, ldv, bsV, lam, bsL, rows, nev, k, power, floor_rel, trans, (double2*)out, ldo, bsO,
+                          batch, s)
+             : factor_b(V, ldv, bsV, lam, bsL, rows, nev, k, power, floor_rel, trans, (float2*)out, ldo, bsO, batch,
+                        s);
+}
+#define K_FACTOR(V, ldv, bsV, lam, bsL, rows, nev, k, pw, fl, tr, out, ldo, bsO, batch, s) \
+  k_factor(V, ldv, bsV, lam, bsL, rows, nev, k, pw, fl, tr, out, ldo, bsO, batch, s, false, __LINE__)
+#define K_FACTOR64(V, ldv, bsV, lam, bsL, rows, nev, k, pw, fl, tr, out, ldo, bsO, batch, s) \
+  k_factor(V, ldv, bsV, lam, bsL, rows, nev, k, pw, fl, tr, out, ldo, bsO, batch, s, true, __LINE__)
+
+hipError_t k_permute(const float2* in, float2* out, int rank, const int* dims, const int* perm, hipStream_t s,
+                     int batch, long long bstride, int line) {
+  long long tot = 1;
+  for (int i = 0; i < rank; ++i) tot *= dims[i];
+  const long long e = (batch - 1) * bstride + tot;
+  if (!fits(in, e, sizeof(float2), "permute input", line) || !fits(out, e, sizeof(float2), "permute output", line))
+    return hipErrorInvalidValue;
+  if (t_dry) return hipSuccess;
+  return permute(in, out, rank, dims, perm, s, batch, bstride);
+}
+#define K_PERMUTE(in, out, rank, dims, perm, s, batch, bs) k_permute(in, out, rank, dims, perm, s, batch, bs, __LINE__)
+
+hipError_t k_keep(const double* lam, long long ldw, int nev, int len_s, int msv, double err, int relative, int* k_out,
+                  double* disc_out, int batch, hipStream_t s, int line) {
+  if (!fits(lam, ext(batch, ldw, 1, 0, std::min(nev, len_s), 1), sizeof(double), "keep_count lambda", line) ||
+      !fits(k_out, batch, sizeof(int), "keep_count k", line) ||
+      !fits(disc_out, batch, sizeof(double), "keep_count discarded", line))
+    return hipErrorInvalidValue;
+  if (t_dry) return hipSuccess;
+  return keep_count(lam, ldw, nev, len_s, msv, err, relative, k_out, disc_out, batch, s);
+}
+#define K_KEEP(lam, ldw, nev, len_s, msv, err, rel, k_out, disc, batch, s) \
+  k_keep(lam, ldw, nev, len_s, msv, err, rel, k_out, disc, batch, s, __LINE__)
+
+hipError_t k_eye(float2* out, int rows, int cols, long long bs, int batch, hipStream_t s, int line) {
+  FITS(out, (batch - 1) * bs + (long long)rows * cols, float2, "identity output");
+  (void)line;
+  if (t_dry) return hipSuccess;
+  return eye_b(out, rows, cols, bs, batch, s);
+}
+#define K_EYE(out, rows, cols, bs, batch, s) k_eye(out, rows, cols, bs, batch, s, __LINE__)
+
+hipError_t k_channel2q(const float2* Y, const float2* Gt, int K, int P1, int P2, float2* C, hipStream_t s, int batch,
+                       long long bsY, long long bsC, int line) {
+  if (!fits(Y, (batch - 1) * bsY + 4LL * P1 * P2, sizeof(float2), "channel input", line) ||
+      !fits(Gt, 16LL * K, sizeof(float2), "channel tensor", line) ||
+      !fits(C, (batch - 1) * bsC + 4LL * P1 * K * P2, sizeof(float2), "channel output", line))
+    return hipErrorInvalidValue;
+  if (t_dry) return hipSuccess;
+  return channel2q(Y, Gt, K, P1, P2, C, s, batch, bsY, bsC);
+}
+
+hipError_t k_bds(const float2* M, int A, int R, double2* out, long long bsM, long long bsO, int batch, hipStream_t s,
+                 int line) {
+  const long long ar = (long long)A * R;
+  if (!fits(M, (batch - 1) * bsM + ar * ar, sizeof(float2), "block-diagonal input", line) ||
+      !fits(out, (batch - 1) * bsO + (long long)R * R, sizeof(double2), "block-diagonal output", line))
+    return hipErrorInvalidValue;
+  if (t_dry) return hipSuccess;
+  return block_diag_sum(M, A, R, out, bsM, bsO, batch, s);
+}
+
+hipError_t k_bds64(const double2* M, int A, int R, double2* out, long long bsM, long long bsO, int batch,
+                   hipStream_t s, int line) {
+  const long long ar = (long long)A * R;
+  if (!fits(M, (batch - 1) * bsM + ar * ar, sizeof(double2), "block-diagonal input", line) ||
+      !fits(out, (batch - 1) * bsO + (long long)R * R, sizeof(double2), "block-diagonal output", line))
+    return hipErrorInvalidValue;
+  if (t_dry) return hipSuccess;
+  return block_diag_sum64(M, A, R, out, bsM, bsO, batch, s);
+}
+
+hipError_t k_widen(const float2* in, double2* out, long long n, long long bsI, long long bsO, int batch, hipStream_t s,
+                   int line) {
+  if (!fits(in, (batch - 1) * bsI + n, sizeof(float2), "widen input", line) ||
+      !fits(out, (batch - 1) * bsO + n, sizeof(double2), "widen output", line))
+    return hipErrorInvalidValue;
+  if (t_dry) return hipSuccess;
+  return widen(in, out, n, bsI, bsO, batch, s);
+}
+
+hipError_t k_mask(const double2* V, long long ldv, long long bsV, int rows, int cols, const int* k, float2* out,
+                  long long ldo, long long bsO, int batch, hipStream_t s, int line) {
+  if (!fits(V, ext(batch, bsV, rows, ldv, cols, 1), sizeof(double2), "mask input", line) ||
+      !fits(k, batch, sizeof(int), "mask counts", line) ||
+      !fits(out, ext(batch, bsO, rows, ldo, cols, 1), sizeof(float2), "mask output", line))
+    return hipErrorInvalidValue;
+  if (t_dry) return hipSuccess;
+  return mask_cols_b(V, ldv, bsV, rows, cols, k, out, ldo, bsO, batch, s);
+}
+
+hipError_t k_copy(void* dst, const void* src, size_t bytes, hipStream_t s, int line) {
+  if (!fits(dst, (long long)bytes, 1, "copy destination", line) || !fits(src, (long long)bytes, 1, "copy source", line))
+    return hipErrorInvalidValue;
+  if (t_dry) return hipSuccess;
+  return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s);
+}
+#define K_COPY(dst, src, bytes, s) k_copy(dst, src, bytes, s, __LINE__)
+
+struct Carver {
+  char* base = nullptr;
+  size_t off = 0;
+  template <class T>
+  T* get(size_t n) {
+    size_t b = (n * sizeof(T) + 255) & ~size_t(255);
+    T* p = base ? (T*)(base + off) : nullptr;
+    if (p) add_span(p, n * sizeof(T));
+    off += b;
+    return p;
+  }
+};
+
+// ------------------------------------------------------------------ batched helpers
+hipError_t mmb_(int line, Prec p, int M, int N, int K, const void* A, long long lda, long long bsA, const void* B,
+               long long ldb, long long bsB, void* C, long long ldc, long long bsC, int batch, hipStream_t s,
+               bool conjTransA = false) {
+  GemmArgs g{};
+  g.M = M; g.N = N; g.K = K; g.batch = batch;
+  g.A = A; g.rsA = conjTransA ? 1 : lda; g.csA = conjTransA ? lda : 1; g.bsA = bsA; g.conjA = conjTransA;
+  g.B = B; g.rsB = ldb; g.csB = 1; g.bsB = bsB;
+  g.C = C; g.rsC = ldc; g.csC = 1; g.bsC = bsC;
+  g.alpha_re = 1;
+  return k_gemm(p, g, s, line);
+}
+#define mmb(...) mmb_(__LINE__, __VA_ARGS__)
+// G[b] (n x n, complex128) = X[b]^H X[b], X (rows x n) row-major
+hipError_t gram_cols_b_(int line, const float2* X, int rows, int n, long long bsX, double2* G, int batch, hipStream_t s) {
+  GemmArgs g{};
+  g.M = n; g.N = n; g.K = rows; g.batch = batch;
+  g.A = X; g.rsA = 1; g.csA = n; g.bsA = bsX; g.conjA = 1;
+  g.B = X; g.rsB = n; g.csB = 1; g.bsB = bsX;
+  g.C = G; g.rsC = n; g.csC = 1; g.bsC = (long long)n * n;
+  g.alpha_re = 1;
+  return k_gemm(C64_F64ACC, g, s, line);
+}
+// G[b] (n x n, complex128) = X[b] X[b]^H, X (n x cols) row-major
+hipError_t gram_rows_b_(int line, const float2* X, int n, int cols, long long bsX, double2* G, int batch, hipStream_t s) {
+  GemmArgs g{};
+  g.M = n; g.N = n; g.K = cols; g.batch = batch;
+  g.A = X; g.rsA = cols; g.csA = 1; g.bsA = bsX;
+  g.B = X; g.rsB = 1; g.csB = cols; g.bsB = bsX; g.conjB = 1;
+  g.C = G; g.rsC = n; g.csC = 1; g.bsC = (long long)n * n;
+  g.alpha_re = 1;
+  return k_gemm(C64_F64ACC, g, s, line);
+}
+
+#define gram_cols_b(...) gram_cols_b_(__LINE__, __VA_ARGS__)
+#define gram_rows_b(...) gram_rows_b_(__LINE__, __VA_ARGS__)
+
+struct EigB {
+  double2* V;
+  double* w;
+  void* ws;
+};
+void carve_eig(Carver& c, int n, int batch, EigB& e) {
+  e.V = c.get<double2>((size_t)n * n * batch);
+  e.w = c.get<double>((size_t)n * batch);
+  e.ws = c.get<char>(eigh_workspace_bytes(n, batch));
+}
+mpdo_status eig_b_(int line, int n, int batch, double2* G, EigB& e, hipStream_t s) {
+  int sweeps = 0;
+  const long long nn = (long long)n * n;
+  if (!fits(G, batch * nn, sizeof(double2), "eigensolver input", line) ||
+      !fits(e.V, batch * nn, sizeof(double2), "eigenvectors", line) ||
+      !fits(e.w, (long long)batch * n, sizeof(double), "eigenvalues", line) ||
+      !fits(e.ws, (long long)eigh_workspace_bytes(n, batch), 1, "eigensolver workspace", line)) {
+    return layer_fail(MPDO_EINVAL, t_oob.c_str());
+  }
+  if (t_dry) return MPDO_OK;
+  LCK(eigh(n, batch, G, n, (long long)n * n, e.w, n, e.V, n, (long long)n * n, e.ws, s, &sweeps));
+  if (sweeps > 60) {
+    char b[160];
+    snprintf(b, sizeof b, "Jacobi eigensolver did not converge (layer.hip:%d, order %d, batch %d)", line, n, batch);
+    return layer_fail(MPDO_ENOCONV, b);
+  }
+  return MPDO_OK;
+}
+#define eig_b(...) eig_b_(__LINE__, __VA_ARGS__)
+
+// Two-pass eigen-QR of a batch of tall matrices M (rows x cols, rows > cols):
+// Q (rows x cols) with orthonormal columns, Rf (cols x cols) complex64 with M = Q Rf.
+struct QRWS {
+  double2 *G, *P1, *P2;
+  float2 *F, *A1;
+  EigB e1, e2;
+};
+void carve_qr(Carver& c, int rows, int cols, int batch, QRWS& w) {
+  w.G = c.get<double2>((size_t)cols * cols * batch);
+  w.P1 = c.get<double2>((size_t)cols * cols * batch);
+  w.P2 = c.get<double2>((size_t)cols * cols * batch);
+  w.F = c.get<float2>((size_t)cols * cols * batch);
+  w.A1 = c.get<float2>((size_t)rows * cols * batch);
+  carve_eig(c, cols, batch, w.e1);
+  carve_eig(c, cols, batch, w.e2);
+}
+mpdo_status qr_b(const float2* M, int rows, int cols, float2* Q, float2* Rf, int batch, QRWS& w, hipStream_t s) {
+  const long long cc = (long long)cols * cols, rc = (long long)rows * cols;
+  LCK(gram_cols_b(M, rows, cols, rc, w.G, batch, s));
+  LST(eig_b(cols, batch, w.G, w.e1, s));
+  LCK(K_FACTOR(w.e1.V, cols, cc, w.e1.w, cols, cols, cols, cols, -0.5, FLOOR, 0, w.F, cols, cc, batch, s));
+  LCK(mmb(g_tensor_prec, rows, cols, cols, M, cols, rc, w.F, cols, cc, w.A1, cols, rc, batch, s));
+  LCK(gram_cols_b(w.A1, rows, cols, rc, w.G, batch, s));
+  LST(eig_b(cols, batch, w.G, w.e2, s));
+  LCK(K_FACTOR(w.e2.V, cols, cc, w.e2.w, cols, cols, cols, cols, -0.5, FLOOR, 0, w.F, cols, cc, batch, s));
+  LCK(mmb(g_tensor_prec, rows, cols, cols, w.A1, cols, rc, w.F, cols, cc, Q, cols, rc, batch, s));
+  LCK(K_FACTOR64(w.e2.V, cols, cc, w.e2.w, cols, cols, cols, cols, 0.5, FLOOR, 1, w.P2, cols, cc, batch, s));
+  LCK(K_FACTOR64(w.e1.V, cols, cc, w.e1.w, cols, cols, cols, cols, 0.5, FLOOR, 1, w.P1, cols, cc, batch, s));
+  LCK(mmb(C128_IN_C64_OUT, cols, cols, cols, w.P2, cols, cc, w.P1, cols, cc, Rf, cols, cc, batch, s));
+  return MPDO_OK;
+}
+// Two-pass eigen-LQ of a batch of wide matrices X (rows x cols, cols > rows):
+// Qt (rows x cols) with orthonormal rows, Lf (rows x rows) complex64 with X = Lf Qt.
+mpdo_status lq_b(const float2* X, int rows, int cols, float2* Qt, float2* Lf, int batch, QRWS& w, hipStream_t s) {
+  const long long rr = (long long)rows * rows, rc = (long long)rows * cols;
+  LCK(gram_rows_b(X, rows, cols, rc, w.G, batch, s));
+  LST(eig_b(rows, batch, w.G, w.e1, s));
+  LCK(K_FACTOR(w.e1.V, rows, rr, w.e1.w, rows, rows, rows, rows, -0.5, FLOOR, 1, w.F, rows, rr, batch, s));
+  LCK(mmb(g_tensor_prec, rows, cols, rows, w.F, rows, rr, X, cols, rc, w.A1, cols, rc, batch, s));
+  LCK(gram_rows_b(w.A1, rows, cols, rc, w.G, batch, s));
+  LST(eig_b(rows, batch, w.G, w.e2, s));
+  LCK(K_FACTOR(w.e2.V, rows, rr, w.e2.w, rows, rows, rows, rows, -0.5, FLOOR, 1, w.F, rows, rr, batch, s));
+  LCK(mmb(g_tensor_prec, rows, cols, rows, w.F, rows, rr, w.A1, cols, rc, Qt, cols, rc, batch, s));
+  // X = (W1 L1^{1/2}) (W2 L2^{1/2}) Qt
+  LCK(K_FACTOR64(w.e1.V, rows, rr, w.e1.w, rows, rows, rows, rows, 0.5, FLOOR, 0, w.P1, rows, rr, batch, s));
+  LCK(K_FACTOR64(w.e2.V, rows, rr, w.e2.w, rows, rows, rows, rows, 0.5, FLOOR, 0, w.P2, rows, rr, batch, s));
+  LCK(mmb(C128_IN_C64_OUT, rows, rows, rows, w.P1, rows, rr, w.P2, rows, rr, Lf, rows, rr, batch, s));
+  return MPDO_OK;
+}
+
+struct Dims {
+  int l, a, r;
+};
+
+}  // namespace
+
+namespace mpdo {
+
+// --------------------------------------------------------------------------------------------
+// Plan: output dims and applicability (host only).
+// --------------------------------------------------------------------------------------------
+struct LayerPlan {
+  int n, K1, K2, parity, chi;  // chi <= 0: none
+  int kappa;                   // <= 0: none
+  std::vector<int> role;       // 0 single, 1 pair min, 2 pair max
+  std::vector<Dims> in, a1;    // input dims, dims after the 1q channels
+  std::vector<int> kq;         // QR bond right of site i (singles and max sites)
+  std::vector<Dims> chi_out, out;
+  std::vector<int> step_len;   // len(s) of each chi step (bond i|i+1), index i
+  std::string why;
+  bool ok = false;
+};
+
+static int capk(int chi, long long len) { return (int)(chi > 0 ? std::min<long long>(chi, len) : len); }
+
+LayerPlan plan_layer(int n, const mpdo_site* in, int K1, int parity, int K2, int chi, int kappa) {
+  LayerPlan P;
+  P.n = n; P.K1 = K1; P.K2 = K2; P.parity = parity; P.chi = chi; P.kappa = kappa;
+  if (n < 2 || K1 < 1 || K2 < 1 || (parity != 0 && parity != 1)) { P.why = "bad arguments"; return P; }
+  P.role.assign(n, 0);
+  for (int i = parity; i + 1 < n; i += 2) { P.role[i] = 1; P.role[i + 1] = 2; }
+  for (int i = 0; i < n; ++i) {
+    P.in.push_back(Dims{in[i].l, in[i].a, in[i].r});
+    if (i + 1 < n && in[i].r != in[i + 1].l) { P.why = "bond mismatch"; return P; }
+  }
+  if (in[0].l != 1 || in[n - 1].r != 1) { P.why = "open boundary bonds must be 1"; return P; }
+  for (int i = 0; i < n; ++i) P.a1.push_back(Dims{in[i].l, in[i].a * K1, in[i].r});
+  auto afin = [&](int i) { return (long long)P.a1[i].a * (P.role[i] == 2 ? K2 : 1); };
+  // QR sweep bonds
+  P.kq.assign(n, 0);
+  long long kprev = 1;
+  for (int i = 0; i < n; ++i) {
+    if (P.role[i] == 1) continue;   // internal to the pair
+    if (i == n - 1) break;
+    if (P.role[i] == 0) {
+      long long rows = kprev * 2 * P.a1[i].a;
+      P.kq[i] = (int)std::min<long long>(rows, P.in[i].r);
+    } else {
+      // pair max: QR bond = min(D' * 2 a K2, r) = r when 2 a K2 >= r (D' >= 1)
+      if (2 * afin(i) < P.in[i].r) { P.why = "pair max site too thin for the fused path"; return P; }
+      P.kq[i] = P.in[i].r;
+    }
+    kprev = P.kq[i];
+  }
+  auto kq_left = [&](int i) -> long long { return i == 0 ? 1 : P.kq[i - 1]; };   // site i-1 single or max
+  P.chi_out.assign(n, Dims{0, 0, 0});
+  P.step_len.assign(n, 0);
+  long long kright = 1;
+  // chi sweep, right to left
+  for (int j = n - 1; j >= 0;) {
+    if (P.role[j] == 0) {
+      P.chi_out[j].r = (int)kright;
+      P.chi_out[j].a = (int)afin(j);
+      if (j == 0) { P.chi_out[0].l = 1; break; }
+      long long cols = 2 * afin(j) * kright;
+      long long len;
+      if (P.role[j - 1] == 2) {
+        long long lower = 2 * afin(j - 1);
+        if (chi > 0 ? lower < std::min<long long>(chi, cols) : lower < cols) { P.why = "bond rank depends on the pair split"; return P; }
+        len = cols;
+      } else {
+        len = std::min<long long>(kq_left(j - 1) * 2 * afin(j - 1), cols);
+      }
+      int k = capk(chi, len);
+      P.step_len[j - 1] = (int)len;
+      P.chi_out[j].l = k;
+      kright = k;
+      j -= 1;
+    } else {
+      // pair (i = j-1, j)
+      int i = j - 1;
+      if (P.role[i] != 1) { P.why = "internal: pair layout"; return P; }
+      long long len_p = std::min<long long>(kq_left(i) * 2 * afin(i), 2 * afin(j) * kright);
+      int kp = capk(chi, len_p);
+      P.step_len[i] = (int)len_p;
+      P.chi_out[j] = Dims{kp, (int)afin(j), (int)kright};
+      P.chi_out[i].a = (int)afin(i);
+      P.chi_out[i].r = kp;
+      if (i == 0) { P.chi_out[0].l = 1; break; }
+      long long cols = 2 * afin(i) * kp;
+      long long len;
+      if (P.role[i - 1] == 2) {
+        long long lower = 2 * afin(i - 1);
+        if (chi > 0 ? lower < std::min<long long>(chi, cols) : lower < cols) { P.why = "bond rank depends on the pair split"; return P; }
+        len = cols;
+      } else {
+        len = std::min<long long>(kq_left(i - 1) * 2 * afin(i - 1), cols);
+      }
+      int k = capk(chi, len);
+      P.step_len[i - 1] = (int)len;
+      P.chi_out[i].l = k;
+      kright = k;
+      j = i - 1;
+    }
+  }
+  P.out = P.chi_out;
+  for (int i = 0; i < n; ++i) {
+    const Dims& d = P.chi_out[i];
+    if (kappa > 0 && d.a > kappa) {
+      long long len_s = std::min<long long>(2LL * d.l * d.r, d.a);
+      P.out[i].a = (int)std::min<long long>(kappa, len_s);
+    } else if (kappa <= 0) {
+      P.out[i].a = d.a;   // no kappa cap and no error bound: svdKappa is a no-op
+    }
+  }
+  P.ok = true;
+  return P;
+}
+
+// --------------------------------------------------------------------------------------------
+struct PairStore {
+  int i;                        // min site
+  int l, rows1t, p1t;           // T_min as [(l a1) x (s1 m)] = Q1t R1t, p1t = min(l a1, 2m)
+  int kprev, rows1, cols1, p1;  // X1r = (R (x) I_a1) Q1t: [(kprev a1) x p1t] = Q1 Ra, p1 = min(rows1, p1t)
+  int m, rows2, cols2, p2;      // T_max as [(m s2) x (a2 r)] = L2 Q2
+  bool q1tid, q1id, q2id;
+  float2 *Q1, *Ra, *Y, *Um, *Q2, *L2;   // Y: the theta core before the channel, [2][2][p1t][p2];
+                                        // Um: kept left singular vectors of the theta core
+  double2* WR;
+  double* lamR;
+  bool has_R, has_P;            // has_P: the gate-time cut drops a direction in some chain
+};
+
+mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const float* kraus1, const float* gates2,
+                      const mpdo_site* out, double* disc_host, hipStream_t s, bool dry) {
+  struct DryScope {
+    explicit DryScope(bool d) { t_dry = d; }
+    ~DryScope() { t_dry = false; }
+  } dry_scope(dry);
+  const int n = P.n, B = batch, K1 = P.K1, K2 = P.K2;
+  // ---------------- layout ----------------
+  std::vector<PairStore> pairs;
+  for (int i = 0; i < n; ++i)
+    if (P.role[i] == 1) {
+      PairStore ps{};
+      ps.i = i;
+      ps.m = P.in[i].r;
+      ps.rows2 = 2 * ps.m;
+      ps.cols2 = P.a1[i + 1].a * P.in[i + 1].r;
+      ps.q2id = ps.cols2 <= ps.rows2;
+      ps.p2 = std::min(ps.rows2, ps.cols2);
+      ps.l = P.in[i].l;
+      ps.rows1t = ps.l * P.a1[i].a;
+      ps.q1tid = ps.rows1t <= 2 * ps.m;
+      ps.p1t = std::min(ps.rows1t, 2 * ps.m);
+      ps.kprev = (i == 0) ? 1 : P.kq[i - 1];
+      ps.rows1 = ps.kprev * P.a1[i].a;
+      ps.cols1 = ps.p1t;
+      ps.q1id = ps.rows1 <= ps.cols1;
+      ps.p1 = std::min(ps.rows1, ps.cols1);
+      ps.has_R = (i + 1 < n - 1);
+      pairs.push_back(ps);
+    }
+  // sizes (elements per chain) of the shared scratch buffers: each the maximum over its uses
+  auto kapplies = [&](int i) { return P.out[i].a != P.chi_out[i].a; };
+  size_t mxQR_rows = 1, mxQR_cols = 1, mxR = 1, mxY = 1, mxOm = 1, mxZ = 1, mxQ2P = 1, mxC = 1, mxC2 = 1;
+  size_t mxTmin = 1, mxWm = 1, mxTm = 1, mxFL = 1, mxPn = 1, mxRT = 1, mxX1 = 1, mxLast = 1, mxTcur = 1;
+  size_t mxTchi = 1, mxMp = 1, mxKout = 1, mxX1t = 1, mxQ1t = 1, mxR1t = 1;
+  int mxEigQR = 1, mxEigC = 1, mxEigChi = 1, mxEigK = 1, mxEigR = 1, mxEigL3 = 1;
+  for (int i = 0; i < n; ++i) {
+    const Dims& c = P.chi_out[i];
+    const size_t szc = (size_t)c.l * 2 * c.a * c.r;
+    mxEigChi = std::max(mxEigChi, c.l);
+    mxPn = std::max(mxPn, (size_t)std::max(c.l, P.in[i].r) * std::max(c.l, c.r));
+    if (P.role[i] != 2) mxTchi = std::max(mxTchi, szc);
+    if (kapplies(i)) {
+      mxEigK = std::max(mxEigK, c.a);
+      mxMp = std::max(mxMp, szc);
+      mxKout = std::max(mxKout, (size_t)2 * c.l * c.r * P.out[i].a);
+    }
+  }
+  long long kprev = 1;
+  for (int i = 0; i < n; ++i) {
+    if (P.role[i] == 0) {
+      const size_t m = (size_t)(kprev * 2 * P.a1[i].a * P.in[i].r);
+      if (i == n - 1) {
+        mxLast = std::max(mxLast, m);
+      } else {
+        long long rows = kprev * 2 * P.a1[i].a, cols = P.in[i].r;
+        mxRT = std::max(mxRT, m);
+        if (rows > cols) { mxQR_rows = std::max(mxQR_rows, (size_t)rows); mxQR_cols = std::max(mxQR_cols, (size_t)cols); }
+        mxEigQR = std::max(mxEigQR, (int)cols);
+        mxTcur = std::max(mxTcur, (size_t)(kprev * 2 * P.chi_out[i].a) * P.chi_out[i].r);   // A_i Pn
+        kprev = P.kq[i];
+      }
+    } else if (P.role[i] == 2 && i < n - 1) {
+      kprev = P.kq[i];
+    }
+    mxR = std::max(mxR, (size_t)(kprev * std::max(P.in[i].l, P.in[i].r)));
+  }
+  for (auto& ps : pairs) {
+    const int kp = P.chi_out[ps.i + 1].l, kr = P.chi_out[ps.i + 1].r, a2 = P.a1[ps.i + 1].a;
+    mxX1t = std::max(mxX1t, (size_t)ps.rows1t * 2 * ps.m);
+    mxQ1t = std::max(mxQ1t, (size_t)ps.rows1t * ps.p1t);
+    mxR1t = std::max(mxR1t, (size_t)ps.p1t * 2 * ps.m);
+    mxX1 = std::max(mxX1, (size_t)ps.rows1 * ps.p1t);
+    if (!ps.q1tid) { mxQR_rows = std::max(mxQR_rows, (size_t)ps.rows1t); mxQR_cols = std::max(mxQR_cols, (size_t)2 * ps.m); }
+    if (!ps.q1id) { mxQR_rows = std::max(mxQR_rows, (size_t)ps.rows1); mxQR_cols = std::max(mxQR_cols, (size_t)ps.cols1); }
+    if (!ps.q2id) { mxQR_rows = std::max(mxQR_rows, (size_t)ps.cols2); mxQR_cols = std::max(mxQR_cols, (size_t)ps.rows2); }
+    mxEigQR = std::max(mxEigQR, std::max({2 * ps.m, ps.cols1, ps.rows2}));
+    mxEigC = std::max(mxEigC, 2 * ps.p1t);
+    mxY = std::max(mxY, (size_t)4 * ps.p1t * ps.p2);
+    mxC = std::max(mxC, (size_t)2 * ps.p1t * 2 * K2 * ps.p2);
+    mxOm = std::max(mxOm, (size_t)ps.p2 * ps.p2);
+    mxZ = std::max(mxZ, (size_t)ps.p2 * ps.cols2);
+    mxEigR = std::max(mxEigR, P.in[ps.i + 1].r);
+    const int c3 = a2 * kr;
+    mxQ2P = std::max(mxQ2P, (size_t)ps.p2 * c3);
+    if (c3 > ps.p2) { mxQR_rows = std::max(mxQR_rows, (size_t)c3); mxQR_cols = std::max(mxQR_cols, (size_t)ps.p2); }
+    mxEigL3 = std::max(mxEigL3, ps.p2);
+    const int p3 = std::min(ps.p2, c3);
+    mxC2 = std::max(mxC2, (size_t)2 * ps.p1 * 2 * K2 * p3);
+    mxEigC = std::max(mxEigC, 2 * ps.p1);
+    mxFL = std::max(mxFL, (size_t)2 * ps.p1 * kp);
+    mxTmin = std::max(mxTmin, (size_t)ps.rows1 * 2 * kp);
+    mxTcur = std::max(mxTcur, (size_t)ps.kprev * 2 * P.a1[ps.i].a * kp);
+    mxWm = std::max(mxWm, (size_t)kp * 2 * K2 * p3);
+    if (c3 > ps.p2) mxTm = std::max(mxTm, (size_t)kp * 2 * K2 * c3);
+    mxPn = std::max(mxPn, (size_t)P.in[ps.i + 1].r * std::max(kr, P.in[ps.i + 1].r));
+  }
+  const int mxEig = std::max({mxEigQR, mxEigC, mxEigChi, mxEigK, mxEigR, mxEigL3});
+  size_t mxBond = 1;
+  for (int i = 0; i < n; ++i) {
+    mxBond = std::max(mxBond, (size_t)std::max(P.in[i].l, P.in[i].r));
+    mxBond = std::max(mxBond, (size_t)std::max(P.chi_out[i].l, P.chi_out[i].r));
+  }
+  const size_t mxBond2 = mxBond * mxBond;
+  mxFL = std::max(mxFL, mxBond2);
+  mxPn = std::max(mxPn, mxBond2);
+
+  // device buffers
+  std::vector<float2*> T1(n, nullptr), Astore(n, nullptr);
+  Site1q* dtab = nullptr;
+  float2 *Mlast = nullptr, *Rcur, *Rnext, *RT, *X1, *X1t, *Q1t, *R1t, *Cs, *Cp, *Cu, *Zb, *Q2P, *L3, *Q3, *C2, *FL, *FR, *Tmin, *Wm,
+         *Tm, *Pn, *Pn2, *PW, *Pp, *Tchi, *Mp, *Vk, *Kout, *dG, *Tcur2, *dK;
+  double2 *G, *GR, *Om64, *Q2w, *Z64;
+  double* ddisc;
+  int* dk;
+  EigB eE{};
+  QRWS qw{};
+  const int nsteps = n;
+  auto layout = [&](Carver& c) {
+    for (int i = 0; i < n; ++i) T1[i] = c.get<float2>((size_t)P.a1[i].l * 2 * P.a1[i].a * P.a1[i].r * B);
+    long long kp = 1;
+    for (int i = 0; i < n - 1; ++i) {
+      if (P.role[i] == 0) {
+        long long rows = kp * 2 * P.a1[i].a;
+        Astore[i] = c.get<float2>((size_t)(rows * P.kq[i]) * B);
+        kp = P.kq[i];
+      } else if (P.role[i] == 2) {
+        kp = P.kq[i];
+      }
+    }
+    for (auto& ps : pairs) {
+      ps.Q1 = ps.q1id ? nullptr : c.get<float2>((size_t)ps.rows1 * ps.p1 * B);
+      ps.Ra = c.get<float2>((size_t)ps.p1 * ps.p1t * B);
+      ps.Y = c.get<float2>((size_t)4 * ps.p1t * ps.p2 * B);
+      ps.Um = c.get<float2>((size_t)4 * ps.p1t * ps.p1t * B);
+      ps.Q2 = ps.q2id ? nullptr : c.get<float2>((size_t)ps.p2 * ps.cols2 * B);
+      ps.L2 = ps.q2id ? nullptr : c.get<float2>((size_t)ps.rows2 * ps.p2 * B);
+      const int r = P.in[ps.i + 1].r;
+      ps.WR = ps.has_R ? c.get<double2>((size_t)r * r * B) : nullptr;
+      ps.lamR = ps.has_R ? c.get<double>((size_t)r * B) : nullptr;
+    }
+    Mlast = c.get<float2>(mxLast * B);
+    Rcur = c.get<float2>(std::max(mxR, mxBond2) * B);
+    Rnext = c.get<float2>(std::max(mxR, mxBond2) * B);
+    RT = c.get<float2>(mxRT * B);
+    X1 = c.get<float2>(mxX1 * B);
+    X1t = c.get<float2>(mxX1t * B);
+    Q1t = c.get<float2>(mxQ1t * B);
+    R1t = c.get<float2>(mxR1t * B);
+    Cs = c.get<float2>(mxC * B);
+    Cp = c.get<float2>(mxC * B);
+    Cu = c.get<float2>(mxC * B);
+    Om64 = c.get<double2>(mxOm * B);
+    Q2w = c.get<double2>(mxZ * B);
+    Z64 = c.get<double2>(mxZ * B);
+    Zb = c.get<float2>(std::max(mxZ, mxBond2) * B);
+    dtab = c.get<Site1q>((size_t)n * B);
+    Q2P = c.get<float2>(mxQ2P * B);
+    L3 = c.get<float2>(mxOm * B);
+    Q3 = c.get<float2>(mxQ2P * B);
+    C2 = c.get<float2>(mxC2 * B);
+    FL = c.get<float2>(mxFL * B);
+    FR = c.get<float2>(mxFL * B);
+    Tmin = c.get<float2>(mxTmin * B);
+    Tcur2 = c.get<float2>(mxTcur * B);
+    Wm = c.get<float2>(mxWm * B);
+    Tm = c.get<float2>(mxTm * B);
+    Pn = c.get<float2>(mxPn * B);
+    Pn2 = c.get<float2>(mxPn * B);
+    PW = c.get<float2>(mxBond2 * B);
+    Pp = c.get<float2>(mxPn * B);
+    Tchi = c.get<float2>(mxTchi * B);
+    Mp = c.get<float2>(mxMp * B);
+    Vk = c.get<float2>((size_t)mxEigK * mxEigK * B);
+    Kout = c.get<float2>(mxKout * B);
+    dG = c.get<float2>((size_t)pairs.size() * 16 * K2 + 16);
+    dK = c.get<float2>((size_t)n * B * 4 * K1);
+    G = c.get<double2>((size_t)mxEig * mxEig * B);
+    GR = c.get<double2>((size_t)mxEig * mxEig * B);
+    carve_eig(c, mxEig, B, eE);
+    carve_qr(c, (int)mxQR_rows, (int)mxQR_cols, B, qw);
+    dk = c.get<int>((size_t)B * 4);
+    ddisc = c.get<double>((size_t)B * (nsteps + 2 * n));
+  };
+  Carver c0;
+  layout(c0);
+  t_ws_bytes = c0.off;
+  char* base;
+  if (dry) base = (char*)(uintptr_t(1) << 44);   // never dereferenced: nothing is launched
+  else LST(layer_ws(c0.off, s, &base));
+  std::vector<Span> spans;
+  struct SpanScope {
+    explicit SpanScope(std::vector<Span>* p) { t_spans = p; t_oob.clear(); }
+    ~SpanScope() { t_spans = nullptr; }
+  } span_scope(&spans);
+  for (int i = 0; i < n; ++i) {
+    add_span(in[i].data, sizeof(float2) * (size_t)P.in[i].l * 2 * P.in[i].a * P.in[i].r * B);
+    add_span(out[i].data, sizeof(float2) * (size_t)P.out[i].l * 2 * P.out[i].a * P.out[i].r * B);
+  }
+  Carver c1;
+  c1.base = base;
+  layout(c1);
+#define HFITS(p, cnt, T, what) \
+  if (!fits((p), (long long)(cnt), sizeof(T), (what), __LINE__)) return layer_fail(MPDO_EINVAL, t_oob.c_str())
+  HFITS(ddisc, (size_t)B * (nsteps + 2 * n), double, "discarded weights");
+  if (!dry) LCK(hipMemsetAsync(ddisc, 0, sizeof(double) * B * (nsteps + 2 * n), s));
+
+  // ---------------- phase 0: single-qubit channels ----------------
+  {
+    std::vector<Site1q> tab((size_t)n * B);
+    for (int b = 0; b < B; ++b)
+      for (int i = 0; i < n; ++i) {
+        const Dims& d = P.in[i];
+        const long long sz_in = (long long)d.l * 2 * d.a * d.r, sz_out = sz_in * K1;
+        tab[(size_t)b * n + i] = Site1q{(const float2*)in[i].data + b * sz_in, T1[i] + b * sz_out, d.l, d.a, d.r, K1,
+                                        ((long long)b * n + i) * 4 * K1};
+      }
+    for (const Site1q& e : tab) {
+      HFITS(e.in, (long long)e.l * 2 * e.a * e.r, float2, "single-qubit channel input");
+      HFITS(e.out, (long long)e.l * 2 * e.a * e.K * e.r, float2, "single-qubit channel output");
+      HFITS(dK + e.koff, 4LL * e.K, float2, "single-qubit Kraus operators");
+    }
+    HFITS(dtab, tab.size(), Site1q, "site table");
+    HFITS(dK, (size_t)n * B * 4 * K1, float2, "Kraus buffer");
+    if (!dry) {
+      LCK(hipMemcpyAsync(dtab, tab.data(), sizeof(Site1q) * tab.size(), hipMemcpyHostToDevice, s));
+      LCK(hipMemcpyAsync(dK, kraus1, sizeof(float2) * (size_t)n * B * 4 * K1, hipMemcpyHostToDevice, s));
+    }
+    long long maxe = 0;
+    for (int i = 0; i < n; ++i) maxe = std::max(maxe, (long long)P.in[i].l * P.in[i].a * P.in[i].r);
+    if (!dry) {
+      LCK(apply_1q(n * B, dtab, dK, maxe, s));
+      LCK(hipStreamSynchronize(s));   // tab is pageable host memory
+    }
+  }
+  HFITS(dG, pairs.size() * 16 * K2, float2, "two-qubit channel buffer");
+  if (!dry) LCK(hipMemcpyAsync(dG, gates2, sizeof(float2) * pairs.size() * 16 * K2, hipMemcpyHostToDevice, s));
+
+  // ---------------- phase 1: LQ of every pair's max site ----------------
+  for (auto& ps : pairs) {
+    if (ps.q2id) continue;
+    // X2 = T1[max] as [(m s2) x (a2 r)] with rows2 < cols2: X2 = L2 Q2
+    LST(lq_b(T1[ps.i + 1], ps.rows2, ps.cols2, ps.Q2, ps.L2, B, qw, s));
+  }
+
+  // Cu = (Ra (x) I_s) P G(Y): the pair core in the basis of Q1 (P = Um Um^H when the gate-time
+  // cut drops directions in some chain, else the identity)
+  auto make_core = [&](PairStore& ps, int gidx) -> mpdo_status {
+    const int p1t = ps.p1t, nT = 2 * p1t;
+    const long long cols = 2LL * K2 * ps.p2, szCt = (long long)nT * cols;
+    LCK(k_channel2q(ps.Y, dG + (size_t)gidx * 16 * K2, K2, p1t, ps.p2, Cs, s, B, 4LL * p1t * ps.p2, szCt,
+                    __LINE__));
+    if (ps.has_P) {
+      LCK(mmb(g_tensor_prec, nT, (int)cols, nT, ps.Um, nT, (long long)nT * nT, Cs, cols, szCt, Cp, cols, szCt, B, s,
+              true));
+      LCK(mmb(g_tensor_prec, nT, (int)cols, nT, ps.Um, nT, (long long)nT * nT, Cp, cols, szCt, Cs, cols, szCt, B, s));
+    }
+    // Cs viewed as [p1t x (s1 cols)]
+    LCK(mmb(g_tensor_prec, ps.p1, (int)(2 * cols), p1t, ps.Ra, p1t, (long long)ps.p1 * p1t, Cs, 2 * cols, szCt, Cu,
+            2 * cols, 2LL * ps.p1 * cols, B, s));
+    return MPDO_OK;
+  };
+
+  // ---------------- phase 2: QR sweep left to right ----------------
+  // Rcur: [B][kp][l] with kp the QR bond so far (1 at the left end)
+  int kcur = 1;
+  {
+    std::vector<float2> one(B, make_float2(1.f, 0.f));
+    HFITS(Rcur, B, float2, "left factor");
+    if (!dry) {
+      LCK(hipMemcpyAsync(Rcur, one.data(), sizeof(float2) * B, hipMemcpyHostToDevice, s));
+      LCK(hipStreamSynchronize(s));
+    }
+  }
+  size_t pi = 0;
+  for (int i = 0; i < n; ++i) {
+    if (P.role[i] == 0) {
+      const Dims& d = P.a1[i];
+      const long long colsT = 2LL * d.a * d.r, rows = (long long)kcur * 2 * d.a;
+      float2* M = (i == n - 1) ? Mlast : RT;
+      LCK(mmb(g_tensor_prec, kcur, (int)colsT, d.l, Rcur, d.l, (long long)kcur * d.l, T1[i], colsT,
+              (long long)d.l * colsT, M, colsT, kcur * colsT, B, s));
+      if (i == n - 1) break;
+      const int cols = d.r, kq = P.kq[i];
+      if (rows <= cols) {
+        LCK(K_EYE(Astore[i], (int)rows, (int)rows, rows * rows, B, s));
+        // R = M (rows x cols)
+        LCK(K_COPY(Rnext, M, sizeof(float2) * rows * cols * B, s));
+      } else {
+        LST(qr_b(M, (int)rows, cols, Astore[i], Rnext, B, qw, s));
+      }
+      std::swap(Rcur, Rnext);
+      kcur = kq;
+      continue;
+    }
+    if (P.role[i] == 2) continue;
+    // pair (i, i+1)
+    PairStore& ps = pairs[pi++];
+    const Dims& dmin = P.a1[i];
+    const int m = ps.m, a1 = dmin.a, a2 = P.a1[i + 1].a, r = P.in[i + 1].r;
+    const long long colsT = 2LL * a1 * m;
+    // (1) the reference's gate-time split, in compressed form. T_min as [(l a1) x (s1 m)] = Q1t R1t
+    //     and T_max = L2 Q2 give theta = (Q1t (x) I) C_theta (I (x) Q2) with C_theta = G(R1t L2):
+    //     its Gram gives the singular values of theta and the kept count m' (absolute tail).
+    const int l = ps.l, p1t = ps.p1t, m2 = 2 * m;
+    {
+      int dims[4] = {l, 2, a1, m}, perm[4] = {0, 2, 1, 3};
+      LCK(K_PERMUTE(T1[i], X1t, 4, dims, perm, s, B, (long long)l * colsT));
+    }
+    const float2* R1tp;
+    long long bsR1t;
+    if (ps.q1tid) {   // Q1t = identity
+      LCK(K_EYE(Q1t, ps.rows1t, ps.rows1t, (long long)ps.rows1t * ps.rows1t, B, s));
+      R1tp = X1t;
+      bsR1t = (long long)ps.rows1t * m2;
+    } else {
+      LST(qr_b(X1t, ps.rows1t, m2, Q1t, R1t, B, qw, s));
+      R1tp = R1t;
+      bsR1t = (long long)m2 * m2;
+    }
+    const float2* L2p = ps.q2id ? T1[i + 1] : ps.L2;
+    const long long bsL2 = ps.q2id ? (long long)ps.rows2 * ps.cols2 : (long long)ps.rows2 * ps.p2;
+    // Y[s1][s2] = R1t[:, s1 m:(s1+1) m] L2[(m s2), :]
+    for (int s1 = 0; s1 < 2; ++s1)
+      for (int s2 = 0; s2 < 2; ++s2) {
+        GemmArgs g{};
+        g.M = p1t; g.N = ps.p2; g.K = m; g.batch = B;
+        g.A = R1tp + (long long)s1 * m; g.rsA = m2; g.csA = 1; g.bsA = bsR1t;
+        g.B = L2p + (long long)s2 * ps.p2; g.rsB = 2LL * ps.p2; g.csB = 1; g.bsB = bsL2;
+        g.C = ps.Y + (long long)(s1 * 2 + s2) * p1t * ps.p2; g.rsC = ps.p2; g.csC = 1;
+        g.bsC = 4LL * p1t * ps.p2;
+        g.alpha_re = 1;
+        LCK(K_GEMM(g_tensor_prec, g, s));
+      }
+    {
+      const long long szCt = 2LL * p1t * 2 * K2 * ps.p2;
+      LCK(k_channel2q(ps.Y, dG + (pi - 1) * 16 * K2, K2, p1t, ps.p2, Cs, s, B, 4LL * p1t * ps.p2, szCt, __LINE__));
+      const int nT = 2 * p1t;
+      const int len_gate = (int)std::min(2LL * l * a1, 2LL * a2 * K2 * r);
+      LCK(gram_rows_b(Cs, nT, (int)(2LL * K2 * ps.p2), szCt, G, B, s));
+      LST(eig_b(nT, B, G, eE, s));
+      LCK(K_KEEP(eE.w, nT, nT, len_gate, -1, GATE_EPS, 0, dk, ddisc + (size_t)B * (nsteps + n + i), B, s));
+      // the projector onto the kept directions is applied only when some chain drops one
+      ps.has_P = dry;
+      if (!dry) {
+        std::vector<int> kept(B);
+        LCK(hipMemcpyAsync(kept.data(), dk, sizeof(int) * B, hipMemcpyDeviceToHost, s));
+        LCK(hipStreamSynchronize(s));
+        for (int b = 0; b < B; ++b) ps.has_P = ps.has_P || kept[b] < nT;
+      }
+      if (ps.has_P)
+        LCK(k_mask(eE.V, nT, (long long)nT * nT, nT, nT, dk, ps.Um, nT, (long long)nT * nT, B, s, __LINE__));
+    }
+    // (2) QR sweep through the pair: X1r = (R (x) I_a1) Q1t = Q1 Ra, [(kcur a1) x p1t]
+    LCK(mmb(g_tensor_prec, kcur, a1 * p1t, l, Rcur, l, (long long)kcur * l, Q1t, a1 * p1t,
+            (long long)ps.rows1t * p1t, X1, a1 * p1t, (long long)kcur * a1 * p1t, B, s));
+    if (ps.q1id) {
+      LCK(K_COPY(ps.Ra, X1, sizeof(float2) * (size_t)ps.rows1 * p1t * B, s));
+    } else {
+      LST(qr_b(X1, ps.rows1, p1t, ps.Q1, ps.Ra, B, qw, s));
+    }
+    if (!ps.has_R) continue;
+    // (3) the core of R theta in the basis of Q1: Cu = (Ra (x) I_s) P C_theta, [(p1 s1) x (s2 K2 p2)]
+    LST(make_core(ps, (int)pi - 1));
+    const long long szC = 2LL * ps.p1 * 2 * K2 * ps.p2;
+    // right-bond Gram of R theta, all in complex128 so that it is Hermitian to fp64 rounding:
+    // Om = C_v^H C_v with C_v = [(2 p1 2 K2) x p2]; GR = sum_{q a2} conj(Q2[q a2, r]) (Om Q2)[q a2, r']
+    {
+      GemmArgs g{};
+      const int rowsv = 2 * ps.p1 * 2 * K2;
+      g.M = ps.p2; g.N = ps.p2; g.K = rowsv; g.batch = B;
+      g.A = Cu; g.rsA = 1; g.csA = ps.p2; g.bsA = szC; g.conjA = 1;
+      g.B = Cu; g.rsB = ps.p2; g.csB = 1; g.bsB = szC;
+      g.C = Om64; g.rsC = ps.p2; g.csC = 1; g.bsC = (long long)ps.p2 * ps.p2;
+      g.alpha_re = 1;
+      LCK(K_GEMM(C64_F64ACC, g, s));
+    }
+    if (ps.q2id) {
+      LCK(k_bds64(Om64, a2, r, GR, (long long)ps.p2 * ps.p2, (long long)r * r, B, s, __LINE__));
+    } else {
+      const long long qz = (long long)ps.p2 * ps.cols2;
+      LCK(k_widen(ps.Q2, Q2w, qz, qz, qz, B, s, __LINE__));
+      LCK(mmb(C128_F64ACC, ps.p2, ps.cols2, ps.p2, Om64, ps.p2, (long long)ps.p2 * ps.p2, Q2w, ps.cols2, qz, Z64,
+              ps.cols2, qz, B, s));
+      GemmArgs g{};
+      const int rowsv = ps.p2 * a2;
+      g.M = r; g.N = r; g.K = rowsv; g.batch = B;
+      g.A = Q2w; g.rsA = 1; g.csA = r; g.bsA = qz; g.conjA = 1;
+      g.B = Z64; g.rsB = r; g.csB = 1; g.bsB = qz;
+      g.C = GR; g.rsC = r; g.csC = 1; g.bsC = (long long)r * r;
+      g.alpha_re = 1;
+      LCK(K_GEMM(C128_F64ACC, g, s));
+    }
+    {
+      EigB eR{ps.WR, ps.lamR, eE.ws};
+      LST(eig_b(r, B, GR, eR, s));
+    }
+    // R_max = L^{1/2} W^H  (r x r)
+    LCK(K_FACTOR(ps.WR, r, (long long)r * r, ps.lamR, r, r, r, r, 0.5, FLOOR, 1, Rcur, r, (long long)r * r, B, s));
+    kcur = r;
+  }
+
+  // ---------------- phase 3: chi sweep right to left, kappa as sites become final ----------
+  // kappa cut of site i given as Mq = [l][2][r][a] (inner index last): out[i] = (Mq V_k) permuted
+  auto kappa_on = [&](int i, const float2* Mq) -> mpdo_status {
+    const Dims& c = P.chi_out[i];
+    const long long sz = (long long)c.l * 2 * c.a * c.r;
+    const int rows = 2 * c.l * c.r, a = c.a, k = P.out[i].a;
+    LCK(gram_cols_b(Mq, rows, a, sz, G, B, s));
+    LST(eig_b(a, B, G, eE, s));
+    LCK(K_KEEP(eE.w, a, a, std::min(rows, a), k, -1.0, 1, dk, ddisc + (size_t)B * (nsteps + i), B, s));
+    LCK(K_FACTOR(eE.V, a, (long long)a * a, eE.w, a, a, a, k, 0.0, 0.0, 0, Vk, k, (long long)a * k, B, s));
+    LCK(mmb(g_tensor_prec, rows, k, a, Mq, a, sz, Vk, k, (long long)a * k, Kout, k, (long long)rows * k, B, s));
+    int d2[4] = {c.l, 2, c.r, k}, p2[4] = {0, 1, 3, 2};
+    LCK(K_PERMUTE(Kout, (float2*)out[i].data, 4, d2, p2, s, B, (long long)rows * k));
+    return MPDO_OK;
+  };
+  // where the chi step writes site i: straight into the output when no kappa cut follows
+  auto chi_target = [&](int i) -> float2* { return kapplies(i) ? Tchi : (float2*)out[i].data; };
+  // kappa cut of site i held as T = [l][2][a][r] in chi_target(i)
+  auto kappa_site = [&](int i, const float2* T) -> mpdo_status {
+    const Dims& c = P.chi_out[i];
+    if (!kapplies(i)) {
+      if (T != out[i].data) LCK(K_COPY(out[i].data, T, sizeof(float2) * (size_t)c.l * 2 * c.a * c.r * B, s));
+      return MPDO_OK;
+    }
+    int dims[4] = {c.l, 2, c.a, c.r}, perm[4] = {0, 1, 3, 2};
+    LCK(K_PERMUTE(T, Mp, 4, dims, perm, s, B, (long long)c.l * 2 * c.a * c.r));
+    return kappa_on(i, Mp);
+  };
+  // canonical chi step on B (D x colsB): out T_right = S^{-1/2} U^H B (k x colsB), Pn = U sqrt(S) (D x k)
+  auto chi_step = [&](const float2* Bt, int D, long long colsB, int k, int len, float2* Tright, float2* PnOut,
+                      int disc_slot) -> mpdo_status {
+    LCK(gram_rows_b(Bt, D, (int)colsB, D * colsB, G, B, s));
+    LST(eig_b(D, B, G, eE, s));
+    LCK(K_KEEP(eE.w, D, D, len, k, -1.0, 1, dk, ddisc + (size_t)B * disc_slot, B, s));
+    LCK(K_FACTOR(eE.V, D, (long long)D * D, eE.w, D, D, D, k, -0.25, SPLIT_FLOOR, 1, FR, D, (long long)k * D, B, s));
+    LCK(mmb(g_tensor_prec, k, (int)colsB, D, FR, D, (long long)k * D, Bt, colsB, D * colsB, Tright, colsB,
+            k * colsB, B, s));
+    LCK(K_FACTOR(eE.V, D, (long long)D * D, eE.w, D, D, D, k, 0.25, SPLIT_FLOOR, 0, PnOut, k, (long long)D * k, B, s));
+    return MPDO_OK;
+  };
+
+  // Bcur: the current explicit tensor of the site right of the next step
+  const float2* Bcur = nullptr;
+  int pidx = (int)pairs.size() - 1;
+  int kP = 1;   // columns of Pn
+  bool havePn = false;
+  for (int j = n - 1; j >= 0;) {
+    if (P.role[j] == 0) {
+      const Dims& c = P.chi_out[j];
+      // current tensor of site j: [l_qr][2][a][kright]
+      const int lq = (j == 0) ? 1 : ((P.role[j - 1] == 2 || P.role[j - 1] == 0) ? P.kq[j - 1] : 1);
+      if (j == n - 1) {
+        Bcur = Mlast;
+      } else {
+        // T_j^{cur} = A_j Pn : [(lq 2 a) x kq] [kq x kP]
+        const long long rowsA = (long long)lq * 2 * c.a;
+        LCK(mmb(g_tensor_prec, (int)rowsA, kP, P.kq[j], Astore[j], P.kq[j], rowsA * P.kq[j], Pn, kP,
+                (long long)P.kq[j] * kP, Tcur2, kP, rowsA * kP, B, s));
+        Bcur = Tcur2;
+      }
+      if (j == 0) {
+        LST(kappa_site(0, Bcur));
+        break;
+      }
+      const long long colsB = 2LL * c.a * c.r;
+      LST(chi_step(Bcur, lq, colsB, c.l, P.step_len[j - 1], chi_target(j), Pn2, j - 1));
+      LST(kappa_site(j, chi_target(j)));
+      std::swap(Pn, Pn2);
+      kP = c.l;
+      havePn = true;
+      j -= 1;
+      continue;
+    }
+    // pair (i, j = i+1)
+    const int i = j - 1;
+    PairStore& ps = pairs[pidx--];
+    const Dims& cmax = P.chi_out[j];
+    const Dims& cmin = P.chi_out[i];
+    const int kp = cmax.l, kr = cmax.r, a1 = P.a1[i].a, a2 = P.a1[j].a, r = P.in[j].r;
+    const int c3 = a2 * kr;
+    // right projector in the original bond: P = W_R L_R^{-1/2} Pn (r x kr), or the identity
+    const float2* Qright;   // Q2 P : [p2][a2][kr] = [p2 x c3]
+    if (ps.has_R) {
+      LCK(K_FACTOR(ps.WR, r, (long long)r * r, ps.lamR, r, r, r, r, -0.5, FLOOR, 0, PW, r, (long long)r * r, B, s));
+      LCK(mmb(g_tensor_prec, r, kr, r, PW, r, (long long)r * r, Pn, kP, (long long)r * kP, Pp, kr, (long long)r * kr, B,
+              s));
+      const float2* Q2p = ps.Q2;
+      long long bsQ2 = (long long)ps.p2 * ps.cols2;
+      if (ps.q2id) {   // Q2 = identity (p2 = cols2): one shared copy
+        LCK(K_EYE(Zb, ps.p2, ps.cols2, 0, 1, s));
+        Q2p = Zb;
+        bsQ2 = 0;
+      }
+      LCK(mmb(g_tensor_prec, ps.p2 * a2, kr, r, Q2p, r, bsQ2, Pp, kr, (long long)r * kr, Q2P, kr,
+              (long long)ps.p2 * a2 * kr, B, s));
+      Qright = Q2P;
+    } else {
+      // last pair: r = kr = 1, Q2 P = Q2
+      if (ps.q2id) {
+        LCK(K_EYE(Q2P, ps.p2, c3, (long long)ps.p2 * c3, B, s));
+      } else {
+        LCK(K_COPY(Q2P, ps.Q2, sizeof(float2) * (size_t)ps.p2 * c3 * B, s));
+      }
+      Qright = Q2P;
+    }
+    // LQ of Q2P: = L3 Q3 with orthonormal rows
+    int p3;
+    const float2* L3p;
+    const float2* Q3p;
+    long long bsL3;
+    if (c3 <= ps.p2) {
+      p3 = c3;
+      L3p = Qright;
+      bsL3 = (long long)ps.p2 * c3;
+      Q3p = nullptr;
+    } else {
+      p3 = ps.p2;
+      LST(lq_b(Qright, ps.p2, c3, Q3, L3, B, qw, s));
+      L3p = L3;
+      bsL3 = (long long)ps.p2 * ps.p2;
+      Q3p = Q3;
+    }
+    // the pair core again, then C'' = C_v [(2p1 2K2) x p2] L3 [p2 x p3]
+    const long long szC = 2LL * ps.p1 * 2 * K2 * ps.p2, szC2 = 2LL * ps.p1 * 2 * K2 * p3;
+    LST(make_core(ps, pidx + 1));
+    LCK(mmb(g_tensor_prec, 2 * ps.p1 * 2 * K2, p3, ps.p2, Cu, ps.p2, szC, L3p, p3, bsL3, C2, p3, szC2, B, s));
+    // core Gram and truncated SVD
+    const int nC = 2 * ps.p1;
+    const long long colsC = 2LL * K2 * p3;
+    LCK(gram_rows_b(C2, nC, (int)colsC, szC2, G, B, s));
+    LST(eig_b(nC, B, G, eE, s));
+    LCK(K_KEEP(eE.w, nC, nC, P.step_len[i], kp, -1.0, 1, dk, ddisc + (size_t)B * i, B, s));
+    // new min site: (Q1 (x) I) U sqrt(S) -> (kprev, a1, s1, kp) -> (kprev, s1, a1, kp)
+    LCK(K_FACTOR(eE.V, nC, (long long)nC * nC, eE.w, nC, nC, nC, kp, 0.25, SPLIT_FLOOR, 0, FL, kp, (long long)nC * kp, B, s));
+    const float2* Zs = FL;
+    long long bsZ = (long long)nC * kp;
+    if (!ps.q1id) {
+      LCK(mmb(g_tensor_prec, ps.rows1, 2 * kp, ps.p1, ps.Q1, ps.p1, (long long)ps.rows1 * ps.p1, FL, 2 * kp,
+              (long long)nC * kp, Tmin, 2 * kp, (long long)ps.rows1 * 2 * kp, B, s));
+      Zs = Tmin;
+      bsZ = (long long)ps.rows1 * 2 * kp;
+    }
+    {
+      int dims[4] = {ps.kprev, a1, 2, kp}, perm[4] = {0, 2, 1, 3};
+      LCK(K_PERMUTE(Zs, Tcur2, 4, dims, perm, s, B, bsZ));
+    }
+    // new max site: S^{-1/2} U^H C'' (I (x) Q3) -> (kp, s2, K2, a2, kr) -> (kp, s2, a2, K2, kr)
+    LCK(K_FACTOR(eE.V, nC, (long long)nC * nC, eE.w, nC, nC, nC, kp, -0.25, SPLIT_FLOOR, 1, FR, nC, (long long)kp * nC, B, s));
+    LCK(mmb(g_tensor_prec, kp, (int)colsC, nC, FR, nC, (long long)kp * nC, C2, colsC, szC2, Wm, colsC,
+            kp * colsC, B, s));
+    const float2* Ts = Wm;
+    long long bsT = kp * colsC;
+    if (Q3p) {
+      LCK(mmb(g_tensor_prec, kp * 2 * K2, c3, p3, Wm, p3, kp * colsC, Q3p, c3, (long long)p3 * c3, Tm, c3,
+              (long long)kp * 2 * K2 * c3, B, s));
+      Ts = Tm;
+      bsT = (long long)kp * 2 * K2 * c3;
+    }
+    if (kapplies(j)) {   // (kp, s2, K2, a2, kr) -> (kp, s2, kr, a2, K2): inner index last for the kappa cut
+      int dims[5] = {kp, 2, K2, a2, kr}, perm[5] = {0, 1, 4, 3, 2};
+      LCK(K_PERMUTE(Ts, Mp, 5, dims, perm, s, B, bsT));
+      LST(kappa_on(j, Mp));
+    } else {             // (kp, s2, K2, a2, kr) -> (kp, s2, a2, K2, kr)
+      int dims[5] = {kp, 2, K2, a2, kr}, perm[5] = {0, 1, 3, 2, 4};
+      LCK(K_PERMUTE(Ts, (float2*)out[j].data, 5, dims, perm, s, B, bsT));
+    }
+    // step at bond (i-1 | i) on the new min-site tensor
+    if (i == 0) {
+      LST(kappa_site(0, Tcur2));
+      break;
+    }
+    const long long colsB = 2LL * a1 * kp;
+    LST(chi_step(Tcur2, ps.kprev, colsB, cmin.l, P.step_len[i - 1], chi_target(i), Pn2, i - 1));
+    LST(kappa_site(i, chi_target(i)));
+    std::swap(Pn, Pn2);
+    kP = cmin.l;
+    havePn = true;
+    j = i - 1;
+  }
+  (void)havePn;
+  if (dry) return MPDO_OK;
+  // discarded weights
+  if (disc_host) {
+    std::vector<double> h((size_t)B * (nsteps + 2 * n));
+    HFITS(ddisc, h.size(), double, "discarded weights");
+    LCK(hipMemcpyAsync(h.data(), ddisc, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+    LCK(hipStreamSynchronize(s));
+    // device layout: slot-major [slot][B]; host layout [B][slot]: chi steps by bond, kappa cuts
+    // by site, gate-time cuts by pair min site
+    for (int b = 0; b < B; ++b)
+      for (int t = 0; t < nsteps + 2 * n; ++t) disc_host[(size_t)b * (nsteps + 2 * n) + t] = h[(size_t)t * B + b];
+  } else {
+    LCK(hipStreamSynchronize(s));
+  }
+  return MPDO_OK;
+}
+
+}  // namespace mpdo
+
+extern "C" {
+
+int mpdo_layer_dims(int n, const mpdo_site* in, int K1, int parity, int K2, int chi, int kappa, mpdo_site* out) {
+  if (!in || !out || n <= 0) return 0;
+  LayerPlan P = plan_layer(n, in, K1, parity, K2, chi, kappa);
+  if (!P.ok) {
+    layer_fail(MPDO_EINVAL, P.why.c_str());
+    return 0;
+  }
+  for (int i = 0; i < n; ++i) {
+    out[i].l = P.out[i].l;
+    out[i].a = P.out[i].a;
+    out[i].r = P.out[i].r;
+  }
+  return 1;
+}
+
+mpdo_status mpdo_layer_batched(int batch, int n, const mpdo_site* in, const float* kraus1, int K1, int parity,
+                               const float* gates2, int K2, int chi, int kappa, const mpdo_site* out,
+                               double* discarded, void* stream) {
+  if (batch <= 0 || n <= 0 || !in || !out || !kraus1 || !gates2) return layer_fail(MPDO_EINVAL, "null argument");
+  for (int i = 0; i < n; ++i)
+    if (!in[i].data || !out[i].data) return layer_fail(MPDO_EINVAL, "null site buffer");
+  LayerPlan P = plan_layer(n, in, K1, parity, K2, chi, kappa);
+  if (!P.ok) return layer_fail(MPDO_EINVAL, P.why.c_str());
+  for (int i = 0; i < n; ++i)
+    if (out[i].l != P.out[i].l || out[i].a != P.out[i].a || out[i].r != P.out[i].r)
+      return layer_fail(MPDO_EINVAL, "output descriptors must carry mpdo_layer_dims dims");
+  return run_layer(batch, P, in, kraus1, gates2, out, discarded, (hipStream_t)stream, false);
+}
+
+mpdo_status mpdo_debug_layer_dry_run(int batch, int n, const mpdo_site* in, int K1, int parity, int K2, int chi,
+                                     int kappa, const mpdo_site* out, unsigned long long* workspace_bytes) {
+  if (batch <= 0 || n <= 0 || !in || !out) return layer_fail(MPDO_EINVAL, "null argument");
+  LayerPlan P = plan_layer(n, in, K1, parity, K2, chi, kappa);
+  if (!P.ok) return layer_fail(MPDO_EINVAL, P.why.c_str());
+  for (int i = 0; i < n; ++i)
+    if (out[i].l != P.out[i].l || out[i].a != P.out[i].a || out[i].r != P.out[i].r)
+      return layer_fail(MPDO_EINVAL, "output descriptors must carry mpdo_layer_dims dims");
+  t_ws_bytes = 0;
+  mpdo_status st = run_layer(batch, P, in, nullptr, nullptr, out, nullptr, nullptr, true);
+  if (workspace_bytes) *workspace_bytes = t_ws_bytes;
+  return st;
+}
+
+}  // extern "C"

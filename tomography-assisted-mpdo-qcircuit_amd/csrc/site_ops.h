@@ -19,7 +19,22 @@ struct PermDesc {
 
 hipError_t apply_1q(int nsites, const Site1q* d_sites, const float2* d_kraus, long long max_elems,
                     hipStream_t s);
-hipError_t channel2q(const float2* Y, const float2* G, int K, int P1, int P2, float2* C, hipStream_t s);
+hipError_t channel2q(const float2* Y, const float2* G, int K, int P1, int P2, float2* C, hipStream_t s, int batch = 1,
+                     long long bsY = 0, long long bsC = 0);
+hipError_t block_diag_sum(const float2* M, int A, int R, double2* out, long long bsM, long long bsO, int batch,
+                          hipStream_t s);
+hipError_t block_diag_sum64(const double2* M, int A, int R, double2* out, long long bsM, long long bsO, int batch,
+                            hipStream_t s);
+hipError_t mask_cols_b(const double2* V, long long ldv, long long bsV, int rows, int cols, const int* k, float2* out,
+                       long long ldo, long long bsO, int batch, hipStream_t s);
+hipError_t widen(const float2* in, double2* out, long long n, long long bsI, long long bsO, int batch, hipStream_t s);
+hipError_t eye_b(float2* out, int rows, int cols, long long bs, int batch, hipStream_t s);
+hipError_t factor_b(const double2* V, long long ldv, long long bsV, const double* lam, long long bsL, int rows,
+                    int nev, int k, double power, double floor_rel, int trans, float2* out, long long ldo,
+                    long long bsO, int batch, hipStream_t s);
+hipError_t factor64_b(const double2* V, long long ldv, long long bsV, const double* lam, long long bsL, int rows,
+                      int nev, int k, double power, double floor_rel, int trans, double2* out, long long ldo,
+                      long long bsO, int batch, hipStream_t s);
 hipError_t permute(const float2* in, float2* out, int rank, const int* dims, const int* perm, hipStream_t s,
                    int batch = 1, long long bstride = 0);
 hipError_t keep_count(const double* lam, long long ldw, int nev, int len_s, int msv, double err, int relative,

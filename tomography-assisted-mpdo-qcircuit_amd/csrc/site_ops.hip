@@ -5,6 +5,14 @@
 
 namespace mpdo {
 
+static int grid1(long long tot, int cap = 8192) {
+  long long b = (tot + 255) / 256;
+  if (b > cap) b = cap;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+
 // ---------------------------------------------------------------------------------------
 // 1q gate / Kraus absorption (Circuit.py:138-178):
 //   T'[l, s', a*K + k, r] = sum_s E_k[s', s] T[l, s, a, r]
@@ -51,7 +59,10 @@ hipError_t apply_1q(int nsites, const Site1q* d_sites, const float2* d_kraus, lo
 // Two-qubit channel on the reduced core (Circuit.py:87-106):
 //   C[(p*2 + s1'), ((s2'*K + k)*P2 + q)] = sum_{s1,s2} G[s1', s2', s1, s2, k] Y[s1][s2][p][q]
 // ---------------------------------------------------------------------------------------
-__global__ void channel2q_kernel(const float2* Y, const float2* G, int K, int P1, int P2, float2* C) {
+__global__ void channel2q_kernel(const float2* Y, const float2* G, int K, int P1, int P2, float2* C, long long bsY,
+                                 long long bsC) {
+  Y += blockIdx.y * bsY;
+  C += blockIdx.y * bsC;
   const long long tot = (long long)P1 * P2;
   const long long ldc = 2LL * K * P2;
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
@@ -77,12 +88,118 @@ __global__ void channel2q_kernel(const float2* Y, const float2* G, int K, int P1
   }
 }
 
-hipError_t channel2q(const float2* Y, const float2* G, int K, int P1, int P2, float2* C, hipStream_t s) {
+hipError_t channel2q(const float2* Y, const float2* G, int K, int P1, int P2, float2* C, hipStream_t s, int batch,
+                     long long bsY, long long bsC) {
   long long tot = (long long)P1 * P2;
   int blocks = (int)((tot + 255) / 256);
   if (blocks > 8192) blocks = 8192;
   if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(channel2q_kernel, dim3(blocks), dim3(256), 0, s, Y, G, K, P1, P2, C);
+  hipLaunchKernelGGL(channel2q_kernel, dim3(blocks, batch), dim3(256), 0, s, Y, G, K, P1, P2, C, bsY, bsC);
+  return hipGetLastError();
+}
+
+// out[b][r][r'] = sum_a M[b][(a R + r)][(a R + r')]  (sum of the diagonal R x R blocks; complex64 in,
+// complex128 out)
+__global__ void block_diag_sum_kernel(const float2* M, int A, int R, double2* out, long long bsM, long long bsO) {
+  M += blockIdx.y * bsM;
+  out += blockIdx.y * bsO;
+  const long long ld = (long long)A * R;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < (long long)R * R;
+       e += (long long)gridDim.x * blockDim.x) {
+    int r = (int)(e / R), q = (int)(e % R);
+    double2 acc = make_double2(0, 0);
+    for (int a = 0; a < A; ++a) {
+      float2 v = M[((long long)a * R + r) * ld + (long long)a * R + q];
+      acc.x += v.x;
+      acc.y += v.y;
+    }
+    out[e] = acc;
+  }
+}
+
+hipError_t block_diag_sum(const float2* M, int A, int R, double2* out, long long bsM, long long bsO, int batch,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(block_diag_sum_kernel, dim3(grid1((long long)R * R), batch), dim3(256), 0, s, M, A, R, out, bsM,
+                     bsO);
+  return hipGetLastError();
+}
+
+// the same with a complex128 input
+__global__ void block_diag_sum64_kernel(const double2* M, int A, int R, double2* out, long long bsM, long long bsO) {
+  M += blockIdx.y * bsM;
+  out += blockIdx.y * bsO;
+  const long long ld = (long long)A * R;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < (long long)R * R;
+       e += (long long)gridDim.x * blockDim.x) {
+    int r = (int)(e / R), q = (int)(e % R);
+    double2 acc = make_double2(0, 0);
+    for (int a = 0; a < A; ++a) {
+      double2 v = M[((long long)a * R + r) * ld + (long long)a * R + q];
+      acc.x += v.x;
+      acc.y += v.y;
+    }
+    out[e] = acc;
+  }
+}
+
+hipError_t block_diag_sum64(const double2* M, int A, int R, double2* out, long long bsM, long long bsO, int batch,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(block_diag_sum64_kernel, dim3(grid1((long long)R * R), batch), dim3(256), 0, s, M, A, R, out,
+                     bsM, bsO);
+  return hipGetLastError();
+}
+
+// out[b][i][j] = V[b][i][j] for j < k[b] (a per-matrix count held on the device), else 0;
+// complex128 in, complex64 out
+__global__ void mask_cols_kernel(const double2* V, long long ldv, long long bsV, int rows, int cols, const int* k,
+                                 float2* out, long long ldo, long long bsO) {
+  const int b = blockIdx.y;
+  V += b * bsV;
+  out += b * bsO;
+  const int kb = k[b];
+  const long long tot = (long long)rows * cols;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot; e += (long long)gridDim.x * blockDim.x) {
+    int i = (int)(e / cols), j = (int)(e % cols);
+    double2 v = V[(long long)i * ldv + j];
+    out[(long long)i * ldo + j] = j < kb ? make_float2((float)v.x, (float)v.y) : make_float2(0.f, 0.f);
+  }
+}
+
+hipError_t mask_cols_b(const double2* V, long long ldv, long long bsV, int rows, int cols, const int* k, float2* out,
+                       long long ldo, long long bsO, int batch, hipStream_t s) {
+  hipLaunchKernelGGL(mask_cols_kernel, dim3(grid1((long long)rows * cols), batch), dim3(256), 0, s, V, ldv, bsV, rows,
+                     cols, k, out, ldo, bsO);
+  return hipGetLastError();
+}
+
+// complex64 -> complex128 copy of a batch of contiguous blocks
+__global__ void widen_kernel(const float2* in, double2* out, long long n, long long bsI, long long bsO) {
+  in += blockIdx.y * bsI;
+  out += blockIdx.y * bsO;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+    float2 v = in[e];
+    out[e] = make_double2(v.x, v.y);
+  }
+}
+
+hipError_t widen(const float2* in, double2* out, long long n, long long bsI, long long bsO, int batch, hipStream_t s) {
+  hipLaunchKernelGGL(widen_kernel, dim3(grid1(n), batch), dim3(256), 0, s, in, out, n, bsI, bsO);
+  return hipGetLastError();
+}
+
+// batched identity
+__global__ void eye_b_kernel(float2* out, int rows, int cols, long long bs) {
+  out += blockIdx.y * bs;
+  long long tot = (long long)rows * cols;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
+       e += (long long)gridDim.x * blockDim.x) {
+    int i = (int)(e / cols), j = (int)(e % cols);
+    out[e] = make_float2(i == j ? 1.f : 0.f, 0.f);
+  }
+}
+
+hipError_t eye_b(float2* out, int rows, int cols, long long bs, int batch, hipStream_t s) {
+  hipLaunchKernelGGL(eye_b_kernel, dim3(grid1((long long)rows * cols), batch), dim3(256), 0, s, out, rows, cols, bs);
   return hipGetLastError();
 }
 
@@ -90,7 +207,9 @@ hipError_t channel2q(const float2* Y, const float2* G, int K, int P1, int P2, fl
 // Permutation of a row-major tensor of rank <= 6 (complex64):
 //   out[idx permuted by perm] = in[idx]; out dims are dims[perm[0]], dims[perm[1]], ...
 // ---------------------------------------------------------------------------------------
-__global__ void permute_kernel(const float2* in, float2* out, PermDesc d, long long tot) {
+__global__ void permute_kernel(const float2* in, float2* out, PermDesc d, long long tot, long long bstride) {
+  in += blockIdx.y * bstride;
+  out += blockIdx.y * bstride;
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
        e += (long long)gridDim.x * blockDim.x) {
     // decompose the OUTPUT index, gather from the input (coalesced writes)
@@ -120,8 +239,11 @@ hipError_t permute(const float2* in, float2* out, int rank, const int* dims, con
   int blocks = (int)((tot + 255) / 256);
   if (blocks > 8192) blocks = 8192;
   if (blocks < 1) blocks = 1;
-  for (int b = 0; b < batch; ++b)
-    hipLaunchKernelGGL(permute_kernel, dim3(blocks), dim3(256), 0, s, in + b * bstride, out + b * bstride, d, tot);
+  for (int b0 = 0; b0 < batch; b0 += 65535) {
+    int nb = batch - b0 < 65535 ? batch - b0 : 65535;
+    hipLaunchKernelGGL(permute_kernel, dim3(blocks, nb), dim3(256), 0, s, in + b0 * bstride, out + b0 * bstride, d,
+                       tot, bstride);
+  }
   return hipGetLastError();
 }
 
@@ -172,9 +294,13 @@ hipError_t keep_count(const double* lam, long long ldw, int nev, int len_s, int 
 // j >= nev (padded zero singular values).
 // ---------------------------------------------------------------------------------------
 template <typename TO>
-__global__ void factor_kernel(const double2* V, long long ldv, const double* lam, int rows, int nev, int k,
-                              double power, double floor_rel, int trans, TO* out, long long ldo) {
+__global__ void factor_kernel(const double2* V, long long ldv, long long bsV, const double* lam, long long bsL,
+                              int rows, int nev, int k, double power, double floor_rel, int trans, TO* out,
+                              long long ldo, long long bsO) {
   const long long tot = (long long)rows * k;
+  V += blockIdx.y * bsV;
+  lam += blockIdx.y * bsL;
+  out += blockIdx.y * bsO;
   const double l0 = lam[0] > 0 ? lam[0] : 0.0;
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
        e += (long long)gridDim.x * blockDim.x) {
@@ -199,23 +325,31 @@ __global__ void factor_kernel(const double2* V, long long ldv, const double* lam
 
 hipError_t factor(const double2* V, long long ldv, const double* lam, int rows, int nev, int k, double power,
                   double floor_rel, int trans, float2* out, long long ldo, hipStream_t s) {
-  long long tot = (long long)rows * k;
-  if (tot <= 0) return hipSuccess;
-  int blocks = (int)((tot + 255) / 256);
-  if (blocks > 8192) blocks = 8192;
-  hipLaunchKernelGGL(factor_kernel<float2>, dim3(blocks), dim3(256), 0, s, V, ldv, lam, rows, nev, k, power,
-                     floor_rel, trans, out, ldo);
-  return hipGetLastError();
+  return factor_b(V, ldv, 0, lam, 0, rows, nev, k, power, floor_rel, trans, out, ldo, 0, 1, s);
 }
 
 hipError_t factor64(const double2* V, long long ldv, const double* lam, int rows, int nev, int k, double power,
                     double floor_rel, int trans, double2* out, long long ldo, hipStream_t s) {
+  return factor64_b(V, ldv, 0, lam, 0, rows, nev, k, power, floor_rel, trans, out, ldo, 0, 1, s);
+}
+
+hipError_t factor_b(const double2* V, long long ldv, long long bsV, const double* lam, long long bsL, int rows,
+                    int nev, int k, double power, double floor_rel, int trans, float2* out, long long ldo,
+                    long long bsO, int batch, hipStream_t s) {
   long long tot = (long long)rows * k;
-  if (tot <= 0) return hipSuccess;
-  int blocks = (int)((tot + 255) / 256);
-  if (blocks > 8192) blocks = 8192;
-  hipLaunchKernelGGL(factor_kernel<double2>, dim3(blocks), dim3(256), 0, s, V, ldv, lam, rows, nev, k, power,
-                     floor_rel, trans, out, ldo);
+  if (tot <= 0 || batch <= 0) return hipSuccess;
+  hipLaunchKernelGGL(factor_kernel<float2>, dim3(grid1(tot), batch), dim3(256), 0, s, V, ldv, bsV, lam, bsL, rows,
+                     nev, k, power, floor_rel, trans, out, ldo, bsO);
+  return hipGetLastError();
+}
+
+hipError_t factor64_b(const double2* V, long long ldv, long long bsV, const double* lam, long long bsL, int rows,
+                      int nev, int k, double power, double floor_rel, int trans, double2* out, long long ldo,
+                      long long bsO, int batch, hipStream_t s) {
+  long long tot = (long long)rows * k;
+  if (tot <= 0 || batch <= 0) return hipSuccess;
+  hipLaunchKernelGGL(factor_kernel<double2>, dim3(grid1(tot), batch), dim3(256), 0, s, V, ldv, bsV, lam, bsL, rows,
+                     nev, k, power, floor_rel, trans, out, ldo, bsO);
   return hipGetLastError();
 }
 

@@ -53,11 +53,14 @@ def load() -> ctypes.CDLL:
         'mpdo_reduced_dm': (I, [I, SP, P(I), VP, VP]),
         'mpdo_trace_rho_rho': (I, [I, SP, SP, P(D), VP]),
         'mpdo_contract_state': (I, [I, SP, VP, VP]),
+        'mpdo_layer_dims': (I, [I, SP, I, I, I, I, I, SP]),
+        'mpdo_layer_batched': (I, [I, I, SP, P(ctypes.c_float), I, I, P(ctypes.c_float), I, I, I, SP, P(D), VP]),
         'mpdo_debug_gemm': (I, [I, I, I, I, I, VP, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, I,
                                 VP, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, I,
                                 VP, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong,
                                 D, D, D, D, VP]),
         'mpdo_debug_eigh': (I, [I, I, VP, P(D), VP, P(I), VP]),
+        'mpdo_debug_layer_dry_run': (I, [I, I, SP, I, I, I, I, I, SP, P(ctypes.c_ulonglong)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -273,6 +276,58 @@ def trace_rho_rho(a: Sequence[torch.Tensor], b: Sequence[torch.Tensor]) -> float
     return out.value
 
 
+def layer_dims(dims: Sequence[Tuple[int, int, int]], K1: int, parity: int, K2: int, chi: Optional[int],
+               kappa: Optional[int]):
+    """Output (l, a, r) of mpdo_layer_batched, or None with the reason when the layer does not
+    qualify for the fused path."""
+    lib = load()
+    n = len(dims)
+    ins = (Site * n)()
+    for i, (l, a, r) in enumerate(dims):
+        ins[i] = Site(None, l, a, r)
+    outs = (Site * n)()
+    ok = lib.mpdo_layer_dims(n, ins, K1, parity, K2, int(chi or 0), int(kappa or 0), outs)
+    if not ok:
+        return None, lib.mpdo_last_error().decode(errors='replace')
+    return [(o.l, o.a, o.r) for o in outs], ''
+
+
+def layer_batched(sites: Sequence[torch.Tensor], kraus1: torch.Tensor, parity: int, gates2: torch.Tensor,
+                  chi: Optional[int], kappa: Optional[int]):
+    """One fused brickwork layer. sites[i]: (B, l, 2, a, r) complex64 on the device;
+    kraus1: (B, n, K1, 2, 2); gates2: (npairs, 2, 2, 2, 2, K2). Returns (new sites, discarded
+    (B, 3n) numpy: chi steps by bond, kappa cuts by site, gate-time splits by pair min site)."""
+    import numpy as np
+    lib = load()
+    n = len(sites)
+    B = sites[0].shape[0]
+    dev = sites[0].device
+    dims = [(t.shape[1], t.shape[3], t.shape[4]) for t in sites]
+    K1, K2 = kraus1.shape[2], gates2.shape[-1]
+    out_dims, why = layer_dims(dims, K1, parity, K2, chi, kappa)
+    if out_dims is None:
+        raise ValueError(f'layer does not qualify for the fused path: {why}')
+    ins = (Site * n)()
+    for i, t in enumerate(sites):
+        _require_device(t)
+        if not t.is_contiguous():
+            raise ValueError('batched site tensors must be contiguous')
+        ins[i] = Site(t.data_ptr(), dims[i][0], dims[i][1], dims[i][2])
+    outs_t = [torch.empty(B, l, 2, a, r, dtype=torch.complex64, device=dev) for (l, a, r) in out_dims]
+    outs = (Site * n)()
+    for i, (t, (l, a, r)) in enumerate(zip(outs_t, out_dims)):
+        outs[i] = Site(t.data_ptr(), l, a, r)
+    k1 = kraus1.detach().to(torch.complex64).cpu().contiguous().view(torch.float32).reshape(-1).contiguous()
+    g2 = gates2.detach().to(torch.complex64).cpu().contiguous().view(torch.float32).reshape(-1).contiguous()
+    disc = np.zeros((B, 3 * n), dtype=np.float64)
+    st = lib.mpdo_layer_batched(B, n, ins, ctypes.cast(k1.data_ptr(), ctypes.POINTER(ctypes.c_float)), K1, parity,
+                                ctypes.cast(g2.data_ptr(), ctypes.POINTER(ctypes.c_float)), K2, int(chi or 0),
+                                int(kappa or 0), outs, disc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                _stream(dev))
+    check(st, 'mpdo_layer_batched')
+    return outs_t, disc
+
+
 def contract_state(sites: Sequence[torch.Tensor]) -> torch.Tensor:
     lib = load()
     n = len(sites)
@@ -307,3 +362,27 @@ def debug_eigh(A: torch.Tensor):
                               ctypes.c_void_p(V.data_ptr()), ctypes.byref(sw), _stream(A.device)),
           'mpdo_debug_eigh')
     return w, V, sw.value
+
+
+def layer_dry_run(batch: int, dims: Sequence[Tuple[int, int, int]], K1: int, parity: int, K2: int, chi: Optional[int],
+                  kappa: Optional[int]) -> int:
+    """Host-only rehearsal of mpdo_layer_batched at these shapes (every launch's operand extents
+    checked against its buffers, nothing launched). Returns the device workspace in bytes;
+    raises MPDOError on a failed check or an unplannable layer."""
+    lib = load()
+    n = len(dims)
+    out_dims, why = layer_dims(dims, K1, parity, K2, chi, kappa)
+    if out_dims is None:
+        raise ValueError(f'layer does not qualify for the fused path: {why}')
+    ins, outs = (Site * n)(), (Site * n)()
+    addr = 1 << 40
+    for i, ((l, a, r), (lo, ao, ro)) in enumerate(zip(dims, out_dims)):
+        ins[i] = Site(ctypes.c_void_p(addr), l, a, r)
+        addr += ((8 * batch * l * 2 * a * r) | 4095) + 1
+        outs[i] = Site(ctypes.c_void_p(addr), lo, ao, ro)
+        addr += ((8 * batch * lo * 2 * ao * ro) | 4095) + 1
+    ws = ctypes.c_ulonglong(0)
+    check(lib.mpdo_debug_layer_dry_run(batch, n, ins, K1, parity, K2, int(chi or 0), int(kappa or 0), outs,
+                                       ctypes.byref(ws)), 'mpdo_debug_layer_dry_run')
+    return int(ws.value)
+

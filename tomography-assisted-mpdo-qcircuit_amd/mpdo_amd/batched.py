@@ -1,0 +1,279 @@
+"""Evolution of a family of circuits with one structure over many MPDO chains at once.
+
+A family is a list of ``TensorCircuit`` objects with the same qubit count, truncation settings,
+noise model and layer list (same gate types on the same qubits; parameters may differ per copy),
+for instance the perturbed copies of one ansatz. ``evolve_batched`` gives the same result per
+copy as ``TensorCircuit.evolve`` (Circuit.py:469-491), with one difference in what it runs:
+
+every circuit-layer of the brickwork form
+
+    single-qubit gates ... | one two-qubit gate on every pair (i, i+1), i = p, p+2, ... | truncate
+
+whose truncation finds the chain connected (TNNOptimizer.py:51-61) runs as ONE call of
+``mpdo_layer_batched`` over all copies. That entry point applies the composed single-qubit
+channel of every site, the two-qubit channel of every pair and the truncation (QR sweep, chi
+sweep, kappa cut) without forming the two-site tensors of the gate-time split (their bond is
+up to 4 chi), including that split's cut of singular values below ``GLOBAL_MINIMUM``
+(Circuit.py:22,110-124), and makes the truncation decisions of the per-gate path (same kept
+counts, shapes and discarded weights, ``('gate', ...)`` entries included). Every other layer runs
+per copy through the per-gate path.
+"""
+from __future__ import annotations
+
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import gates as G
+from .circuit import TensorCircuit
+from .node import SiteNode
+from .noise import choi_compress, compose_kraus
+
+__all__ = ['evolve_batched', 'FusedLayer', 'fused_layer_of', 'steady_dims']
+
+
+class FusedLayer:
+    """One brickwork circuit-layer in the operands of ``mpdo_layer_batched``."""
+
+    __slots__ = ('parity', 'kraus1', 'noisy1', 'gates2', 'noisy2')
+
+    def __init__(self, parity: int, kraus1: torch.Tensor, noisy1: List[bool], gates2: torch.Tensor, noisy2: bool):
+        self.parity = parity        # first pair (parity, parity + 1)
+        self.kraus1 = kraus1        # (B, n, K1, 2, 2) complex128, operator k as [out][in]
+        self.noisy1 = noisy1        # per site: a noisy single-qubit channel (the site gains I_i)
+        self.gates2 = gates2        # (npairs, 2, 2, 2, 2, K2) complex128 (o_min, o_max, i_min, i_max, k)
+        self.noisy2 = noisy2        # two-qubit tensors carry a Kraus axis (the max site gains I_i)
+
+
+def _segments(c: TensorCircuit) -> List[Tuple[int, int]]:
+    """[start, stop) index ranges of the layer list, each ending after a truncation (the last one
+    possibly without)."""
+    out, start = [], 0
+    for i, layer in enumerate(c.layers):
+        if 'truncate' in layer.name.lower():
+            out.append((start, i + 1))
+            start = i + 1
+    if start < len(c.layers):
+        out.append((start, len(c.layers)))
+    return out
+
+
+def _same_structure(cs: Sequence[TensorCircuit]) -> Optional[str]:
+    c0 = cs[0]
+    for c in cs[1:]:
+        if (c.qnumber, c.chi, c.kappa, c.max_truncation_err, c.ideal, c.noiseType, str(c.device)) != \
+                (c0.qnumber, c0.chi, c0.kappa, c0.max_truncation_err, c0.ideal, c0.noiseType, str(c0.device)):
+            return 'circuits differ in qubit count, truncation settings, noise or device'
+        if len(c.layers) != len(c0.layers) or c._oqs_list != c0._oqs_list:
+            return 'circuits differ in their layer lists'
+        for a, b in zip(c.layers, c0.layers):
+            if type(a) is not type(b):
+                return 'circuits differ in their gate types'
+    return None
+
+
+def fused_layer_of(cs: Sequence[TensorCircuit], start: int, stop: int) -> Tuple[Optional[FusedLayer], str]:
+    """The fused operands of layer-list entries [start, stop) of every copy, or (None, reason) when
+    the entries are not one brickwork circuit-layer with a truncation at the end."""
+    c0 = cs[0]
+    n = c0.qnumber
+    if c0.chi is None or c0.max_truncation_err is not None:
+        return None, 'fused layers need a chi cap and no error threshold'
+    if 'truncate' not in c0.layers[stop - 1].name.lower():
+        return None, 'no truncation at the end of the layer'
+    singles: List[List[int]] = []     # layer indices of single-qubit gates per qubit
+    pairs: List[Tuple[int, int]] = []  # (min qubit, layer index)
+    singles = [[] for _ in range(n)]
+    seen2 = set()
+    for i in range(start, stop - 1):
+        layer = c0.layers[i]
+        name = layer.name.lower()
+        if 'barrier' in name:
+            continue
+        if 'truncate' in name:
+            return None, 'truncation inside the layer'
+        if not isinstance(layer, G.QuantumGate) or layer.name == 'MeasureZ':
+            return None, f'entry {layer.name} is not a gate'
+        oqs = c0._oqs_list[i]
+        if layer.single:
+            for q in oqs:
+                if q in seen2:
+                    return None, 'single-qubit gate after a two-qubit gate on its qubit'
+                singles[q].append(i)
+        else:
+            if len(oqs) != 2 or abs(oqs[0] - oqs[1]) != 1:
+                return None, 'two-qubit gate on non-neighbouring qubits'
+            mn = min(oqs)
+            if mn in seen2 or mn + 1 in seen2:
+                return None, 'two two-qubit gates on one qubit'
+            seen2.update((mn, mn + 1))
+            pairs.append((mn, i))
+    if not pairs:
+        return None, 'no two-qubit gates'
+    parity = min(p for p, _ in pairs) % 2
+    want = list(range(parity, n - 1, 2))
+    if sorted(p for p, _ in pairs) != want:
+        return None, 'two-qubit gates do not cover every pair of one parity'
+    pairs.sort()
+    # two-qubit tensors: shared by all copies
+    gl, noisy2 = [], None
+    for mn, i in pairs:
+        Gt = c0._double_kraus(c0.layers[i], c0._oqs_list[i])
+        for c in cs[1:]:
+            if not torch.equal(c._double_kraus(c.layers[i], c._oqs_list[i]), Gt):
+                return None, 'two-qubit channels differ between copies'
+        d5 = Gt.dim() == 5
+        if noisy2 is not None and d5 != noisy2:
+            return None, 'mixed noisy and ideal two-qubit gates'
+        noisy2 = d5
+        gl.append(Gt if d5 else Gt.unsqueeze(-1))
+    K2 = gl[0].shape[-1]
+    if any(g.shape[-1] != K2 for g in gl):
+        return None, 'two-qubit channels of different Kraus rank'
+    gates2 = torch.stack(gl)
+    # single-qubit channels per copy and site
+    ks, noisy1 = [], [False] * n
+    K1 = None
+    for c in cs:
+        row = []
+        for q in range(n):
+            E, nz = None, False
+            for i in singles[q]:
+                Ei, gi = c._single_kraus(c.layers[i])
+                E = Ei if E is None else compose_kraus(E, Ei)
+                nz = nz or gi
+            if E is None:
+                E = torch.eye(2, dtype=torch.complex128)
+            if E.dim() == 3 and E.shape[2] > 1 and c.fuse:
+                E = choi_compress(E)
+            if E.dim() == 2:
+                E = E.unsqueeze(-1)
+            noisy1[q] = noisy1[q] or nz
+            if K1 is None:
+                K1 = E.shape[2]
+            if E.shape[2] != K1:
+                return None, 'single-qubit channels of different Kraus rank'
+            row.append(E.permute(2, 0, 1))
+        ks.append(torch.stack(row))
+    return FusedLayer(parity, torch.stack(ks), noisy1, gates2, bool(noisy2)), ''
+
+
+def _stack_sites(states: Sequence[Sequence[SiteNode]], cache) -> List[torch.Tensor]:
+    """(B, l, 2, a, r) per site; reuses the batched output of the previous fused layer when every
+    copy still holds its views."""
+    B, n = len(states), len(states[0])
+    if cache is not None:
+        outs, views = cache
+        if all(states[b][i].tensor is views[b][i] for b in range(B) for i in range(n)):
+            return outs
+    return [torch.stack([states[b][i].tensor for b in range(B)]).contiguous() for i in range(n)]
+
+
+def evolve_batched(circuits: Sequence[TensorCircuit], states: Sequence[List[SiteNode]], *, fuse_layers: bool = True,
+                   stats: Optional[dict] = None) -> None:
+    """Evolve ``states[b]`` under ``circuits[b]`` for every copy b, in place (see the module
+    docstring). ``stats`` (optional dict) receives the count of fused and per-gate layers."""
+    cs = list(circuits)
+    if not cs or len(cs) != len(states):
+        raise ValueError('one state per circuit is required')
+    why = _same_structure(cs)
+    if why:
+        raise ValueError(why)
+    for c, st in zip(cs, states):
+        c._begin(st)
+    n = cs[0].qnumber
+    B = len(cs)
+    cache = None
+    nf = ng = 0
+    for start, stop in _segments(cs[0]):
+        layer, reason = (fused_layer_of(cs, start, stop) if fuse_layers else (None, 'disabled'))
+        if layer is not None:
+            ok, reason = _fusable_state(cs, states, layer)
+            if not ok:
+                layer = None
+        if layer is None:
+            for c, st in zip(cs, states):
+                for i in range(start, stop):
+                    c._step(st, i)
+            cache = None
+            ng += 1
+            continue
+        c0 = cs[0]
+        sites = _stack_sites(states, cache)
+        kappa = c0.kappa if (not c0.ideal and c0.kappa is not None) else None
+        outs, disc = _lib.layer_batched(sites, layer.kraus1, layer.parity, layer.gates2, c0.chi, kappa)
+        a_before = [sites[i].shape[3] * layer.kraus1.shape[2] for i in range(n)]
+        for i in range(layer.parity, n - 1, 2):
+            a_before[i + 1] *= layer.gates2.shape[-1]
+        views = [[outs[i][b] for i in range(n)] for b in range(B)]
+        for b, (c, st) in enumerate(zip(cs, states)):
+            for i in range(n):
+                nd = st[i]
+                nd.tensor = views[b][i]
+                if layer.noisy1[i]:
+                    nd.has_I = True
+            for i in range(layer.parity, n - 1, 2):
+                st[i].bond_right = True
+                st[i + 1].bond_left = True
+                if layer.noisy2:
+                    st[i + 1].has_I = True
+            if c.record_discarded:
+                c.discarded.extend(('gate', i, float(disc[b, 2 * n + i])) for i in range(layer.parity, n - 1, 2))
+                c.discarded.extend(('chi', i + 1, float(disc[b, i])) for i in reversed(range(n - 1)))
+                if kappa is not None:
+                    c.discarded.extend(('kappa', i, float(disc[b, n + i])) for i in range(n)
+                                       if st[i].has_I and a_before[i] > kappa)
+        cache = (outs, views)
+        nf += 1
+    for c, st in zip(cs, states):
+        c._finish(st)
+    if stats is not None:
+        stats['fused_layers'] = stats.get('fused_layers', 0) + nf
+        stats['per_gate_layers'] = stats.get('per_gate_layers', 0) + ng
+
+
+def steady_dims(n: int, chi: int, kappa: Optional[int], K1: int, K2: int, parity: int, max_layers: int = 64):
+    """Site shapes (l, a, r) that a long brickwork circuit (alternating pair parity, K1/K2 Kraus
+    operators per single-/two-qubit channel) has on entry to a layer of the given parity: the
+    fixed point of the layer's shape rule (mpdo_layer_dims), iterated from inner dim kappa and
+    bonds min(chi, 4^distance to the nearer end)."""
+    k = kappa or 1
+    bonds = [min(chi, 4 ** (i + 1), 4 ** (n - 1 - i)) for i in range(n - 1)]
+    dims = [(1 if i == 0 else bonds[i - 1], k, 1 if i == n - 1 else bonds[i]) for i in range(n)]
+    par = 1 - parity
+    hist = []
+    for _ in range(max_layers):
+        out, why = _lib.layer_dims(dims, K1, par, K2, chi, kappa)
+        if out is None:
+            raise ValueError(f'no fused steady state: {why}')
+        dims, par = out, 1 - par
+        if par == parity:
+            if hist and hist[-1] == dims:
+                return dims
+            hist.append(dims)
+    raise ValueError('shapes did not settle')
+
+
+def _fusable_state(cs, states, layer: FusedLayer) -> Tuple[bool, str]:
+    n = cs[0].qnumber
+    ref = states[0]
+    for st in states[1:]:
+        for a, b in zip(st, ref):
+            if a.tensor.shape != b.tensor.shape or (a.has_I, a.bond_left, a.bond_right) != \
+                    (b.has_I, b.bond_left, b.bond_right):
+                return False, 'copies differ in site shapes or flags'
+    pair_min = set(range(layer.parity, n - 1, 2))
+    for i in range(n - 1):
+        if not (ref[i].bond_right or i in pair_min):
+            return False, 'the chain is not connected at the truncation'
+    for nd in ref:
+        if nd.tensor.shape[2] > 1 and not nd.has_I:
+            return False, 'inner index without the I flag'
+    dims = [tuple(nd.tensor.shape[i] for i in (0, 2, 3)) for nd in ref]
+    kappa = cs[0].kappa if (not cs[0].ideal and cs[0].kappa is not None) else None
+    out, why = _lib.layer_dims(dims, layer.kraus1.shape[2], layer.parity, layer.gates2.shape[-1], cs[0].chi, kappa)
+    if out is None:
+        return False, why
+    return True, ''

@@ -357,12 +357,18 @@ class TensorCircuit(QuantumCircuit):
             return self.noiseTensorDict.setdefault(gate.name, self.Noise.single_kraus(G1))
         return self.Noise.single_kraus(G1)
 
-    def _apply_single_qubit_gate(self, _qNodes: List[SiteNode], gate, _oqs: List[int]):
-        """Circuit.py:138-178."""
+    def _single_kraus(self, gate) -> Tuple[torch.Tensor, bool]:
+        """Kraus tensor (2, 2[, K]) a single-qubit gate applies (Circuit.py:140-155) and whether it
+        carries noise (then the site gains the inner index I_i)."""
         gnoise = (self.idealNoise or self.unified) and not gate.ideal
         E = self._noisy_single(gate) if gnoise else gate.tensor.to(torch.complex128)
         if E.dim() not in (2, 3) or tuple(E.shape[:2]) != (2, 2):
             raise ValueError(f'single-qubit gate tensor of shape {tuple(E.shape)}')
+        return E, gnoise
+
+    def _apply_single_qubit_gate(self, _qNodes: List[SiteNode], gate, _oqs: List[int]):
+        """Circuit.py:138-178."""
+        E, gnoise = self._single_kraus(gate)
         if self.fuse:
             for q in _oqs:
                 prev = self._pending[q]
@@ -394,12 +400,12 @@ class TensorCircuit(QuantumCircuit):
                 _qNodes[q].has_I = True
             self._pending[q] = None
 
-    def _apply_two_qubits_gate(self, _qNodes: List[SiteNode], gate, _oqs: List[int]):
-        """Circuit.py:74-136 for adjacent qubits."""
+    def _double_kraus(self, gate, _oqs: List[int]) -> torch.Tensor:
+        """Tensor (o_min, o_max, i_min, i_max[, K]) a two-qubit gate applies to neighbouring qubits
+        (Circuit.py:80-106), oriented so that the lower qubit index comes first."""
         if len(_oqs) != 2 or _oqs[0] == _oqs[1]:
             raise ValueError('Invalid operating qubits for a two-qubit gate.')
-        mx, mn = max(_oqs), min(_oqs)
-        if mx - mn != 1:
+        if max(_oqs) - min(_oqs) != 1:
             raise NotImplementedError('two-qubit gates act on neighbouring qubits of the chain')
         gnoise = (self.idealNoise and not gate.ideal) or self.realNoise
         Gt = gate.tensor.to(torch.complex128)
@@ -413,6 +419,12 @@ class TensorCircuit(QuantumCircuit):
             raise ValueError(f'two-qubit gate tensor of shape {tuple(Gt.shape)}')
         if _oqs[0] > _oqs[1]:
             Gt = Gt.transpose(0, 1).transpose(2, 3)
+        return Gt
+
+    def _apply_two_qubits_gate(self, _qNodes: List[SiteNode], gate, _oqs: List[int]):
+        """Circuit.py:74-136 for adjacent qubits."""
+        Gt = self._double_kraus(gate, _oqs)
+        mx, mn = max(_oqs), min(_oqs)
         self._flush(_qNodes, [mn, mx])
         tmin, tmax = _qNodes[mn].tensor.contiguous(), _qNodes[mx].tensor.contiguous()
         for attempt in range(MAX_ATTEMPTS):
@@ -471,9 +483,7 @@ class TensorCircuit(QuantumCircuit):
         if self.record_discarded and d:
             self.discarded.extend(('kappa', i, w) for i, w in d)
 
-    def evolve(self, state: List[SiteNode]):
-        """Circuit.py:469-491. The caller's list is evolved in place; ``initial_state`` keeps a
-        device copy of what it held on entry."""
+    def _begin(self, state: List[SiteNode]) -> None:
         if not isinstance(state, list):
             raise TypeError('Qubit must be a list of nodes.')
         if len(state) != self.qnumber:
@@ -484,18 +494,30 @@ class TensorCircuit(QuantumCircuit):
         self._initState = replicate_nodes(state)
         self._pending = [None] * self.qnumber
         self._dm, self._dmNodes = None, None
-        for _i, layer in enumerate(self.layers):
-            name = layer.name.lower()
-            if 'truncate' in name:
-                self._truncate(state)
-            elif 'barrier' in name:
-                pass
-            else:
-                self._add_gate(state, _i, _oqs=self._oqs_list[_i])
+
+    def _step(self, state: List[SiteNode], _i: int) -> None:
+        """One entry of the layer list (the body of the loop at Circuit.py:476-485)."""
+        name = self.layers[_i].name.lower()
+        if 'truncate' in name:
+            self._truncate(state)
+        elif 'barrier' in name:
+            pass
+        else:
+            self._add_gate(state, _i, _oqs=self._oqs_list[_i])
+
+    def _finish(self, state: List[SiteNode]) -> None:
         self._flush(state, range(len(state)))
         if not self.ideal and len(self.layers) and 'truncate' not in self.layers[-1].name:
             self._kappa(state)
         self._stateNodes = state
+
+    def evolve(self, state: List[SiteNode]):
+        """Circuit.py:469-491. The caller's list is evolved in place; ``initial_state`` keeps a
+        device copy of what it held on entry."""
+        self._begin(state)
+        for _i in range(len(self.layers)):
+            self._step(state, _i)
+        self._finish(state)
 
     def forward(self, state: List[SiteNode]):
         self.evolve(state)

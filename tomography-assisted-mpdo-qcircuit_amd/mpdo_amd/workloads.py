@@ -157,3 +157,33 @@ def apply_ops(circuit, ops: Sequence[Op]) -> None:
             getattr(circuit, n.lower())(p[0], q[0], q[1])
         else:
             raise ValueError(f'unknown op {name}')
+
+
+def steady_state(dims: Sequence[Tuple[int, int, int]], seed: int = 0, decay: float = 0.97) -> List[np.ndarray]:
+    """Synthetic chain with the given site shapes (l, a, r): random complex64 site tensors
+    T[l, 2, a, r] whose bond and inner indices carry geometrically decaying weights (a physical,
+    non-flat spectrum), each of unit Frobenius norm. ``batched.steady_dims`` gives the shapes a long
+    brickwork circuit settles into."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for (l, a, r) in dims:
+        T = (rng.standard_normal((l, 2, a, r)) + 1j * rng.standard_normal((l, 2, a, r))).astype(np.complex64)
+        T *= (decay ** np.arange(l, dtype=np.float32))[:, None, None, None]
+        T *= (decay ** np.arange(r, dtype=np.float32))[None, None, None, :]
+        T *= (0.5 ** np.arange(a, dtype=np.float32))[None, None, :, None]
+        T /= np.linalg.norm(T)
+        out.append(T)
+    return out
+
+
+def layer_ops(ops: Sequence[Op], layer: int) -> List[Op]:
+    """The ops of one circuit-layer (everything up to and including its truncation)."""
+    out, cur = [], 0
+    for op in ops:
+        if cur == layer:
+            out.append(op)
+        if op[0] == 'TRUNCATE':
+            cur += 1
+            if cur > layer:
+                break
+    return out

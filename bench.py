@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""Throughput of the MPDO noisy-circuit update path on one MI355X (gfx950).
+
+Workload (the 32-qubit precision configuration): depth-30 brickwork of RY(theta) RZ(phi) on every
+qubit then CNOT on (0,1),(2,3),... / (1,2),(3,4),... alternately, single- and two-qubit
+depolarizing noise p = 0.002, chi = 256, kappa = 8, 32 copies whose angles differ by
+U[-0.01, 0.01] (mpdo_amd.workloads.config4). One step = one circuit-layer of all 32 copies:
+the composed single-qubit channel of every site, CNOT + two-qubit depolarizing on every pair of
+the layer's parity and the truncation (QR sweep, chi sweep, kappa cut), as one call of
+mpdo_layer_batched. The chains enter in the steady state of the circuit: synthetic complex64
+site tensors (seeded, decaying bond and inner weights) at the shapes a long run of this circuit
+settles into; successive steps alternate the parity and feed each other. Inputs are resident in
+HBM when the timed region starts.
+
+Prints one JSON line: circuit-layers/s (32 x steps / seconds), the roofline of the dominant
+kernel class (algorithmic flops per launch / launch duration, HIP events on the launch stream),
+and the CPU oracle timed on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, 'tomography-assisted-mpdo-qcircuit_amd')
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = 'circuit-layers/sec at χ=256,κ=8 (batched vs plain) @ ≤1e-5 DM error'
+PEAK_TFLOPS = {'fp64': 78.6, 'fp32': 157.3}     # MI355X dense peaks
+N_QUBITS, CHI, KAPPA, P_NOISE, FIRST_LAYER = 32, 256, 8, 0.002, 10
+
+
+def _steady_chains(dims, copies, dev, seed=2024):
+    """(copies, l, 2, a, r) complex64 per site: random tensors with geometric weights 0.97^l,
+    0.97^r on the bonds and 0.5^a on the inner index, unit Frobenius norm per copy and site."""
+    import torch
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    out = []
+    for (l, a, r) in dims:
+        T = torch.complex(torch.randn(copies, l, 2, a, r, generator=g, device=dev),
+                          torch.randn(copies, l, 2, a, r, generator=g, device=dev))
+        wl = 0.97 ** torch.arange(l, device=dev, dtype=torch.float32)
+        wr = 0.97 ** torch.arange(r, device=dev, dtype=torch.float32)
+        wa = 0.5 ** torch.arange(a, device=dev, dtype=torch.float32)
+        T = T * (wl[:, None, None, None] * wa[None, None, :, None] * wr[None, None, None, :])
+        T = T / torch.linalg.vector_norm(T.reshape(copies, -1), dim=1).reshape(copies, 1, 1, 1, 1)
+        out.append(T.contiguous())
+    return out
+
+
+def _layers(copies, count, dev):
+    """Fused-layer operands of layers FIRST_LAYER .. FIRST_LAYER + count - 1 of the workload."""
+    import mpdo_amd as M
+    from mpdo_amd import batched
+    from mpdo_amd import workloads as W
+    ws = W.config4(copies=copies, n=N_QUBITS)
+    cs = []
+    for w in ws:
+        c = M.TensorCircuit(device=str(dev), **w.circuit_kwargs())
+        W.apply_ops(c, w.ops)
+        cs.append(c)
+    segs = batched._segments(cs[0])
+    out = []
+    for k in range(count):
+        start, stop = segs[(FIRST_LAYER + k) % len(segs)]
+        layer, why = batched.fused_layer_of(cs, start, stop)
+        if layer is None:
+            raise RuntimeError(f'layer {FIRST_LAYER + k} does not fuse: {why}')
+        out.append(layer)
+    return out
+
+
+def cpu_baseline(threads: int = 16):
+    """The oracle (numpy complex128) on one pair of a steady-state circuit-layer: the composed
+    single-qubit channels of both sites and the gate-time split of the pair (Circuit.py:74-178);
+    a layer has 16 such pairs plus the QR, chi and kappa steps, so 16 x this time is a lower
+    bound on the oracle's time per circuit-layer."""
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+
+    from mpdo_amd import batched
+    from mpdo_amd import workloads as W
+    from oracle import mpdo_oracle as O
+
+    dims = batched.steady_dims(N_QUBITS, CHI, KAPPA, 4, 16, 0)
+    i = N_QUBITS // 2 - 2
+    Ti, Tj = [t.astype(np.complex128) for t in W.steady_state(dims[i:i + 2], seed=7)]
+    noise = O.NoiseSpec(chip='best', dpc_error_rate=P_NOISE, single='depolarizing', p1=P_NOISE)
+    E1 = noise.single_kraus(O.gate_matrix('RY', [1.1]))
+    E2 = noise.single_kraus(O.gate_matrix('RZ', [0.3]))
+    G2 = noise.double_kraus(O.gate_matrix('CNOT'))
+    with threadpool_limits(threads):
+        t0 = time.perf_counter()
+        a = O.apply_1q(O.apply_1q(Ti, E1), E2)
+        b = O.apply_1q(O.apply_1q(Tj, E1), E2)
+        O.apply_2q(a, b, G2)
+        dt = time.perf_counter() - t0
+    pairs = len(range(0, N_QUBITS - 1, 2))
+    return {'value': 1.0 / (pairs * dt), 'unit': 'circuit-layers/s', 'cores': threads, 'kind': 'port',
+            'sample': (f'oracle (numpy complex128, {threads} BLAS threads) on one middle pair of a steady-state '
+                       f'circuit-layer (n=32, chi=256, kappa=8, sites {i},{i + 1} of shape {dims[i]}): both '
+                       f'sites\' RY, RZ + depolarizing channels and the CNOT + depolarizing gate-time split took '
+                       f'{dt:.1f} s; value = 1 / ({pairs} x that), an upper bound on the oracle\'s rate since the '
+                       f'QR, chi and kappa steps of the layer are not timed')}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split('\n\n')[0])
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=1)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--copies', type=int, default=32)
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--cpu-threads', type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f'bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). '
+              'Chains are independent, so more GPUs would each take a share of the copies; run with --gpus 1.',
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from mpdo_amd import _lib, batched
+
+    if not torch.cuda.is_available():
+        raise SystemExit('bench.py needs the MI355X (cuda:0)')
+    dev = torch.device('cuda:0')
+    lib = _lib.load()
+    prev_mode = lib.mpdo_set_math_mode(0)
+    B = args.copies
+    layers = _layers(B, args.warmup + args.steps, dev)
+    dims = batched.steady_dims(N_QUBITS, CHI, KAPPA, layers[0].kraus1.shape[2], layers[0].gates2.shape[-1],
+                               layers[0].parity)
+    sites = _steady_chains(dims, B, dev)
+    torch.cuda.synchronize()
+    print(f'# {B} copies, {N_QUBITS} qubits, chi={CHI}, kappa={KAPPA}; warmup {args.warmup}, steps {args.steps}',
+          file=sys.stderr, flush=True)
+
+    def step(k, sites):
+        L = layers[k]
+        outs, _ = _lib.layer_batched(sites, L.kraus1, L.parity, L.gates2, CHI, KAPPA)
+        return outs
+
+    for k in range(args.warmup):
+        t = time.perf_counter()
+        sites = step(k, sites)
+        torch.cuda.synchronize()
+        print(f'# warmup step {k}: {time.perf_counter() - t:.2f} s', file=sys.stderr, flush=True)
+
+    _lib.prof_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        sites = step(args.warmup + k, sites)
+        print(f'# timed step {k} issued', file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    T = time.perf_counter() - t0
+    prof = _lib.prof_read()
+    _lib.prof_enable(False)
+
+    # dominant kernel class by summed launch time among those with algorithmic flops
+    name, (n_l, ms, fl, by) = max(((k, v) for k, v in prof.items() if v[2] > 0), key=lambda kv: kv[1][1])
+    prec = 'fp64'
+    achieved = fl / (ms * 1e-3) / 1e12
+    kernels = {'gemm': 'mpdo::gemm_kernel (complex GEMM, fp64 MFMA accumulate)',
+               'eig_rotate': 'mpdo::bj_rotate (block-Jacobi rotation update, fp64)',
+               'eig_solve': 'mpdo::bj_solve (block-Jacobi 64x64 subproblem, fp64)'}
+    roofline = {'bound': 'mfma', 'achieved': round(achieved, 3), 'peak': PEAK_TFLOPS[prec], 'unit': 'TFLOP/s',
+                'frac': round(achieved / PEAK_TFLOPS[prec], 4), 'traffic': None,
+                'kernel': kernels.get(name, name), 'launches': n_l, 'avg_launch_ms': round(ms / max(n_l, 1), 4),
+                'flops_per_launch': fl / max(n_l, 1),
+                'share_of_kernel_time': round(ms / max(sum(v[1] for v in prof.values()), 1e-9), 3),
+                'classes_ms': {k: round(v[1], 1) for k, v in prof.items()}}
+    value = B * args.steps / T
+    cpu = None
+    if not args.no_cpu_baseline:
+        print('# cpu baseline (oracle sample)', file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.cpu_threads)
+    lib.mpdo_set_math_mode(prev_mode)
+    line = {'metric': METRIC, 'value': round(value, 6), 'unit': 'circuit-layers/s', 'n_gpus': 1,
+            'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': round(T / args.steps * 1e3, 3),
+            'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+            'dtype': 'c64 data, fp64 accumulate/eigensolve', 'data': 'synthetic',
+            'config': {'workload': ('32-qubit depth-30 RY/RZ+CNOT brickwork, depolarizing p=0.002 (1q, 2q), '
+                                    'chi=256, kappa=8, 32 angle-perturbed copies; step = one circuit-layer of '
+                                    'every copy from the steady state (fused batched layer)'),
+                       'n_qubits': N_QUBITS, 'chi': CHI, 'kappa': KAPPA, 'copies': B, 'global_batch': B,
+                       'parallelism': 'none (one GPU)'},
+            'roofline': roofline, 'cpu_baseline': cpu}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -33,6 +33,15 @@ mpdo_status mpdo_debug_eigh(int n, int batch, void* A, double* w, void* V, int* 
 mpdo_status mpdo_debug_layer_dry_run(int batch, int n, const mpdo_site* in, int K1, int parity, int K2, int chi,
                                      int kappa, const mpdo_site* out, unsigned long long* workspace_bytes);
 
+/* Launch timing by kernel class (0 complex GEMM, 1 block-Jacobi subproblem solve, 2 block-Jacobi
+ * rotation update, 3 single-workgroup Jacobi, 4 permutation): HIP events recorded on each
+ * launch's own stream. enable(1) resets the totals and starts recording, enable(0) stops;
+ * read() waits for the recorded launches and returns the totals of one class: launches, summed
+ * duration in ms, algorithmic flops (the arithmetic the launches are asked to do) and operand
+ * bytes. */
+mpdo_status mpdo_debug_prof_enable(int on);
+mpdo_status mpdo_debug_prof_read(int cls, long long* launches, double* ms, double* flops, double* bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,6 +10,7 @@
 // Rotations skip |a_pq| <= 1e-15 sqrt(a_pp a_qq) (relative criterion; for the positive
 // semidefinite Gram matrices this yields eigenvalues with small relative error).
 #include "mpdo_internal.h"
+#include "prof.h"
 
 #include <vector>
 
@@ -382,8 +383,11 @@ hipError_t eigh(int n, int batch, void* A, long long lda, long long bsA, double*
   if (n <= NS) {
     int* sw_dev = (int*)ws;
     if ((err = hipMemsetAsync(sw_dev, 0, sizeof(int), st)) != hipSuccess) return err;
-    hipLaunchKernelGGL(eigh_small_kernel, dim3(batch), dim3(256), 0, st, n, (double2*)A, lda, bsA, w, ldw,
-                       (double2*)V, ldv, bsV, max_sweeps, sw_dev);
+    {
+      prof::Scope ps(prof::EIG_SMALL, st, 0.0, 2.0 * 16.0 * n * n * batch);
+      hipLaunchKernelGGL(eigh_small_kernel, dim3(batch), dim3(256), 0, st, n, (double2*)A, lda, bsA, w, ldw,
+                         (double2*)V, ldv, bsV, max_sweeps, sw_dev);
+    }
     if ((err = hipGetLastError()) != hipSuccess) return err;
     if (sweeps_out) {
       if ((err = hipMemcpyAsync(sweeps_out, sw_dev, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess) return err;
@@ -410,10 +414,25 @@ hipError_t eigh(int n, int batch, void* A, long long lda, long long bsA, double*
   int sw = 0;
   for (; sw < max_sweeps; ++sw) {
     if ((err = hipMemsetAsync(s.flag, 0, sizeof(int) * batch, st)) != hipSuccess) return err;
+    // algorithmic work: one inner sweep of a 64x64 subproblem applies 63 x 32 rotations, each
+    // updating 192 element pairs at 18 flops; a rotation tile is a 64x64x64 complex product
+    const double solve_flops = (double)(P / 2) * batch * 63.0 * 32.0 * 192.0 * 18.0;
+    const double solve_bytes = (double)(P / 2) * batch * 2.0 * NS * NS * 16.0;
+    const double tile_flops = 8.0 * NS * NS * NS, tile_bytes = 3.0 * NS * NS * 16.0;
+    const double tiles_c = (double)(s.N / NS) * (P / 2) * 2 * batch, tiles_r = (double)(s.N / NS) * (P / 2) * batch;
     for (int r = 0; r < P - 1; ++r) {
-      hipLaunchKernelGGL(bj_solve, dim3(P / 2, batch), dim3(256), 0, st, s, r, 1);
-      hipLaunchKernelGGL(bj_rotate, dim3(s.N / NS, P / 2, 2 * batch), dim3(256), 0, st, s, r, 1);
-      hipLaunchKernelGGL(bj_rotate, dim3(s.N / NS, P / 2, batch), dim3(256), 0, st, s, r, 0);
+      {
+        prof::Scope ps(prof::EIG_SOLVE, st, solve_flops, solve_bytes);
+        hipLaunchKernelGGL(bj_solve, dim3(P / 2, batch), dim3(256), 0, st, s, r, 1);
+      }
+      {
+        prof::Scope ps(prof::EIG_ROTATE, st, tiles_c * tile_flops, tiles_c * tile_bytes);
+        hipLaunchKernelGGL(bj_rotate, dim3(s.N / NS, P / 2, 2 * batch), dim3(256), 0, st, s, r, 1);
+      }
+      {
+        prof::Scope ps(prof::EIG_ROTATE, st, tiles_r * tile_flops, tiles_r * tile_bytes);
+        hipLaunchKernelGGL(bj_rotate, dim3(s.N / NS, P / 2, batch), dim3(256), 0, st, s, r, 0);
+      }
     }
     if ((err = hipGetLastError()) != hipSuccess) return err;
     if ((err = hipMemcpyAsync(hflag.data(), s.flag, sizeof(int) * batch, hipMemcpyDeviceToHost, st)) != hipSuccess)

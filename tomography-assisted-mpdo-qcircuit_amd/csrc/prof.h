@@ -1,0 +1,30 @@
+// Launch timing by kernel class with HIP events recorded on each launch's own stream. Off by
+// default; mpdo_debug_prof_enable turns it on. Each record carries the launch's algorithmic work
+// (flops of the arithmetic it is asked to do, bytes of its operands) for roofline accounting.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace mpdo {
+namespace prof {
+
+enum Cls { GEMM = 0, EIG_SOLVE = 1, EIG_ROTATE = 2, EIG_SMALL = 3, PERMUTE = 4, NCLS = 5 };
+
+bool enabled();
+
+// RAII: records a start event on construction and a stop event on destruction
+class Scope {
+ public:
+  Scope(int cls, hipStream_t s, double flops, double bytes);
+  ~Scope();
+  Scope(const Scope&) = delete;
+  Scope& operator=(const Scope&) = delete;
+
+ private:
+  int cls_;
+  hipStream_t s_;
+  double flops_, bytes_;
+  hipEvent_t e0_ = nullptr, e1_ = nullptr;
+};
+
+}  // namespace prof
+}  // namespace mpdo

@@ -1,0 +1,106 @@
+#include "prof.h"
+
+#include <mutex>
+#include <vector>
+
+#include "mpdo.h"
+#include "mpdo_debug.h"
+
+namespace mpdo {
+namespace prof {
+namespace {
+
+struct Rec {
+  int cls;
+  hipEvent_t e0, e1;
+  double flops, bytes;
+};
+struct Acc {
+  long long launches = 0;
+  double ms = 0, flops = 0, bytes = 0;
+};
+
+std::mutex g_mu;
+bool g_on = false;
+std::vector<Rec> g_recs;
+std::vector<hipEvent_t> g_free;
+Acc g_acc[NCLS];
+
+// fold finished records into the totals (waits for the last one) and recycle their events
+void drain_locked() {
+  if (g_recs.empty()) return;
+  hipEventSynchronize(g_recs.back().e1);
+  for (const Rec& r : g_recs) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, r.e0, r.e1) == hipSuccess) {
+      Acc& a = g_acc[r.cls];
+      a.launches += 1;
+      a.ms += ms;
+      a.flops += r.flops;
+      a.bytes += r.bytes;
+    }
+    g_free.push_back(r.e0);
+    g_free.push_back(r.e1);
+  }
+  g_recs.clear();
+}
+
+hipEvent_t take_locked() {
+  if (!g_free.empty()) {
+    hipEvent_t e = g_free.back();
+    g_free.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  hipEventCreate(&e);
+  return e;
+}
+
+}  // namespace
+
+bool enabled() { return g_on; }
+
+Scope::Scope(int cls, hipStream_t s, double flops, double bytes) : cls_(cls), s_(s), flops_(flops), bytes_(bytes) {
+  if (!g_on) return;
+  std::lock_guard<std::mutex> lk(g_mu);
+  e0_ = take_locked();
+  e1_ = take_locked();
+  hipEventRecord(e0_, s_);
+}
+
+Scope::~Scope() {
+  if (!e0_) return;
+  std::lock_guard<std::mutex> lk(g_mu);
+  hipEventRecord(e1_, s_);
+  g_recs.push_back(Rec{cls_, e0_, e1_, flops_, bytes_});
+  if (g_recs.size() >= 4096) drain_locked();
+}
+
+}  // namespace prof
+}  // namespace mpdo
+
+extern "C" {
+
+mpdo_status mpdo_debug_prof_enable(int on) {
+  using namespace mpdo::prof;
+  std::lock_guard<std::mutex> lk(g_mu);
+  drain_locked();
+  for (auto& a : g_acc) a = Acc{};
+  g_on = on != 0;
+  return MPDO_OK;
+}
+
+mpdo_status mpdo_debug_prof_read(int cls, long long* launches, double* ms, double* flops, double* bytes) {
+  using namespace mpdo::prof;
+  if (cls < 0 || cls >= NCLS) return MPDO_EINVAL;
+  std::lock_guard<std::mutex> lk(g_mu);
+  drain_locked();
+  const Acc& a = g_acc[cls];
+  if (launches) *launches = a.launches;
+  if (ms) *ms = a.ms;
+  if (flops) *flops = a.flops;
+  if (bytes) *bytes = a.bytes;
+  return MPDO_OK;
+}
+
+}  // extern "C"

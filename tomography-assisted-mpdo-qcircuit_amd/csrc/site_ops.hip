@@ -2,6 +2,7 @@
 // the two-qubit channel on the QR-reduced core, tensor permutations, the truncation rule of
 // decompositions.svd, and the conversion of fp64 eigenvectors into complex64 factors.
 #include "site_ops.h"
+#include "prof.h"
 
 namespace mpdo {
 
@@ -239,6 +240,7 @@ hipError_t permute(const float2* in, float2* out, int rank, const int* dims, con
   int blocks = (int)((tot + 255) / 256);
   if (blocks > 8192) blocks = 8192;
   if (blocks < 1) blocks = 1;
+  prof::Scope ps(prof::PERMUTE, s, 0.0, 16.0 * (double)tot * batch);
   for (int b0 = 0; b0 < batch; b0 += 65535) {
     int nb = batch - b0 < 65535 ? batch - b0 : 65535;
     hipLaunchKernelGGL(permute_kernel, dim3(blocks, nb), dim3(256), 0, s, in + b0 * bstride, out + b0 * bstride, d,

@@ -61,6 +61,8 @@ def load() -> ctypes.CDLL:
                                 D, D, D, D, VP]),
         'mpdo_debug_eigh': (I, [I, I, VP, P(D), VP, P(I), VP]),
         'mpdo_debug_layer_dry_run': (I, [I, I, SP, I, I, I, I, I, SP, P(ctypes.c_ulonglong)]),
+        'mpdo_debug_prof_enable': (I, [I]),
+        'mpdo_debug_prof_read': (I, [I, P(ctypes.c_longlong), P(D), P(D), P(D)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -385,4 +387,24 @@ def layer_dry_run(batch: int, dims: Sequence[Tuple[int, int, int]], K1: int, par
     check(lib.mpdo_debug_layer_dry_run(batch, n, ins, K1, parity, K2, int(chi or 0), int(kappa or 0), outs,
                                        ctypes.byref(ws)), 'mpdo_debug_layer_dry_run')
     return int(ws.value)
+
+
+PROF_CLASSES = ('gemm', 'eig_solve', 'eig_rotate', 'eig_small', 'permute')
+
+
+def prof_enable(on: bool) -> None:
+    """Start (reset) or stop recording HIP events around every launch, by kernel class."""
+    check(load().mpdo_debug_prof_enable(1 if on else 0), 'mpdo_debug_prof_enable')
+
+
+def prof_read() -> dict:
+    """{class: (launches, ms, flops, bytes)} of the launches recorded since prof_enable(True)."""
+    lib = load()
+    out = {}
+    for i, name in enumerate(PROF_CLASSES):
+        n, ms, fl, by = ctypes.c_longlong(0), ctypes.c_double(0), ctypes.c_double(0), ctypes.c_double(0)
+        check(lib.mpdo_debug_prof_read(i, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(fl), ctypes.byref(by)),
+              'mpdo_debug_prof_read')
+        out[name] = (int(n.value), float(ms.value), float(fl.value), float(by.value))
+    return out
 

@@ -11,13 +11,18 @@ index (``I_i``), ``r`` = bond to site i+1. An absent edge is a dimension-1 axis,
 (``has_I``, ``bond``) for the two places where the reference's behaviour depends on the presence of
 an edge rather than on its dimension (``Circuit.py:83,140`` and ``TNNOptimizer.py:51-61,176-181``).
 
-Parity pinning (how each part is checked, see ``tests/test_oracle_*.py``):
-  * gate matrices, Kraus builders, RealNoise Kraus sets: closed forms / CPTP identities / the
-    ``ideal_cz.mat`` = CZ(K=1) and ``czDefault.mat`` K=16, Tr-non-preserving known answers;
-  * gate application, sweeps, observables: an independent dense complex128 density-matrix
-    simulator (``oracle/dense.py``) whenever truncation is inactive, and brute-force dense
-    purification + explicit SVD of the same schedule at n <= 4 with chi and kappa truncating;
-  * decompositions.svd keep-count rule: hand-computed spectra (ties, relative mode, min(chi, k)).
+Parity pinning (how each part is checked, ``tests/test_oracle.py``):
+  * gate matrices: matrix exponentials of their generators (the reference's RYY is the XY
+    rotation exp(-i t/2 X (x) Y), YGates.py:138-143), fixed gates from projector identities;
+  * Kraus builders: Bloch-vector / matrix-element actions of depolarizing, amplitude damping
+    and dephasing, CPTP identities, the noisy-gate composition order; RealNoise: ``ideal_cz.mat``
+    = CZ with one operator and the recorded sum E^H E diagonal of ``czDefault.mat`` (K=16);
+  * decompositions.svd keep-count rule: hand-computed spectra (ties, relative mode,
+    min(chi, k), the last index); the rank-aware split against the explicit SVD;
+  * gate application, sweeps without truncation, observables: the independent dense complex128
+    density-matrix simulator (``oracle/dense.py``);
+  * chi cut: the Schmidt values of a dense pure state at the first cut of the sweep;
+    kappa cut: the eigen-decomposition of a single site's density matrix.
 Functions whose numbers no such check reaches are marked "parity unpinned" in their docstring.
 """
 from __future__ import annotations
@@ -101,10 +106,11 @@ def gate_matrix(name: str, params: Sequence[float] = ()) -> np.ndarray:
         t = p[0]
         a, b = c(t), -1j * s(t)
         two = [[a, 0, 0, b], [0, a, b, 0], [0, b, a, 0], [b, 0, 0, a]]
-    elif n == 'RYY':                                                 # YGates.py:124-161
+    elif n == 'RYY':                                                 # YGates.py:124-161 (:138-143)
+        # the reference's matrix is exp(-i t/2 X (x) Y), not exp(-i t/2 Y (x) Y)
         t = p[0]
         a, b = c(t), s(t)
-        two = [[a, 0, 0, -b], [0, a, b, 0], [0, b, a, 0], [-b, 0, 0, a]]
+        two = [[a, 0, 0, -b], [0, a, b, 0], [0, -b, a, 0], [b, 0, 0, a]]
     elif n == 'RZZ':                                                 # ZGates.py:122-159
         t = p[0]
         m, pl = np.exp(-0.5j * t), np.exp(0.5j * t)
@@ -741,7 +747,7 @@ def pauli_expect(st: MPDO, observables, oqs) -> float:
         P = np.eye(2, dtype=np.complex128)
         if i in oqs:
             P = PAULIS[1 + obs[oqs.index(i)]]
-        env = np.einsum('ln,lsar,ts,ntam->rm', env, T, P, T.conj())
+        env = np.einsum('ln,lsar,ts,ntam->rm', env, T, P, T.conj(), optimize=True)
     return float(np.real(env[0, 0]))
 
 
@@ -749,7 +755,7 @@ def trace_rho(st: MPDO) -> float:
     """dmOperations.py:40-48."""
     env = np.ones((1, 1), dtype=np.complex128)
     for T in st.sites:
-        env = np.einsum('ln,lsar,nsam->rm', env, T, T.conj())
+        env = np.einsum('ln,lsar,nsam->rm', env, T, T.conj(), optimize=True)
     return float(np.real(env[0, 0]))
 
 
@@ -759,5 +765,5 @@ def trace_rho_rho(st0: MPDO, st1: Optional[MPDO] = None) -> float:
     env = np.ones((1, 1, 1, 1), dtype=np.complex128)
     for A, B in zip(st0.sites, st1.sites):
         # rho0[s, s'] rho1[s', s]: ket0 s with bra1 s, bra0 s' with ket1 s'
-        env = np.einsum('abcd,asxe,btxf,ctyg,dsyh->efgh', env, A, A.conj(), B, B.conj())
+        env = np.einsum('abcd,asxe,btxf,ctyg,dsyh->efgh', env, A, A.conj(), B, B.conj(), optimize=True)
     return float(np.real(env[0, 0, 0, 0]))

@@ -1,7 +1,7 @@
 """Fused brickwork layers (mpdo_layer_batched through batched.evolve_batched) against the per-gate
 path of every copy and against the oracle: density matrix within 1e-5 of max|rho|, the chi
 cuts' discarded weights in the same order within 1e-4, kappa totals within 1e-4, gate-time split
-weights (all below 7.4e-16 by the 2.7e-8 tail rule) within 1e-15."""
+weights (at most 2 x 2.7e-8 x |theta| by the split's error bound) within 1e-3 or 1e-12."""
 import numpy as np
 import pytest
 import torch
@@ -86,7 +86,7 @@ def test_fused_layers_match_per_gate_path(cuda, n, depth, copies, chi, kappa, no
         gate_f = [d for d in cf.discarded if d[0] == 'gate']
         gate_g = [d for d in cg.discarded if d[0] == 'gate']
         assert [d[:2] for d in gate_f] == [d[:2] for d in gate_g]
-        assert np.allclose([d[2] for d in gate_f], [d[2] for d in gate_g], rtol=1e-3, atol=1e-15)
+        assert np.allclose([d[2] for d in gate_f], [d[2] for d in gate_g], rtol=1e-3, atol=1e-12)
     # the first copy against the oracle (complex128)
     st = bridge.run_oracle(ws[0])
     assert _rho_err(fused[0].cal_dm().cpu().numpy(), O.density_matrix(st)) < 1e-5

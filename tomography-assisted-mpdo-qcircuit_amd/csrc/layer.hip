@@ -58,8 +58,9 @@ namespace {
 
 constexpr double FLOOR = 1e-13;         // relative eigenvalue floor of the QR-type factorisations
 constexpr double SPLIT_FLOOR = 1e-12;   // the same floor as the per-gate sqrt(S) splits (s_j < 1e-6 s_0)
-// absolute tail of the gate-time split (Circuit.py:22,120-124): the smallest k with
-// sqrt(sum_{j >= k} s_j^2) <= GATE_EPS singular values of the two-site tensor are kept
+// absolute error bound of the gate-time split (Circuit.py:22,120-124): with te = sqrt(cumsum s^2)
+// the split keeps idx+1 singular values for the first idx with te[-1] - te[idx] <= GATE_EPS
+// (decompositions.py:121-131)
 constexpr double GATE_EPS = 2.718281828459045e-8;
 
 // ------------------------------------------------------------------ host-side bounds checks

@@ -196,9 +196,10 @@ CPGate = _param('CP', lambda t: np.diag([1, 1, 1, np.exp(1j * t)]), single=False
 RXXGate = _param('RXX', lambda t: [[_c(t), 0, 0, -1j * _s(t)], [0, _c(t), -1j * _s(t), 0],
                                    [0, -1j * _s(t), _c(t), 0], [-1j * _s(t), 0, 0, _c(t)]],
                  single=False)                                                            # XGates.py:123-160
+# YGates.py:124-161: the reference's RYY matrix (:138-143) is exp(-i theta/2 X (x) Y)
 RYYGate = _param('RYY', lambda t: [[_c(t), 0, 0, -_s(t)], [0, _c(t), _s(t), 0],
-                                   [0, _s(t), _c(t), 0], [_s(t), 0, 0, _c(t)]],
-                 single=False)                                                            # YGates.py:124-161
+                                   [0, -_s(t), _c(t), 0], [_s(t), 0, 0, _c(t)]],
+                 single=False)
 RZZGate = _param('RZZ', lambda t: np.diag([np.exp(-0.5j * t), np.exp(0.5j * t), np.exp(0.5j * t),
                                            np.exp(-0.5j * t)]), single=False)              # ZGates.py:122-159
 PSWAPGate = _param('SWAP', lambda t: [[1, 0, 0, 0], [0, math.cos(t), math.sin(t), 0],

@@ -33,6 +33,11 @@ mpdo_status mpdo_debug_eigh(int n, int batch, void* A, double* w, void* V, int* 
 mpdo_status mpdo_debug_layer_dry_run(int batch, int n, const mpdo_site* in, int K1, int parity, int K2, int chi,
                                      int kappa, const mpdo_site* out, unsigned long long* workspace_bytes);
 
+/* Per pair of the last mpdo_layer_batched call on this thread: the largest number of singular
+ * values its gate-time split kept in any chain (2 p1t when none was dropped). Writes up to `cap`
+ * values, returns the number of pairs. */
+int mpdo_debug_layer_kept(int* out, int cap);
+
 /* Launch timing by kernel class (0 complex GEMM, 1 block-Jacobi subproblem solve, 2 block-Jacobi
  * rotation update, 3 single-workgroup Jacobi, 4 permutation): HIP events recorded on each
  * launch's own stream. enable(1) resets the totals and starts recording, enable(0) stops;

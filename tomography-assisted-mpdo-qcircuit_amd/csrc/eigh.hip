@@ -18,6 +18,8 @@ namespace mpdo {
 
 namespace {
 
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
 constexpr int NS = 64;       // LDS matrix order
 constexpr int LD = NS + 1;   // padded LDS leading dimension
 constexpr double REL_TOL = 1e-15;
@@ -256,8 +258,9 @@ __global__ __launch_bounds__(256) void bj_solve(BJ s, int r, int inner_sweeps) {
 // X[K u L, cols] <- W_KL^H * X[K u L, cols] (cols=0), in place, 64x64 tile per workgroup.
 // blockIdx.z selects the matrix: z < batch -> A[z], else V[z - batch] (columns only).
 __global__ __launch_bounds__(256) void bj_rotate(BJ s, int r, int cols) {
+  // only the 64x64 tile of M is staged in LDS (66.5 KiB, two workgroups per CU); the rotation W
+  // of the block pair is read from global memory, where every tile of the pair finds it in L2
   __shared__ double2 X[NS][LD];
-  __shared__ double2 W[NS][LD];
   const int P = s.N / 32, pi = blockIdx.y, t0 = blockIdx.x * NS, tid = threadIdx.x;
   int z = blockIdx.z;
   bool isV = z >= s.batch;
@@ -268,7 +271,6 @@ __global__ __launch_bounds__(256) void bj_rotate(BJ s, int r, int cols) {
   const double2* Wb = s.W + ((long long)b * (P / 2) + pi) * NS * NS;
   for (int e = tid; e < NS * NS; e += blockDim.x) {
     int i = e / NS, j = e % NS;
-    W[i][j] = Wb[e];
     if (cols) {
       int gj = (j < 32 ? I * 32 + j : J * 32 + j - 32);
       X[i][j] = M[(long long)(t0 + i) * s.N + gj];
@@ -278,60 +280,65 @@ __global__ __launch_bounds__(256) void bj_rotate(BJ s, int r, int cols) {
     }
   }
   __syncthreads();
-  // each thread: 4x4 outputs
-  const int ti = (tid / 16) * 4, tj = (tid % 16) * 4;
-  double2 acc[4][4];
+  // out = P Q (64x64 complex) on the fp64 matrix cores: P = X, Q = W for column updates, P = W^H,
+  // Q = X for row updates. Wave w computes the 32x32 quadrant (w/2, w%2) as 2x2 tiles of 16x16,
+  // four real v_mfma_f64_16x16x4f64 per tile and k-step (Re += Pr Qr - Pi Qi, Im += Pr Qi + Pi Qr).
+  // Operand layouts: P[i][k] in lane i + 16k, Q[k][j] in lane j + 16k; result D[i][j] in lane
+  // j + 16 (i % 4), item i / 4.
+  const int wave = tid / 64, lane = tid % 64, li = lane % 16, lk = lane / 16;
+  const int r0 = (wave / 2) * 32, c0 = (wave % 2) * 32;
+  doublex4 re[2][2], im[2][2];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) acc[a][c] = make_double2(0, 0);
-  if (cols) {
-    // out[i][j] = sum_k X[i][k] W[k][j]
-    for (int k = 0; k < NS; ++k) {
-      double2 x[4], y[4];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) x[a] = X[ti + a][k];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) y[c] = W[k][tj + c];
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          acc[a][c].x += x[a].x * y[c].x - x[a].y * y[c].y;
-          acc[a][c].y += x[a].x * y[c].y + x[a].y * y[c].x;
-        }
+    for (int c = 0; c < 2; ++c) {
+      re[a][c] = doublex4{0, 0, 0, 0};
+      im[a][c] = doublex4{0, 0, 0, 0};
     }
-  } else {
-    // out[i][j] = sum_k conj(W[k][i]) X[k][j]
-    for (int k = 0; k < NS; ++k) {
-      double2 x[4], y[4];
+  for (int kk = 0; kk < NS; kk += 4) {
+    const int k = kk + lk;
+    double pr[2], pi_[2], pn[2], qr[2], qi[2];
 #pragma unroll
-      for (int a = 0; a < 4; ++a) x[a] = W[k][ti + a];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) y[c] = X[k][tj + c];
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          acc[a][c].x += x[a].x * y[c].x + x[a].y * y[c].y;
-          acc[a][c].y += x[a].x * y[c].y - x[a].y * y[c].x;
-        }
+    for (int a = 0; a < 2; ++a) {
+      const int row = r0 + 16 * a + li;
+      double2 v;
+      if (cols) v = X[row][k];
+      else { v = Wb[k * NS + row]; v.y = -v.y; }
+      pr[a] = v.x; pi_[a] = v.y; pn[a] = -v.y;
     }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int col = c0 + 16 * c + li;
+      double2 v = cols ? Wb[k * NS + col] : X[k][col];
+      qr[c] = v.x; qi[c] = v.y;
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        re[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(pr[a], qr[c], re[a][c], 0, 0, 0);
+        re[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(pn[a], qi[c], re[a][c], 0, 0, 0);
+        im[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(pr[a], qi[c], im[a][c], 0, 0, 0);
+        im[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(pi_[a], qr[c], im[a][c], 0, 0, 0);
+      }
   }
   __syncthreads();
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      int i = ti + a, j = tj + c;
-      if (cols) {
-        int gj = (j < 32 ? I * 32 + j : J * 32 + j - 32);
-        M[(long long)(t0 + i) * s.N + gj] = acc[a][c];
-      } else {
-        int gi = (i < 32 ? I * 32 + i : J * 32 + i - 32);
-        M[(long long)gi * s.N + t0 + j] = acc[a][c];
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = r0 + 16 * a + 4 * q + lk, j = c0 + 16 * c + li;
+        const double2 o = make_double2(re[a][c][q], im[a][c][q]);
+        if (cols) {
+          int gj = (j < 32 ? I * 32 + j : J * 32 + j - 32);
+          M[(long long)(t0 + i) * s.N + gj] = o;
+        } else {
+          int gi = (i < 32 ? I * 32 + i : J * 32 + i - 32);
+          M[(long long)gi * s.N + t0 + j] = o;
+        }
       }
-    }
 }
 
 // eigenvalues = diag(A), sorted descending (stable), eigenvectors = matching columns of V

@@ -75,6 +75,7 @@ thread_local std::string t_oob;
 // dry run: every check is made, nothing is launched or copied (host-only validation of a plan)
 thread_local bool t_dry = false;
 thread_local size_t t_ws_bytes = 0;   // workspace size of the last planned layer
+thread_local std::vector<int> t_kept;   // per pair of the last layer: max kept count of its gate-time split
 
 void add_span(const void* p, size_t bytes) {
   if (t_spans && p && bytes) t_spans->push_back(Span{(uintptr_t)p, (uintptr_t)p + bytes});
@@ -484,7 +485,10 @@ struct PairStore {
   int m, rows2, cols2, p2;      // T_max as [(m s2) x (a2 r)] = L2 Q2
   bool q1tid, q1id, q2id;
   float2 *Q1, *Ra, *Y, *Um, *Q2, *L2;   // Y: the theta core before the channel, [2][2][p1t][p2];
-                                        // Um: kept left singular vectors of the theta core
+                                        // Um: kept left singular vectors of the theta core [nT x mq]
+  float2 *Qz, *Rz;              // (Ra (x) I) Um = Qz Rz when the gate-time cut is active
+  int mq, pz, nc;               // kept directions (max over chains), rows of Rz, rows of the core
+  bool qzid;
   double2* WR;
   double* lamR;
   bool has_R, has_P;            // has_P: the gate-time cut drops a direction in some chain
@@ -568,6 +572,9 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     if (!ps.q2id) { mxQR_rows = std::max(mxQR_rows, (size_t)ps.cols2); mxQR_cols = std::max(mxQR_cols, (size_t)ps.rows2); }
     mxEigQR = std::max(mxEigQR, std::max({2 * ps.m, ps.cols1, ps.rows2}));
     mxEigC = std::max(mxEigC, 2 * ps.p1t);
+    // QR of (Ra (x) I) Um, [2 p1 x mq] with mq <= 2 p1t, factored only when 2 p1 > mq
+    mxQR_rows = std::max(mxQR_rows, (size_t)2 * ps.p1);
+    mxQR_cols = std::max(mxQR_cols, (size_t)std::min(2 * ps.p1, 2 * ps.p1t));
     mxY = std::max(mxY, (size_t)4 * ps.p1t * ps.p2);
     mxC = std::max(mxC, (size_t)2 * ps.p1t * 2 * K2 * ps.p2);
     mxOm = std::max(mxOm, (size_t)ps.p2 * ps.p2);
@@ -600,7 +607,7 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
   // device buffers
   std::vector<float2*> T1(n, nullptr), Astore(n, nullptr);
   Site1q* dtab = nullptr;
-  float2 *Mlast = nullptr, *Rcur, *Rnext, *RT, *X1, *X1t, *Q1t, *R1t, *Cs, *Cp, *Cu, *Zb, *Q2P, *L3, *Q3, *C2, *FL, *FR, *Tmin, *Wm,
+  float2 *Mlast = nullptr, *FLq, *Rcur, *Rnext, *RT, *X1, *X1t, *Q1t, *R1t, *Cs, *Cp, *Cu, *Zb, *Q2P, *L3, *Q3, *C2, *FL, *FR, *Tmin, *Wm,
          *Tm, *Pn, *Pn2, *PW, *Pp, *Tchi, *Mp, *Vk, *Kout, *dG, *Tcur2, *dK;
   double2 *G, *GR, *Om64, *Q2w, *Z64;
   double* ddisc;
@@ -625,6 +632,8 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
       ps.Ra = c.get<float2>((size_t)ps.p1 * ps.p1t * B);
       ps.Y = c.get<float2>((size_t)4 * ps.p1t * ps.p2 * B);
       ps.Um = c.get<float2>((size_t)4 * ps.p1t * ps.p1t * B);
+      ps.Qz = c.get<float2>((size_t)4 * ps.p1 * ps.p1 * B);
+      ps.Rz = c.get<float2>((size_t)2 * ps.p1 * 2 * ps.p1t * B);
       ps.Q2 = ps.q2id ? nullptr : c.get<float2>((size_t)ps.p2 * ps.cols2 * B);
       ps.L2 = ps.q2id ? nullptr : c.get<float2>((size_t)ps.rows2 * ps.p2 * B);
       const int r = P.in[ps.i + 1].r;
@@ -652,6 +661,7 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     Q3 = c.get<float2>(mxQ2P * B);
     C2 = c.get<float2>(mxC2 * B);
     FL = c.get<float2>(mxFL * B);
+    FLq = c.get<float2>(mxFL * B);
     FR = c.get<float2>(mxFL * B);
     Tmin = c.get<float2>(mxTmin * B);
     Tcur2 = c.get<float2>(mxTcur * B);
@@ -735,24 +745,31 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     LST(lq_b(T1[ps.i + 1], ps.rows2, ps.cols2, ps.Q2, ps.L2, B, qw, s));
   }
 
-  // Cu = (Ra (x) I_s) P G(Y): the pair core in the basis of Q1 (P = Um Um^H when the gate-time
-  // cut drops directions in some chain, else the identity)
+  // The pair core in a row basis of Q1's (p s1) space: (Ra (x) I_s) P G(Y) = Qc Rc. Without an
+  // active gate-time cut P = I, Qc = I and Rc = (Ra (x) I_s) G(Y) has nc = 2 p1 rows; with it
+  // P = Um Um^H and Rc = Rz (Um^H G(Y)) has nc = pz <= mq rows (Qc = Qz). Rc goes to Cu.
   auto make_core = [&](PairStore& ps, int gidx) -> mpdo_status {
     const int p1t = ps.p1t, nT = 2 * p1t;
     const long long cols = 2LL * K2 * ps.p2, szCt = (long long)nT * cols;
     LCK(k_channel2q(ps.Y, dG + (size_t)gidx * 16 * K2, K2, p1t, ps.p2, Cs, s, B, 4LL * p1t * ps.p2, szCt,
                     __LINE__));
-    if (ps.has_P) {
-      LCK(mmb(g_tensor_prec, nT, (int)cols, nT, ps.Um, nT, (long long)nT * nT, Cs, cols, szCt, Cp, cols, szCt, B, s,
-              true));
-      LCK(mmb(g_tensor_prec, nT, (int)cols, nT, ps.Um, nT, (long long)nT * nT, Cp, cols, szCt, Cs, cols, szCt, B, s));
+    if (!ps.has_P) {
+      // Cs viewed as [p1t x (s1 cols)]
+      LCK(mmb(g_tensor_prec, ps.p1, (int)(2 * cols), p1t, ps.Ra, p1t, (long long)ps.p1 * p1t, Cs, 2 * cols, szCt,
+              Cu, 2 * cols, 2LL * ps.p1 * cols, B, s));
+      ps.nc = 2 * ps.p1;
+      return MPDO_OK;
     }
-    // Cs viewed as [p1t x (s1 cols)]
-    LCK(mmb(g_tensor_prec, ps.p1, (int)(2 * cols), p1t, ps.Ra, p1t, (long long)ps.p1 * p1t, Cs, 2 * cols, szCt, Cu,
-            2 * cols, 2LL * ps.p1 * cols, B, s));
+    const int mq = ps.mq;
+    LCK(mmb(g_tensor_prec, mq, (int)cols, nT, ps.Um, mq, (long long)nT * mq, Cs, cols, szCt, Cp, cols,
+            (long long)mq * cols, B, s, true));
+    LCK(mmb(g_tensor_prec, ps.pz, (int)cols, mq, ps.Rz, mq, (long long)ps.pz * mq, Cp, cols, (long long)mq * cols,
+            Cu, cols, (long long)ps.pz * cols, B, s));
+    ps.nc = ps.pz;
     return MPDO_OK;
   };
 
+  t_kept.clear();
   // ---------------- phase 2: QR sweep left to right ----------------
   // Rcur: [B][kp][l] with kp the QR bond so far (1 at the left end)
   int kcur = 1;
@@ -832,16 +849,24 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
       LCK(gram_rows_b(Cs, nT, (int)(2LL * K2 * ps.p2), szCt, G, B, s));
       LST(eig_b(nT, B, G, eE, s));
       LCK(K_KEEP(eE.w, nT, nT, len_gate, -1, GATE_EPS, 0, dk, ddisc + (size_t)B * (nsteps + n + i), B, s));
-      // the projector onto the kept directions is applied only when some chain drops one
+      // the projector onto the kept directions is applied only when some chain drops one; the
+      // core then lives in the mq-dimensional span of the kept directions (mq = max over chains)
       ps.has_P = dry;
+      ps.mq = nT;
       if (!dry) {
         std::vector<int> kept(B);
         LCK(hipMemcpyAsync(kept.data(), dk, sizeof(int) * B, hipMemcpyDeviceToHost, s));
         LCK(hipStreamSynchronize(s));
-        for (int b = 0; b < B; ++b) ps.has_P = ps.has_P || kept[b] < nT;
+        ps.mq = 1;
+        for (int b = 0; b < B; ++b) {
+          ps.has_P = ps.has_P || kept[b] < nT;
+          ps.mq = std::max(ps.mq, std::min(kept[b], nT));
+        }
       }
+      if (!dry) t_kept.push_back(ps.has_P ? ps.mq : nT);
       if (ps.has_P)
-        LCK(k_mask(eE.V, nT, (long long)nT * nT, nT, nT, dk, ps.Um, nT, (long long)nT * nT, B, s, __LINE__));
+        LCK(k_mask(eE.V, nT, (long long)nT * nT, nT, ps.mq, dk, ps.Um, ps.mq, (long long)nT * ps.mq, B, s,
+                   __LINE__));
     }
     // (2) QR sweep through the pair: X1r = (R (x) I_a1) Q1t = Q1 Ra, [(kcur a1) x p1t]
     LCK(mmb(g_tensor_prec, kcur, a1 * p1t, l, Rcur, l, (long long)kcur * l, Q1t, a1 * p1t,
@@ -851,15 +876,28 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     } else {
       LST(qr_b(X1, ps.rows1, p1t, ps.Q1, ps.Ra, B, qw, s));
     }
+    if (ps.has_P) {
+      // Za = (Ra (x) I_s) Um: Um viewed as [p1t x (s1 mq)] -> [p1 x (s1 mq)] = [(p s1) x mq] = Qz Rz
+      const int mq = ps.mq, nz = 2 * ps.p1;
+      LCK(mmb(g_tensor_prec, ps.p1, 2 * mq, p1t, ps.Ra, p1t, (long long)ps.p1 * p1t, ps.Um, 2 * mq,
+              (long long)2 * p1t * mq, Cp, 2 * mq, (long long)nz * mq, B, s));
+      ps.qzid = nz <= mq;
+      ps.pz = std::min(nz, mq);
+      if (ps.qzid) {
+        LCK(K_COPY(ps.Rz, Cp, sizeof(float2) * (size_t)nz * mq * B, s));
+      } else {
+        LST(qr_b(Cp, nz, mq, ps.Qz, ps.Rz, B, qw, s));
+      }
+    }
     if (!ps.has_R) continue;
-    // (3) the core of R theta in the basis of Q1: Cu = (Ra (x) I_s) P C_theta, [(p1 s1) x (s2 K2 p2)]
+    // (3) the core of R theta: (Q1 (x) I) Qc Rc with Rc = [nc x (s2 K2 p2)] in Cu
     LST(make_core(ps, (int)pi - 1));
-    const long long szC = 2LL * ps.p1 * 2 * K2 * ps.p2;
+    const long long szC = (long long)ps.nc * 2 * K2 * ps.p2;
     // right-bond Gram of R theta, all in complex128 so that it is Hermitian to fp64 rounding:
     // Om = C_v^H C_v with C_v = [(2 p1 2 K2) x p2]; GR = sum_{q a2} conj(Q2[q a2, r]) (Om Q2)[q a2, r']
     {
       GemmArgs g{};
-      const int rowsv = 2 * ps.p1 * 2 * K2;
+      const int rowsv = ps.nc * 2 * K2;
       g.M = ps.p2; g.N = ps.p2; g.K = rowsv; g.batch = B;
       g.A = Cu; g.rsA = 1; g.csA = ps.p2; g.bsA = szC; g.conjA = 1;
       g.B = Cu; g.rsB = ps.p2; g.csB = 1; g.bsB = szC;
@@ -1014,12 +1052,12 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
       bsL3 = (long long)ps.p2 * ps.p2;
       Q3p = Q3;
     }
-    // the pair core again, then C'' = C_v [(2p1 2K2) x p2] L3 [p2 x p3]
-    const long long szC = 2LL * ps.p1 * 2 * K2 * ps.p2, szC2 = 2LL * ps.p1 * 2 * K2 * p3;
+    // the pair core again, then C'' = Rc [(nc 2K2) x p2] L3 [p2 x p3]
     LST(make_core(ps, pidx + 1));
-    LCK(mmb(g_tensor_prec, 2 * ps.p1 * 2 * K2, p3, ps.p2, Cu, ps.p2, szC, L3p, p3, bsL3, C2, p3, szC2, B, s));
+    const int nC = ps.nc;
+    const long long szC = (long long)nC * 2 * K2 * ps.p2, szC2 = (long long)nC * 2 * K2 * p3;
+    LCK(mmb(g_tensor_prec, nC * 2 * K2, p3, ps.p2, Cu, ps.p2, szC, L3p, p3, bsL3, C2, p3, szC2, B, s));
     // core Gram and truncated SVD
-    const int nC = 2 * ps.p1;
     const long long colsC = 2LL * K2 * p3;
     LCK(gram_rows_b(C2, nC, (int)colsC, szC2, G, B, s));
     LST(eig_b(nC, B, G, eE, s));
@@ -1028,9 +1066,15 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     LCK(K_FACTOR(eE.V, nC, (long long)nC * nC, eE.w, nC, nC, nC, kp, 0.25, SPLIT_FLOOR, 0, FL, kp, (long long)nC * kp, B, s));
     const float2* Zs = FL;
     long long bsZ = (long long)nC * kp;
+    if (ps.has_P && !ps.qzid) {   // back to Q1's (p s1) basis: Qz [2 p1 x pz] U sqrt(S)
+      LCK(mmb(g_tensor_prec, 2 * ps.p1, kp, ps.pz, ps.Qz, ps.pz, (long long)2 * ps.p1 * ps.pz, FL, kp, bsZ, FLq, kp,
+              (long long)2 * ps.p1 * kp, B, s));
+      Zs = FLq;
+      bsZ = (long long)2 * ps.p1 * kp;
+    }
     if (!ps.q1id) {
-      LCK(mmb(g_tensor_prec, ps.rows1, 2 * kp, ps.p1, ps.Q1, ps.p1, (long long)ps.rows1 * ps.p1, FL, 2 * kp,
-              (long long)nC * kp, Tmin, 2 * kp, (long long)ps.rows1 * 2 * kp, B, s));
+      LCK(mmb(g_tensor_prec, ps.rows1, 2 * kp, ps.p1, ps.Q1, ps.p1, (long long)ps.rows1 * ps.p1, Zs, 2 * kp,
+              bsZ, Tmin, 2 * kp, (long long)ps.rows1 * 2 * kp, B, s));
       Zs = Tmin;
       bsZ = (long long)ps.rows1 * 2 * kp;
     }
@@ -1120,6 +1164,12 @@ mpdo_status mpdo_layer_batched(int batch, int n, const mpdo_site* in, const floa
     if (out[i].l != P.out[i].l || out[i].a != P.out[i].a || out[i].r != P.out[i].r)
       return layer_fail(MPDO_EINVAL, "output descriptors must carry mpdo_layer_dims dims");
   return run_layer(batch, P, in, kraus1, gates2, out, discarded, (hipStream_t)stream, false);
+}
+
+int mpdo_debug_layer_kept(int* out, int cap) {
+  const int n = (int)t_kept.size();
+  for (int i = 0; i < n && i < cap; ++i) out[i] = t_kept[i];
+  return n;
 }
 
 mpdo_status mpdo_debug_layer_dry_run(int batch, int n, const mpdo_site* in, int K1, int parity, int K2, int chi,

@@ -62,6 +62,7 @@ def load() -> ctypes.CDLL:
         'mpdo_debug_eigh': (I, [I, I, VP, P(D), VP, P(I), VP]),
         'mpdo_debug_layer_dry_run': (I, [I, I, SP, I, I, I, I, I, SP, P(ctypes.c_ulonglong)]),
         'mpdo_debug_prof_enable': (I, [I]),
+        'mpdo_debug_layer_kept': (I, [P(I), I]),
         'mpdo_debug_prof_read': (I, [I, P(ctypes.c_longlong), P(D), P(D), P(D)]),
     }
     for name, (res, args) in sig.items():
@@ -407,4 +408,12 @@ def prof_read() -> dict:
               'mpdo_debug_prof_read')
         out[name] = (int(n.value), float(ms.value), float(fl.value), float(by.value))
     return out
+
+
+def layer_kept() -> List[int]:
+    """Per pair of the last fused layer on this thread: the largest kept count of its gate-time
+    split over the chains."""
+    buf = (ctypes.c_int * 1024)()
+    n = load().mpdo_debug_layer_kept(buf, 1024)
+    return [int(buf[i]) for i in range(min(n, 1024))]
 

@@ -162,17 +162,24 @@ def apply_ops(circuit, ops: Sequence[Op]) -> None:
 def steady_state(dims: Sequence[Tuple[int, int, int]], seed: int = 0, decay: float = 0.97) -> List[np.ndarray]:
     """Synthetic chain with the given site shapes (l, a, r): random complex64 site tensors
     T[l, 2, a, r] whose bond and inner indices carry geometrically decaying weights (a physical,
-    non-flat spectrum), each of unit Frobenius norm. ``batched.steady_dims`` gives the shapes a long
-    brickwork circuit settles into."""
+    non-flat spectrum), each site scaled so that the running transfer-matrix trace stays 1
+    (Tr rho = 1, as for a state evolved from |0..0> by trace-preserving channels).
+    ``batched.steady_dims`` gives the shapes a long brickwork circuit settles into."""
     rng = np.random.Generator(np.random.PCG64(seed))
     out = []
+    env = np.ones((1, 1), dtype=np.complex128)
     for (l, a, r) in dims:
-        T = (rng.standard_normal((l, 2, a, r)) + 1j * rng.standard_normal((l, 2, a, r))).astype(np.complex64)
-        T *= (decay ** np.arange(l, dtype=np.float32))[:, None, None, None]
-        T *= (decay ** np.arange(r, dtype=np.float32))[None, None, None, :]
-        T *= (0.5 ** np.arange(a, dtype=np.float32))[None, None, :, None]
-        T /= np.linalg.norm(T)
-        out.append(T)
+        T = rng.standard_normal((l, 2, a, r)) + 1j * rng.standard_normal((l, 2, a, r))
+        T *= (decay ** np.arange(l))[:, None, None, None]
+        T *= (decay ** np.arange(r))[None, None, None, :]
+        T *= (0.5 ** np.arange(a))[None, None, :, None]
+        if env.shape[0] != l:      # a segment cut out of a longer chain: start from the identity
+            env = np.eye(l, dtype=np.complex128) / l
+        nxt = np.einsum('xy,xsar,ysaz->rz', env, T, T.conj(), optimize=True)
+        f = float(np.trace(nxt).real)
+        T /= np.sqrt(f)
+        env = nxt / f
+        out.append(T.astype(np.complex64))
     return out
 
 

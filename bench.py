@@ -177,6 +177,13 @@ def main():
     prof = _lib.prof_read()
     _lib.prof_enable(False)
     print(f'# gate-time split kept counts of the last layer: {_lib.layer_kept()}', file=sys.stderr, flush=True)
+    eigs = {}
+    for (n_e, b_e, sw_e) in _lib.prof_eig_log():
+        c = eigs.setdefault(n_e, [0, 0])
+        c[0] += 1
+        c[1] += sw_e
+    print('# blocked eigensolver calls (order: calls, mean sweeps): ' +
+          ', '.join(f'{k}: {v[0]}, {v[1] / v[0]:.1f}' for k, v in sorted(eigs.items())), file=sys.stderr, flush=True)
 
     # dominant kernel class by summed launch time among those with algorithmic flops
     name, (n_l, ms, fl, by) = max(((k, v) for k, v in prof.items() if v[2] > 0), key=lambda kv: kv[1][1])

@@ -46,6 +46,9 @@ int mpdo_debug_layer_kept(int* out, int cap);
  * bytes. */
 mpdo_status mpdo_debug_prof_enable(int on);
 mpdo_status mpdo_debug_prof_read(int cls, long long* launches, double* ms, double* flops, double* bytes);
+/* The blocked eigensolver calls (order > 64) made while timing was on: (order, batch, sweeps)
+ * triples, up to cap ints written; returns the number of calls. */
+int mpdo_debug_prof_eig_log(int* out, int cap);
 
 #ifdef __cplusplus
 }

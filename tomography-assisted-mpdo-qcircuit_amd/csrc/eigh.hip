@@ -40,10 +40,22 @@ __device__ inline void pair_of(int N, int r, int i, int& p, int& q) {
   else { p = (r + i) % (N - 1); q = (r - i + (N - 1)) % (N - 1); }
 }
 
-// One parallel Jacobi round on the LDS matrix A (order N, even) accumulating into V.
-// Returns through `cnt` (LDS) the number of rotations applied.
-__device__ void jacobi_round(double2 (*A)[LD], double2 (*V)[LD], Rot* rot, int N, int r,
-                             double floor_abs, int* cnt) {
+// One parallel Jacobi round on the LDS matrix A (order N, even) accumulating into V: rotations
+// for the round-robin pairs (one per thread, skipped below the thresholds), then one pass over
+// A: the round's pairs partition the indices, so A <- J^H A J is a 2x2-block update
+// B <- J_i^H B J_j of every block (pair i, pair j), each owned by one thread; V <- V J likewise
+// by (row, pair). Two barriers per round; `cnt` (LDS) counts the rotations applied.
+__device__ inline double2 rot_x(const Rot& R, double2 x, double2 y, bool left, bool second) {
+  // right (columns, w conjugated): x' = cs x - sn conj(w) y,  y' = sn x + cs conj(w) y
+  // left (rows):                   x' = cs x - sn w y,        y' = sn x + cs w y
+  const double wi = left ? R.wi : -R.wi;
+  const double2 wy = make_double2(R.wr * y.x - wi * y.y, R.wr * y.y + wi * y.x);
+  return second ? make_double2(R.sn * x.x + R.cs * wy.x, R.sn * x.y + R.cs * wy.y)
+                : make_double2(R.cs * x.x - R.sn * wy.x, R.cs * x.y - R.sn * wy.y);
+}
+
+__device__ void jacobi_round_blk(double2 (*A)[LD], double2 (*V)[LD], Rot* rot, int N, int r, double floor_abs,
+                                 int* cnt) {
   const int tid = threadIdx.x, nt = blockDim.x;
   const int np = N / 2;
   if (tid < np) {
@@ -66,49 +78,44 @@ __device__ void jacobi_round(double2 (*A)[LD], double2 (*V)[LD], Rot* rot, int N
     rot[tid] = R;
   }
   __syncthreads();
-  // columns: A <- A J, V <- V J with J = [[cs, sn], [-sn conj(w), cs conj(w)]]
-  for (int task = tid; task < np * N; task += nt) {
-    int i = task / N, k = task % N;
-    Rot R = rot[i];
-    if (!R.on) continue;
-    int p, q;
-    pair_of(N, r, i, p, q);
-    if (p > q) { int t = p; p = q; q = t; }
-    // conj(w) * x
-    double2 x = A[k][p], y = A[k][q];
-    double2 wy = make_double2(R.wr * y.x + R.wi * y.y, R.wr * y.y - R.wi * y.x);
-    A[k][p] = make_double2(R.cs * x.x - R.sn * wy.x, R.cs * x.y - R.sn * wy.y);
-    A[k][q] = make_double2(R.sn * x.x + R.cs * wy.x, R.sn * x.y + R.cs * wy.y);
-    x = V[k][p]; y = V[k][q];
-    wy = make_double2(R.wr * y.x + R.wi * y.y, R.wr * y.y - R.wi * y.x);
-    V[k][p] = make_double2(R.cs * x.x - R.sn * wy.x, R.cs * x.y - R.sn * wy.y);
-    V[k][q] = make_double2(R.sn * x.x + R.cs * wy.x, R.sn * x.y + R.cs * wy.y);
-  }
-  __syncthreads();
-  // rows: A <- J^H A with J^H = [[cs, -sn w], [sn, cs w]]
-  for (int task = tid; task < np * N; task += nt) {
-    int i = task / N, k = task % N;
-    Rot R = rot[i];
-    if (!R.on) continue;
-    int p, q;
-    pair_of(N, r, i, p, q);
-    if (p > q) { int t = p; p = q; q = t; }
-    double2 x = A[p][k], y = A[q][k];
-    double2 wy = make_double2(R.wr * y.x - R.wi * y.y, R.wr * y.y + R.wi * y.x);
-    A[p][k] = make_double2(R.cs * x.x - R.sn * wy.x, R.cs * x.y - R.sn * wy.y);
-    A[q][k] = make_double2(R.sn * x.x + R.cs * wy.x, R.sn * x.y + R.cs * wy.y);
-  }
-  __syncthreads();
-  if (tid < np) {
-    Rot R = rot[tid];
-    if (R.on) {
-      int p, q;
-      pair_of(N, r, tid, p, q);
-      if (p > q) { int t = p; p = q; q = t; }
-      A[p][q] = make_double2(0, 0);
-      A[q][p] = make_double2(0, 0);
-      A[p][p].y = 0; A[q][q].y = 0;
+  for (int task = tid; task < np * np; task += nt) {
+    const int bi = task / np, bj = task % np;
+    const Rot Ri = rot[bi], Rj = rot[bj];
+    if (!Ri.on && !Rj.on) continue;
+    int pi, qi, pj, qj;
+    pair_of(N, r, bi, pi, qi);
+    if (pi > qi) { int t = pi; pi = qi; qi = t; }
+    pair_of(N, r, bj, pj, qj);
+    if (pj > qj) { int t = pj; pj = qj; qj = t; }
+    double2 b00 = A[pi][pj], b01 = A[pi][qj], b10 = A[qi][pj], b11 = A[qi][qj];
+    if (Rj.on) {   // B J_j
+      double2 x0 = rot_x(Rj, b00, b01, false, false), y0 = rot_x(Rj, b00, b01, false, true);
+      double2 x1 = rot_x(Rj, b10, b11, false, false), y1 = rot_x(Rj, b10, b11, false, true);
+      b00 = x0; b01 = y0; b10 = x1; b11 = y1;
     }
+    if (Ri.on) {   // J_i^H B
+      double2 x0 = rot_x(Ri, b00, b10, true, false), y0 = rot_x(Ri, b00, b10, true, true);
+      double2 x1 = rot_x(Ri, b01, b11, true, false), y1 = rot_x(Ri, b01, b11, true, true);
+      b00 = x0; b10 = y0; b01 = x1; b11 = y1;
+    }
+    if (bi == bj) {   // the rotated pair: exactly diagonal, real diagonal
+      b01 = make_double2(0, 0);
+      b10 = make_double2(0, 0);
+      b00.y = 0;
+      b11.y = 0;
+    }
+    A[pi][pj] = b00; A[pi][qj] = b01; A[qi][pj] = b10; A[qi][qj] = b11;
+  }
+  for (int task = tid; task < np * N; task += nt) {
+    const int i = task / N, k = task % N;
+    const Rot R = rot[i];
+    if (!R.on) continue;
+    int p, q;
+    pair_of(N, r, i, p, q);
+    if (p > q) { int t = p; p = q; q = t; }
+    const double2 x = V[k][p], y = V[k][q];
+    V[k][p] = rot_x(R, x, y, false, false);
+    V[k][q] = rot_x(R, x, y, false, true);
   }
   __syncthreads();
 }
@@ -155,7 +162,7 @@ __global__ __launch_bounds__(256) void eigh_small_kernel(int n, double2* Ag, lon
   for (; sw < max_sweeps; ++sw) {
     if (tid == 0) cnt = 0;
     __syncthreads();
-    for (int r = 0; r < N - 1; ++r) jacobi_round(A, V, rot, N, r, fl, &cnt);
+    for (int r = 0; r < N - 1; ++r) jacobi_round_blk(A, V, rot, N, r, fl, &cnt);
     if (cnt == 0) break;
     __syncthreads();
   }
@@ -247,7 +254,7 @@ __global__ __launch_bounds__(256) void bj_solve(BJ s, int r, int inner_sweeps) {
   const double fl = s.floor_abs[b];
   (void)red;
   for (int sw = 0; sw < inner_sweeps; ++sw) {
-    for (int rr = 0; rr < NS - 1; ++rr) jacobi_round(A, V, rot, NS, rr, fl, &cnt);
+    for (int rr = 0; rr < NS - 1; ++rr) jacobi_round_blk(A, V, rot, NS, rr, fl, &cnt);
   }
   double2* Wb = s.W + ((long long)b * (P / 2) + pi) * NS * NS;
   for (int e = tid; e < NS * NS; e += blockDim.x) Wb[e] = V[e / NS][e % NS];
@@ -257,12 +264,11 @@ __global__ __launch_bounds__(256) void bj_solve(BJ s, int r, int inner_sweeps) {
 // X[rows, K u L] <- X[rows, K u L] * W_KL   (cols=1)   or
 // X[K u L, cols] <- W_KL^H * X[K u L, cols] (cols=0), in place, 64x64 tile per workgroup.
 // blockIdx.z selects the matrix: z < batch -> A[z], else V[z - batch] (columns only).
-__global__ __launch_bounds__(256) void bj_rotate(BJ s, int r, int cols) {
+__device__ __forceinline__ void rotate_body(BJ s, int r, int cols, int z) {
   // only the 64x64 tile of M is staged in LDS (66.5 KiB, two workgroups per CU); the rotation W
   // of the block pair is read from global memory, where every tile of the pair finds it in L2
   __shared__ double2 X[NS][LD];
   const int P = s.N / 32, pi = blockIdx.y, t0 = blockIdx.x * NS, tid = threadIdx.x;
-  int z = blockIdx.z;
   bool isV = z >= s.batch;
   int b = isV ? z - s.batch : z;
   int I, J;
@@ -338,6 +344,103 @@ __global__ __launch_bounds__(256) void bj_rotate(BJ s, int r, int cols) {
           int gi = (i < 32 ? I * 32 + i : J * 32 + i - 32);
           M[(long long)gi * s.N + t0 + j] = o;
         }
+      }
+}
+
+// X <- X J (cols = 1) or J^H X (cols = 0) on the block pair's columns/rows; blockIdx.z selects
+// the matrix (z < batch: A, else V)
+__global__ __launch_bounds__(256) void bj_rotate(BJ s, int r, int cols) { rotate_body(s, r, cols, blockIdx.z); }
+// V <- V J on the columns of the round's block pairs (blockIdx.z = matrix)
+__global__ __launch_bounds__(256) void bj_rotate_v(BJ s, int r) { rotate_body(s, r, 1, blockIdx.z + s.batch); }
+
+// Two-sided update of A by the round's block rotations, one 64x64 block pair per workgroup:
+// A[R, C] <- W_R^H A[R, C] W_C for the lower block triangle (R >= C in pair order), the block
+// A[C, R] written as its conjugate transpose (A stays Hermitian), so every block of A is read
+// and written once per round. R = I_pr u J_pr, C = I_pc u J_pc (two 32-wide blocks each).
+__device__ inline void mfma_tile64(const double2 (*X)[LD], const double2* Wg, bool w_left, doublex4 (&re)[2][2],
+                                   doublex4 (&im)[2][2], int r0, int c0, int li, int lk) {
+  // w_left: out = W^H X (P[i][k] = conj(W[k][i])); else out = X W (Q[k][j] = W[k][j])
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      re[a][c] = doublex4{0, 0, 0, 0};
+      im[a][c] = doublex4{0, 0, 0, 0};
+    }
+  for (int kk = 0; kk < NS; kk += 4) {
+    const int k = kk + lk;
+    double pr[2], pi_[2], pn[2], qr[2], qi[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const int row = r0 + 16 * a + li;
+      double2 v;
+      if (w_left) { v = Wg[k * NS + row]; v.y = -v.y; }
+      else v = X[row][k];
+      pr[a] = v.x; pi_[a] = v.y; pn[a] = -v.y;
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int col = c0 + 16 * c + li;
+      double2 v = w_left ? X[k][col] : Wg[k * NS + col];
+      qr[c] = v.x; qi[c] = v.y;
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        re[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(pr[a], qr[c], re[a][c], 0, 0, 0);
+        re[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(pn[a], qi[c], re[a][c], 0, 0, 0);
+        im[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(pr[a], qi[c], im[a][c], 0, 0, 0);
+        im[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(pi_[a], qr[c], im[a][c], 0, 0, 0);
+      }
+  }
+}
+
+__global__ __launch_bounds__(256) void bj_sym(BJ s, int r) {
+  __shared__ double2 X[NS][LD];
+  const int P = s.N / 32, h = P / 2, tid = threadIdx.x, b = blockIdx.y;
+  // lower-triangle block pair (pr >= pc) from the linear index
+  int e = blockIdx.x, pr = 0;
+  while (e >= pr + 1) { e -= pr + 1; ++pr; }
+  const int pc = e;
+  (void)h;
+  int Ir, Jr, Ic, Jc;
+  block_pair(P, r, pr, Ir, Jr);
+  block_pair(P, r, pc, Ic, Jc);
+  double2* A = s.A + (long long)b * s.N * s.N;
+  const double2* Wr = s.W + ((long long)b * (P / 2) + pr) * NS * NS;
+  const double2* Wc = s.W + ((long long)b * (P / 2) + pc) * NS * NS;
+  auto grow = [&](int i) { return i < 32 ? Ir * 32 + i : Jr * 32 + i - 32; };
+  auto gcol = [&](int j) { return j < 32 ? Ic * 32 + j : Jc * 32 + j - 32; };
+  for (int q = tid; q < NS * NS; q += blockDim.x) {
+    int i = q / NS, j = q % NS;
+    X[i][j] = A[(long long)grow(i) * s.N + gcol(j)];
+  }
+  __syncthreads();
+  const int wave = tid / 64, lane = tid % 64, li = lane % 16, lk = lane / 16;
+  const int r0 = (wave / 2) * 32, c0 = (wave % 2) * 32;
+  doublex4 re[2][2], im[2][2];
+  mfma_tile64(X, Wr, true, re, im, r0, c0, li, lk);     // T = W_R^H X
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        X[r0 + 16 * a + 4 * q + lk][c0 + 16 * c + li] = make_double2(re[a][c][q], im[a][c][q]);
+  __syncthreads();
+  mfma_tile64(X, Wc, false, re, im, r0, c0, li, lk);    // T W_C
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = r0 + 16 * a + 4 * q + lk, j = c0 + 16 * c + li;
+        const double2 o = make_double2(re[a][c][q], im[a][c][q]);
+        A[(long long)grow(i) * s.N + gcol(j)] = o;
+        if (pr != pc) A[(long long)gcol(j) * s.N + grow(i)] = make_double2(o.x, -o.y);
       }
 }
 
@@ -426,19 +529,20 @@ hipError_t eigh(int n, int batch, void* A, long long lda, long long bsA, double*
     const double solve_flops = (double)(P / 2) * batch * 63.0 * 32.0 * 192.0 * 18.0;
     const double solve_bytes = (double)(P / 2) * batch * 2.0 * NS * NS * 16.0;
     const double tile_flops = 8.0 * NS * NS * NS, tile_bytes = 3.0 * NS * NS * 16.0;
-    const double tiles_c = (double)(s.N / NS) * (P / 2) * 2 * batch, tiles_r = (double)(s.N / NS) * (P / 2) * batch;
+    const int npairs_sym = (P / 2) * (P / 2 + 1) / 2;
+    const double tiles_sym = (double)npairs_sym * batch, tiles_r = (double)(s.N / NS) * (P / 2) * batch;
     for (int r = 0; r < P - 1; ++r) {
       {
         prof::Scope ps(prof::EIG_SOLVE, st, solve_flops, solve_bytes);
         hipLaunchKernelGGL(bj_solve, dim3(P / 2, batch), dim3(256), 0, st, s, r, 1);
       }
-      {
-        prof::Scope ps(prof::EIG_ROTATE, st, tiles_c * tile_flops, tiles_c * tile_bytes);
-        hipLaunchKernelGGL(bj_rotate, dim3(s.N / NS, P / 2, 2 * batch), dim3(256), 0, st, s, r, 1);
+      {   // A <- J^H A J block pair by block pair (Hermitian: lower triangle + mirror)
+        prof::Scope ps(prof::EIG_ROTATE, st, tiles_sym * 2 * tile_flops, tiles_sym * tile_bytes);
+        hipLaunchKernelGGL(bj_sym, dim3(npairs_sym, batch), dim3(256), 0, st, s, r);
       }
-      {
+      {   // V <- V J: the columns of the round's block pairs, all rows
         prof::Scope ps(prof::EIG_ROTATE, st, tiles_r * tile_flops, tiles_r * tile_bytes);
-        hipLaunchKernelGGL(bj_rotate, dim3(s.N / NS, P / 2, batch), dim3(256), 0, st, s, r, 0);
+        hipLaunchKernelGGL(bj_rotate_v, dim3(s.N / NS, P / 2, batch), dim3(256), 0, st, s, r);
       }
     }
     if ((err = hipGetLastError()) != hipSuccess) return err;
@@ -451,6 +555,7 @@ hipError_t eigh(int n, int batch, void* A, long long lda, long long bsA, double*
   }
   hipLaunchKernelGGL(bj_finish, dim3(batch), dim3(256), 0, st, s, w, ldw, (double2*)V, ldv, bsV, order);
   if (sweeps_out) *sweeps_out = sw + 1;
+  prof::eig_log(n, batch, sw + 1);
   return hipGetLastError();
 }
 

@@ -14,10 +14,11 @@
 //   X1r = (R (x) I) Q1t = Q1 Ra and R_max, a factor of the right-bond Gram of R theta', which is
 //   sum_a Q2_a^H (Cu^H Cu summed over the channel/physical index) Q2_a with the core
 //   Cu = (Ra (x) I) P C_theta. The chi step at the pair bond is the truncated SVD of
-//   Theta = R theta' R_max^{-1} P_right = (Q1 (x) I) C'' (I (x) Q3) (Q3 from the LQ of Q2 P_right),
-//   read off the Gram of the core C'' = Cu L3: U sqrt(S) gives the new min site, sqrt(S) V^H the
-//   new max site. The kappa cut of every site runs as soon as the site is final, so the large
-//   max-site tensor exists for one pair at a time.
+//   Theta = R theta' R_max^{-1} P_right = (Q1 (x) I) Cu (I (x) Q2 P_right), read off the Gram
+//   Cu (I (x) H) Cu^H of the core (H the Gram of Q2 P_right, so that factor is never factored):
+//   U sqrt(S) gives the new min site, S^{-1/2} U^H Cu (I (x) Q2 P_right) the new max site. The
+//   kappa cut of every site runs as soon as the site is final, so the large max-site tensor
+//   exists for one pair at a time.
 // Every stage is batched over the chains (same shapes; rank rules deterministic because no
 // error threshold is used).
 #include "mpdo.h"
@@ -526,10 +527,10 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     }
   // sizes (elements per chain) of the shared scratch buffers: each the maximum over its uses
   auto kapplies = [&](int i) { return P.out[i].a != P.chi_out[i].a; };
-  size_t mxQR_rows = 1, mxQR_cols = 1, mxR = 1, mxY = 1, mxOm = 1, mxZ = 1, mxQ2P = 1, mxC = 1, mxC2 = 1;
+  size_t mxQR_rows = 1, mxQR_cols = 1, mxR = 1, mxY = 1, mxOm = 1, mxZ = 1, mxQ2P = 1, mxC = 1;
   size_t mxTmin = 1, mxWm = 1, mxTm = 1, mxFL = 1, mxPn = 1, mxRT = 1, mxX1 = 1, mxLast = 1, mxTcur = 1;
   size_t mxTchi = 1, mxMp = 1, mxKout = 1, mxX1t = 1, mxQ1t = 1, mxR1t = 1;
-  int mxEigQR = 1, mxEigC = 1, mxEigChi = 1, mxEigK = 1, mxEigR = 1, mxEigL3 = 1;
+  int mxEigQR = 1, mxEigC = 1, mxEigChi = 1, mxEigK = 1, mxEigR = 1;
   for (int i = 0; i < n; ++i) {
     const Dims& c = P.chi_out[i];
     const size_t szc = (size_t)c.l * 2 * c.a * c.r;
@@ -582,19 +583,16 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     mxEigR = std::max(mxEigR, P.in[ps.i + 1].r);
     const int c3 = a2 * kr;
     mxQ2P = std::max(mxQ2P, (size_t)ps.p2 * c3);
-    if (c3 > ps.p2) { mxQR_rows = std::max(mxQR_rows, (size_t)c3); mxQR_cols = std::max(mxQR_cols, (size_t)ps.p2); }
-    mxEigL3 = std::max(mxEigL3, ps.p2);
-    const int p3 = std::min(ps.p2, c3);
-    mxC2 = std::max(mxC2, (size_t)2 * ps.p1 * 2 * K2 * p3);
+
     mxEigC = std::max(mxEigC, 2 * ps.p1);
     mxFL = std::max(mxFL, (size_t)2 * ps.p1 * kp);
     mxTmin = std::max(mxTmin, (size_t)ps.rows1 * 2 * kp);
     mxTcur = std::max(mxTcur, (size_t)ps.kprev * 2 * P.a1[ps.i].a * kp);
-    mxWm = std::max(mxWm, (size_t)kp * 2 * K2 * p3);
-    if (c3 > ps.p2) mxTm = std::max(mxTm, (size_t)kp * 2 * K2 * c3);
+    mxWm = std::max(mxWm, (size_t)kp * 2 * K2 * ps.p2);
+    mxTm = std::max(mxTm, (size_t)kp * 2 * K2 * c3);
     mxPn = std::max(mxPn, (size_t)P.in[ps.i + 1].r * std::max(kr, P.in[ps.i + 1].r));
   }
-  const int mxEig = std::max({mxEigQR, mxEigC, mxEigChi, mxEigK, mxEigR, mxEigL3});
+  const int mxEig = std::max({mxEigQR, mxEigC, mxEigChi, mxEigK, mxEigR});
   size_t mxBond = 1;
   for (int i = 0; i < n; ++i) {
     mxBond = std::max(mxBond, (size_t)std::max(P.in[i].l, P.in[i].r));
@@ -607,9 +605,9 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
   // device buffers
   std::vector<float2*> T1(n, nullptr), Astore(n, nullptr);
   Site1q* dtab = nullptr;
-  float2 *Mlast = nullptr, *FLq, *Rcur, *Rnext, *RT, *X1, *X1t, *Q1t, *R1t, *Cs, *Cp, *Cu, *Zb, *Q2P, *L3, *Q3, *C2, *FL, *FR, *Tmin, *Wm,
+  float2 *Mlast = nullptr, *FLq, *Rcur, *Rnext, *RT, *X1, *X1t, *Q1t, *R1t, *Cs, *Cp, *Cu, *Zb, *Q2P, *FL, *FR, *Tmin, *Wm,
          *Tm, *Pn, *Pn2, *PW, *Pp, *Tchi, *Mp, *Vk, *Kout, *dG, *Tcur2, *dK;
-  double2 *G, *GR, *Om64, *Q2w, *Z64;
+  double2 *G, *GR, *Om64, *Q2w, *Z64, *H64, *Cw64, *Z64c;
   double* ddisc;
   int* dk;
   EigB eE{};
@@ -657,9 +655,9 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     Zb = c.get<float2>(std::max(mxZ, mxBond2) * B);
     dtab = c.get<Site1q>((size_t)n * B);
     Q2P = c.get<float2>(mxQ2P * B);
-    L3 = c.get<float2>(mxOm * B);
-    Q3 = c.get<float2>(mxQ2P * B);
-    C2 = c.get<float2>(mxC2 * B);
+    H64 = c.get<double2>(mxOm * B);
+    Cw64 = c.get<double2>(mxC * B);
+    Z64c = c.get<double2>(mxC * B);
     FL = c.get<float2>(mxFL * B);
     FLq = c.get<float2>(mxFL * B);
     FR = c.get<float2>(mxFL * B);
@@ -1035,31 +1033,24 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
       }
       Qright = Q2P;
     }
-    // LQ of Q2P: = L3 Q3 with orthonormal rows
-    int p3;
-    const float2* L3p;
-    const float2* Q3p;
-    long long bsL3;
-    if (c3 <= ps.p2) {
-      p3 = c3;
-      L3p = Qright;
-      bsL3 = (long long)ps.p2 * c3;
-      Q3p = nullptr;
-    } else {
-      p3 = ps.p2;
-      LST(lq_b(Qright, ps.p2, c3, Q3, L3, B, qw, s));
-      L3p = L3;
-      bsL3 = (long long)ps.p2 * ps.p2;
-      Q3p = Q3;
-    }
-    // the pair core again, then C'' = Rc [(nc 2K2) x p2] L3 [p2 x p3]
+    // Theta = (Q1 (x) I) Qc Rc (I (x) Q2P): its core's Gram is Rc (I (x) H) Rc^H with
+    // H = Q2P Q2P^H [p2 x p2], all in complex128 (Q2P itself is never factored)
+    LCK(gram_rows_b(Qright, ps.p2, c3, (long long)ps.p2 * c3, H64, B, s));
     LST(make_core(ps, pidx + 1));
     const int nC = ps.nc;
-    const long long szC = (long long)nC * 2 * K2 * ps.p2, szC2 = (long long)nC * 2 * K2 * p3;
-    LCK(mmb(g_tensor_prec, nC * 2 * K2, p3, ps.p2, Cu, ps.p2, szC, L3p, p3, bsL3, C2, p3, szC2, B, s));
-    // core Gram and truncated SVD
-    const long long colsC = 2LL * K2 * p3;
-    LCK(gram_rows_b(C2, nC, (int)colsC, szC2, G, B, s));
+    const long long colsR = 2LL * K2 * ps.p2, szC = (long long)nC * colsR;
+    LCK(k_widen(Cu, Cw64, szC, szC, szC, B, s, __LINE__));
+    LCK(mmb(C128_F64ACC, nC * 2 * K2, ps.p2, ps.p2, Cw64, ps.p2, szC, H64, ps.p2, (long long)ps.p2 * ps.p2, Z64c,
+            ps.p2, szC, B, s));
+    {
+      GemmArgs g{};
+      g.M = nC; g.N = nC; g.K = (int)colsR; g.batch = B;
+      g.A = Z64c; g.rsA = colsR; g.csA = 1; g.bsA = szC;
+      g.B = Cw64; g.rsB = 1; g.csB = colsR; g.bsB = szC; g.conjB = 1;
+      g.C = G; g.rsC = nC; g.csC = 1; g.bsC = (long long)nC * nC;
+      g.alpha_re = 1;
+      LCK(K_GEMM(C128_F64ACC, g, s));
+    }
     LST(eig_b(nC, B, G, eE, s));
     LCK(K_KEEP(eE.w, nC, nC, P.step_len[i], kp, -1.0, 1, dk, ddisc + (size_t)B * i, B, s));
     // new min site: (Q1 (x) I) U sqrt(S) -> (kprev, a1, s1, kp) -> (kprev, s1, a1, kp)
@@ -1082,18 +1073,14 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
       int dims[4] = {ps.kprev, a1, 2, kp}, perm[4] = {0, 2, 1, 3};
       LCK(K_PERMUTE(Zs, Tcur2, 4, dims, perm, s, B, bsZ));
     }
-    // new max site: S^{-1/2} U^H C'' (I (x) Q3) -> (kp, s2, K2, a2, kr) -> (kp, s2, a2, K2, kr)
+    // new max site: S^{-1/2} U^H Rc (I (x) Q2P) -> (kp, s2, K2, a2, kr) -> (kp, s2, a2, K2, kr)
     LCK(K_FACTOR(eE.V, nC, (long long)nC * nC, eE.w, nC, nC, nC, kp, -0.25, SPLIT_FLOOR, 1, FR, nC, (long long)kp * nC, B, s));
-    LCK(mmb(g_tensor_prec, kp, (int)colsC, nC, FR, nC, (long long)kp * nC, C2, colsC, szC2, Wm, colsC,
-            kp * colsC, B, s));
-    const float2* Ts = Wm;
-    long long bsT = kp * colsC;
-    if (Q3p) {
-      LCK(mmb(g_tensor_prec, kp * 2 * K2, c3, p3, Wm, p3, kp * colsC, Q3p, c3, (long long)p3 * c3, Tm, c3,
-              (long long)kp * 2 * K2 * c3, B, s));
-      Ts = Tm;
-      bsT = (long long)kp * 2 * K2 * c3;
-    }
+    LCK(mmb(g_tensor_prec, kp, (int)colsR, nC, FR, nC, (long long)kp * nC, Cu, colsR, szC, Wm, colsR,
+            kp * colsR, B, s));
+    LCK(mmb(g_tensor_prec, kp * 2 * K2, c3, ps.p2, Wm, ps.p2, kp * colsR, Qright, c3, (long long)ps.p2 * c3, Tm, c3,
+            (long long)kp * 2 * K2 * c3, B, s));
+    const float2* Ts = Tm;
+    const long long bsT = (long long)kp * 2 * K2 * c3;
     if (kapplies(j)) {   // (kp, s2, K2, a2, kr) -> (kp, s2, kr, a2, K2): inner index last for the kappa cut
       int dims[5] = {kp, 2, K2, a2, kr}, perm[5] = {0, 1, 4, 3, 2};
       LCK(K_PERMUTE(Ts, Mp, 5, dims, perm, s, B, bsT));

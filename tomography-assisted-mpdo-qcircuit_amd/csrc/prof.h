@@ -10,6 +10,8 @@ namespace prof {
 enum Cls { GEMM = 0, EIG_SOLVE = 1, EIG_ROTATE = 2, EIG_SMALL = 3, PERMUTE = 4, NCLS = 5 };
 
 bool enabled();
+// one record per eigensolver call while timing is on: order, batch, sweeps
+void eig_log(int n, int batch, int sweeps);
 
 // RAII: records a start event on construction and a stop event on destruction
 class Scope {

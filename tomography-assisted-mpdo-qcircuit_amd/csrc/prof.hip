@@ -25,6 +25,7 @@ bool g_on = false;
 std::vector<Rec> g_recs;
 std::vector<hipEvent_t> g_free;
 Acc g_acc[NCLS];
+std::vector<int> g_eig;   // (n, batch, sweeps) triples
 
 // fold finished records into the totals (waits for the last one) and recycle their events
 void drain_locked() {
@@ -60,6 +61,14 @@ hipEvent_t take_locked() {
 
 bool enabled() { return g_on; }
 
+void eig_log(int n, int batch, int sweeps) {
+  if (!g_on) return;
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_eig.push_back(n);
+  g_eig.push_back(batch);
+  g_eig.push_back(sweeps);
+}
+
 Scope::Scope(int cls, hipStream_t s, double flops, double bytes) : cls_(cls), s_(s), flops_(flops), bytes_(bytes) {
   if (!g_on) return;
   std::lock_guard<std::mutex> lk(g_mu);
@@ -85,7 +94,10 @@ mpdo_status mpdo_debug_prof_enable(int on) {
   using namespace mpdo::prof;
   std::lock_guard<std::mutex> lk(g_mu);
   drain_locked();
-  for (auto& a : g_acc) a = Acc{};
+  if (on) {
+    for (auto& a : g_acc) a = Acc{};
+    g_eig.clear();
+  }
   g_on = on != 0;
   return MPDO_OK;
 }
@@ -101,6 +113,14 @@ mpdo_status mpdo_debug_prof_read(int cls, long long* launches, double* ms, doubl
   if (flops) *flops = a.flops;
   if (bytes) *bytes = a.bytes;
   return MPDO_OK;
+}
+
+int mpdo_debug_prof_eig_log(int* out, int cap) {
+  using namespace mpdo::prof;
+  std::lock_guard<std::mutex> lk(g_mu);
+  const int n = (int)g_eig.size() / 3;
+  for (int i = 0; i < 3 * n && i < cap; ++i) out[i] = g_eig[i];
+  return n;
 }
 
 }  // extern "C"

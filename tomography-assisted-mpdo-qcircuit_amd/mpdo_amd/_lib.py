@@ -63,6 +63,7 @@ def load() -> ctypes.CDLL:
         'mpdo_debug_layer_dry_run': (I, [I, I, SP, I, I, I, I, I, SP, P(ctypes.c_ulonglong)]),
         'mpdo_debug_prof_enable': (I, [I]),
         'mpdo_debug_layer_kept': (I, [P(I), I]),
+        'mpdo_debug_prof_eig_log': (I, [P(I), I]),
         'mpdo_debug_prof_read': (I, [I, P(ctypes.c_longlong), P(D), P(D), P(D)]),
     }
     for name, (res, args) in sig.items():
@@ -416,4 +417,12 @@ def layer_kept() -> List[int]:
     buf = (ctypes.c_int * 1024)()
     n = load().mpdo_debug_layer_kept(buf, 1024)
     return [int(buf[i]) for i in range(min(n, 1024))]
+
+
+def prof_eig_log() -> List[Tuple[int, int, int]]:
+    """(order, batch, sweeps) of every blocked eigensolver call recorded since prof_enable(True)."""
+    cap = 3 * 65536
+    buf = (ctypes.c_int * cap)()
+    n = load().mpdo_debug_prof_eig_log(buf, cap)
+    return [(int(buf[3 * i]), int(buf[3 * i + 1]), int(buf[3 * i + 2])) for i in range(min(n, cap // 3))]
 

@@ -158,6 +158,28 @@ mpdo_status mpdo_layer_batched(int batch, int n, const mpdo_site* in, const floa
                                double* discarded, void* stream);
 
 /*
+ * Shot sampling (replaces the contractions of Circuit.py:297-332, _conditional_prb, driven by
+ * _conditional_batch_sample / _conditional_sample, Circuit.py:334-391).
+ *
+ * mpdo_env_right: traced tail environments R_i[l, l'] = sum over sites i..n-1 of the network
+ * rho~ with every physical index traced (R_n = 1), complex128, packed back to back: R_i starts at
+ * element sum_{j<i} l_j^2 and is l_i x l_i (ket bond, bra bond); mpdo_env_right_elems gives the
+ * total count. Open boundary bonds must be 1.
+ *
+ * mpdo_sample_step: one site of the left-to-right sweep for `groups` histories. Group g starts
+ * from environment prev[sel[g]] (sel: host array of `groups` indices < prev_count; NULL = prev[g]),
+ * each environment l x l complex128 with l = site->l (the first site starts from the 1 x 1
+ * identity). sampled != 0: out[g][s] = E projected on physical value s (r x r each, s = 0, 1) and
+ * probs[g][s] (host) = Re sum E_s .* R_next, the unnormalised P(history, s) the reference reads
+ * off the diagonal of its contraction; the call synchronises. sampled == 0 (a qubit that is
+ * traced, Circuit.py:299-300,313): out[g] = E_0 + E_1 and probs may be NULL.
+ */
+long long mpdo_env_right_elems(int n, const mpdo_site* sites);
+mpdo_status mpdo_env_right(int n, const mpdo_site* sites, void* out, void* stream);
+mpdo_status mpdo_sample_step(const mpdo_site* site, int groups, const void* prev, int prev_count, const int* sel,
+                             int sampled, const void* R_next, void* out, double* probs, void* stream);
+
+/*
  * State vector of a pure MPDO (every inner dimension 1), amplitudes in ket-bit order with the
  * lowest qubit index most significant. Replaces Circuit.py:264-272 (cal_vector).
  * out: device complex64, 2^n entries (n <= 30).

@@ -26,6 +26,12 @@ mpdo_status mpdo_debug_gemm(int prec, int M, int N, int K, int batch,
  * destroyed): w[b][n] descending, V[b][n][n] with eigenvectors in columns. *sweeps = sweeps used. */
 mpdo_status mpdo_debug_eigh(int n, int batch, void* A, double* w, void* V, int* sweeps, void* stream);
 
+/* The same for positive semidefinite (Gram) matrices through the pivoted-Cholesky-preconditioned
+ * solver the fused layer uses: eigenpairs below the solver's floor (about 1e-14 x n/64 of the
+ * largest eigenvalue) or past the numerical rank come back as zero eigenvalues with zero vectors.
+ * *sweeps = Jacobi sweeps spent on the rank-sized problem. */
+mpdo_status mpdo_debug_eigh_gram(int n, int batch, void* A, double* w, void* V, int* sweeps, void* stream);
+
 /* Host-only rehearsal of mpdo_layer_batched: plans the layer and walks every launch with fake
  * device addresses, checking each operand's extent against the buffer it falls in, without
  * touching a GPU. `in[i].data`/`out[i].data` may be any distinct non-overlapping addresses.

@@ -22,7 +22,10 @@ Parity pinning (how each part is checked, ``tests/test_oracle.py``):
   * gate application, sweeps without truncation, observables: the independent dense complex128
     density-matrix simulator (``oracle/dense.py``);
   * chi cut: the Schmidt values of a dense pure state at the first cut of the sweep;
-    kappa cut: the eigen-decomposition of a single site's density matrix.
+    kappa cut: the eigen-decomposition of a single site's density matrix;
+  * sampling: conditional probabilities against marginals of the dense density matrix (every
+    history, reduced qubits, rotated bases), the draws against the exact distribution
+    (chi-square), the reference's rules for p1 within GLOBAL_MINIMUM of 0 or 1 (no draw).
 Functions whose numbers no such check reaches are marked "parity unpinned" in their docstring.
 """
 from __future__ import annotations
@@ -767,3 +770,132 @@ def trace_rho_rho(st0: MPDO, st1: Optional[MPDO] = None) -> float:
         # rho0[s, s'] rho1[s', s]: ket0 s with bra1 s, bra0 s' with ket1 s'
         env = np.einsum('abcd,asxe,btxf,ctyg,dsyh->efgh', env, A, A.conj(), B, B.conj(), optimize=True)
     return float(np.real(env[0, 0, 0, 0]))
+
+
+# ----------------------------------------------------------------------------------------------
+# Shot sampling (Circuit.py:297-467). The draws use torch's default CPU generator with the
+# reference's own calls, so a seeded run gives the reference's bitstrings for the same
+# probabilities.
+# ----------------------------------------------------------------------------------------------
+MEASURE_ROTATIONS = {0: gate_matrix('H'),                                       # SingleGates.py:240-271
+                     1: np.array([[_S2, -1j * _S2], [_S2, 1j * _S2]])}          # SingleGates.py:274-305
+
+
+def sampling_state(st: MPDO, orientation: Sequence[int]) -> MPDO:
+    """Circuit.py:413-419: a copy of the state with MeasureX (orientation 0) and MeasureY (1)
+    applied as ideal gates. The reference passes the positions in ``orientation`` as the qubit
+    indices (``_indices`` enumerates the orientation list), so with qubits reduced the rotation
+    lands on qubit ``position``; that behaviour is kept."""
+    out = st.copy()
+    for pos, o in enumerate(orientation):
+        if o in MEASURE_ROTATIONS:
+            out.sites[pos] = apply_1q(out.sites[pos], MEASURE_ROTATIONS[o])
+    return out
+
+
+def conditional_probs(st: MPDO, order: Sequence[int], history: Sequence[int]) -> np.ndarray:
+    """Circuit.py:297-326: the unnormalised diagonal the reference reads off its contraction for
+    qubit ``order[len(history)]``: qubits ``order[:k]`` projected on ``history``, the open qubit
+    kept, every other qubit traced (``reduce_dmNodes`` on the reduced and later qubits)."""
+    k = len(history)
+    proj = {q: int(h) for q, h in zip(order[:k], history)}
+    open_q = order[k]
+    env = np.ones((1, 1, 1), dtype=np.complex128)            # [s_open, l, l']
+    for i, T in enumerate(st.sites):
+        if i in proj:
+            Ts = T[:, proj[i]]
+            env = np.einsum('xln,lar,nam->xrm', env, Ts, Ts.conj(), optimize=True)
+        elif i == open_q:
+            env = np.einsum('ln,lsar,nsam->srm', env[0], T, T.conj(), optimize=True)
+        else:
+            env = np.einsum('xln,lsar,nsam->xrm', env, T, T.conj(), optimize=True)
+    return np.real(env[:, 0, 0])
+
+
+def normalise_prb(P: Sequence[float]) -> float:
+    """Circuit.py:328-332 in the arithmetic of a complex64 state: real parts in float32 plus
+    GLOBAL_MINIMUM, RuntimeError on a negative entry, the normalised last entry."""
+    import torch
+    probs = torch.tensor([float(P[0]), float(P[1])], dtype=torch.float32) + GLOBAL_MINIMUM
+    if (probs < 0).sum() > 0:
+        raise RuntimeError(f"State is illegal, and your probability distribution is {probs}.")
+    return (probs / probs.sum())[-1].item()
+
+
+def conditional_batch_sample(st: MPDO, order: Sequence[int], shots: int, bool_result: bool = False,
+                             trace: Optional[list] = None) -> list:
+    """Circuit.py:355-391: groups of shots sharing a history; per group one Bernoulli count
+    (skipped when p1 is within GLOBAL_MINIMUM of 0 or 1), the ones first."""
+    import torch
+    L = len(order)
+    seq = torch.zeros((shots, L), dtype=torch.bool if bool_result else torch.int)
+    val = True if bool_result else 1
+    groups = [([], 0, shots)]
+    for j in range(L):
+        new = []
+        for hist, start, length in groups:
+            p1 = normalise_prb(conditional_probs(st, order, hist))
+            if trace is not None:
+                trace.append((tuple(hist), p1))
+            if p1 < GLOBAL_MINIMUM:
+                k1, k0 = 0, length
+            elif p1 > 1 - GLOBAL_MINIMUM:
+                k1, k0 = length, 0
+            else:
+                k1 = int(torch.bernoulli(torch.full((length,), p1)).sum())
+                k0 = length - k1
+            seq[start:start + k1, j] = val
+            if j < L - 1:
+                if k1 > 0:
+                    new.append((hist + [1], start, k1))
+                if k0 > 0:
+                    new.append((hist + [0], start + k1, k0))
+        groups = new
+    return seq.tolist()
+
+
+def conditional_sample(st: MPDO, order: Sequence[int], shots: int) -> list:
+    """Circuit.py:334-342: shot by shot, qubit by qubit, one two-outcome multinomial draw."""
+    import torch
+    out = []
+    for _ in range(shots):
+        ch = [-1] * len(order)
+        for j in range(len(order)):
+            p = normalise_prb(conditional_probs(st, order, ch[:j]))
+            ch[j] = torch.multinomial(torch.tensor([1 - p, p]), num_samples=1).item()
+        out.append(ch)
+    return out
+
+
+def count_item(data) -> dict:
+    """Tools.py:554-570: occurrence counts sorted by the bitstring's value."""
+    from collections import Counter
+
+    def key(x):
+        if isinstance(x[0], tuple):
+            return int(''.join(str(int(v)) for v in x[0]))
+        if isinstance(x[0], str):
+            return x[0]
+        return int(x[0])
+    counted = dict(Counter(tuple(i) if isinstance(i, list) else i for i in data))
+    return dict(sorted(counted.items(), key=key))
+
+
+def sample(st: MPDO, shots: Optional[int] = None, orientation: Optional[Sequence[int]] = None,
+           reduced: Optional[Sequence[int]] = None, sample_string: bool = True, sequential: bool = False,
+           bool_result: bool = False, counts: bool = True):
+    """Circuit.py:393-445 (TensorCircuit.sample)."""
+    shots = shots or 1024
+    reduced = list(reduced or [])
+    order = [q for q in range(st.n) if q not in reduced]
+    orientation = list(orientation or [2] * len(order))
+    if len(orientation) != len(order):
+        raise ValueError('Length of orientation must match the sample length. Check reduced or unmeasured qubits.')
+    rot = sampling_state(st, orientation)
+    if sequential:
+        bits = conditional_sample(rot, order, shots)
+    else:
+        bits = conditional_batch_sample(rot, order, shots, bool_result)
+    if sample_string and not bool_result:
+        bits = [''.join(map(str, b)) for b in bits]
+    return (bits, count_item(bits)) if counts else bits

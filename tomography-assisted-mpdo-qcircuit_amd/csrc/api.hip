@@ -985,4 +985,18 @@ mpdo_status mpdo_debug_eigh(int n, int batch, void* A, double* w, void* V, int* 
   return MPDO_OK;
 }
 
+mpdo_status mpdo_debug_eigh_gram(int n, int batch, void* A, double* w, void* V, int* sweeps, void* stream) {
+  if (n <= 0 || batch <= 0) return fail(MPDO_EINVAL, "n=%d batch=%d", n, batch);
+  hipStream_t s = S(stream);
+  Carver c0;
+  c0.get<char>(eigh_gram_workspace_bytes(n, batch));
+  char* base;
+  ST(ws_reserve(c0.off, s, &base));
+  int sw = 0;
+  CK(eigh_gram(n, batch, A, n, (long long)n * n, w, n, V, n, (long long)n * n, base, s, &sw));
+  CK(hipStreamSynchronize(s));
+  if (sweeps) *sweeps = sw;
+  return MPDO_OK;
+}
+
 }  // extern "C"

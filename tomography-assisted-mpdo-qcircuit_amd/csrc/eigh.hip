@@ -11,6 +11,7 @@
 // semidefinite Gram matrices this yields eigenvalues with small relative error).
 #include "mpdo_internal.h"
 #include "prof.h"
+#include "site_ops.h"
 
 #include <vector>
 
@@ -556,6 +557,174 @@ hipError_t eigh(int n, int batch, void* A, long long lda, long long bsA, double*
   hipLaunchKernelGGL(bj_finish, dim3(batch), dim3(256), 0, st, s, w, ldw, (double2*)V, ldv, bsV, order);
   if (sweeps_out) *sweeps_out = sw + 1;
   prof::eig_log(n, batch, sw + 1);
+  return hipGetLastError();
+}
+
+}  // namespace mpdo
+
+// =======================================================================================
+// Eigen-decomposition of positive semidefinite Gram matrices, preconditioned by pivoted
+// Cholesky: G = P L L^H P^T stopped at the numerical rank r (largest remaining diagonal at
+// or below the Jacobi floor), the Jacobi method on the r x r matrix M = L^H L (same nonzero
+// eigenvalues, strongly diagonally dominant for graded spectra, so it converges in a few sweeps),
+// eigenvectors L U Lambda^{-1/2}. Directions beyond the rank (and eigenvalues below the floor)
+// come back as zero vectors with zero eigenvalues.
+// =======================================================================================
+namespace mpdo {
+namespace {
+
+constexpr int PC_MAX = 1024;
+
+// L is stored by columns (L[j * n + i] = L_ij) so that the row updates of one column read
+// consecutive addresses across the block.
+__global__ __launch_bounds__(256) void pchol_kernel(const double2* A, int n, long long lda, long long bsA,
+                                                    double tol_rel, double2* L, int* rank) {
+  __shared__ double d[PC_MAX];
+  __shared__ unsigned char chosen[PC_MAX];
+  __shared__ double2 Lp[PC_MAX];
+  __shared__ double rv[256];
+  __shared__ int ri[256];
+  __shared__ int s_p, s_stop;
+  __shared__ double s_thr;
+  const int b = blockIdx.x, tid = threadIdx.x;
+  A += (long long)b * bsA;
+  L += (long long)b * n * n;
+  double m = 0;
+  for (int i = tid; i < n; i += blockDim.x) {
+    d[i] = A[(long long)i * lda + i].x;
+    chosen[i] = 0;
+    m = fmax(m, d[i]);
+  }
+  rv[tid] = m;
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if (tid < st) rv[tid] = fmax(rv[tid], rv[tid + st]);
+    __syncthreads();
+  }
+  if (tid == 0) s_thr = tol_rel * rv[0];
+  __syncthreads();
+  const double thr = s_thr;
+  int k = 0;
+  for (; k < n; ++k) {
+    double best = -1.0;
+    int bi = n;
+    for (int i = tid; i < n; i += blockDim.x)
+      if (!chosen[i] && d[i] > best) { best = d[i]; bi = i; }
+    rv[tid] = best;
+    ri[tid] = bi;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+      if (tid < st) {
+        const double o = rv[tid + st];
+        const int oi = ri[tid + st];
+        if (o > rv[tid] || (o == rv[tid] && oi < ri[tid])) { rv[tid] = o; ri[tid] = oi; }
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      s_p = ri[0];
+      s_stop = !(rv[0] > thr) || s_p >= n;
+      if (!s_stop) chosen[s_p] = 1;
+    }
+    __syncthreads();
+    if (s_stop) break;
+    const int p = s_p;
+    const double piv = sqrt(d[p]);
+    for (int j = tid; j < k; j += blockDim.x) Lp[j] = L[(long long)j * n + p];
+    __syncthreads();
+    for (int i = tid; i < n; i += blockDim.x) {
+      double2 v;
+      if (i == p) {
+        v = make_double2(piv, 0.0);
+      } else if (chosen[i]) {
+        v = make_double2(0.0, 0.0);
+      } else {
+        double2 acc = A[(long long)i * lda + p];
+        for (int j = 0; j < k; ++j) {   // acc -= L[i][j] conj(L[p][j]); column j is contiguous
+          const double2 x = L[(long long)j * n + i], y = Lp[j];
+          acc.x -= x.x * y.x + x.y * y.y;
+          acc.y -= x.y * y.x - x.x * y.y;
+        }
+        v = make_double2(acc.x / piv, acc.y / piv);
+        d[i] -= v.x * v.x + v.y * v.y;
+      }
+      L[(long long)k * n + i] = v;
+    }
+    __syncthreads();
+  }
+  for (long long e = (long long)k * n + tid; e < (long long)n * n; e += blockDim.x) L[e] = make_double2(0.0, 0.0);
+  if (tid == 0) rank[b] = k;
+}
+
+// w[b][j] <- w_M[b][j] for j < r, 0 beyond
+__global__ void pad_eigs_kernel(const double* wm, int r, double* w, long long ldw, int n) {
+  const int b = blockIdx.x;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) w[(long long)b * ldw + j] = j < r ? wm[(long long)b * r + j] : 0.0;
+}
+
+}  // namespace
+
+size_t eigh_gram_workspace_bytes(int n, int batch) {
+  const size_t mat = align256((size_t)n * n * sizeof(double2) * batch);
+  return 4 * mat + align256((size_t)n * sizeof(double) * batch) + align256(sizeof(int) * batch) +
+         eigh_workspace_bytes(n, batch) + 256;
+}
+
+hipError_t eigh_gram(int n, int batch, void* A, long long lda, long long bsA, double* w, long long ldw, void* V,
+                     long long ldv, long long bsV, void* ws, hipStream_t st, int* sweeps_out) {
+  if (n <= 0 || batch <= 0) return hipSuccess;
+  if (n <= NS || n > PC_MAX) return eigh(n, batch, A, lda, bsA, w, ldw, V, ldv, bsV, ws, st, sweeps_out);
+  hipError_t err;
+  char* p = (char*)ws;
+  const size_t mat = align256((size_t)n * n * sizeof(double2) * batch);
+  double2* L = (double2*)p; p += mat;
+  double2* M = (double2*)p; p += mat;
+  double2* U = (double2*)p; p += mat;
+  double2* Y = (double2*)p; p += mat;
+  double* wm = (double*)p; p += align256((size_t)n * sizeof(double) * batch);
+  int* rank = (int*)p; p += align256(sizeof(int) * batch);
+  void* ws_inner = p;
+  const int N = ((n + 63) / 64) * 64;
+  const double tol = ABS_TOL * (double)(N / NS);
+  {
+    prof::Scope ps(prof::EIG_SMALL, st, 0.0, 16.0 * n * n * batch);
+    hipLaunchKernelGGL(pchol_kernel, dim3(batch), dim3(256), 0, st, (const double2*)A, n, lda, bsA, tol, L, rank);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+  }
+  std::vector<int> hr(batch);
+  if ((err = hipMemcpyAsync(hr.data(), rank, sizeof(int) * batch, hipMemcpyDeviceToHost, st)) != hipSuccess) return err;
+  if ((err = hipStreamSynchronize(st)) != hipSuccess) return err;
+  int r = 0;
+  for (int b = 0; b < batch; ++b) r = hr[b] > r ? hr[b] : r;
+  if (r == 0) r = 1;   // all-zero input: one zero column, zero eigenvalue
+  // M = L[:, :r]^H L[:, :r]  (r x r); columns of L past a matrix's own rank are zero
+  {
+    GemmArgs g{};
+    g.M = r; g.N = r; g.K = n; g.batch = batch;
+    g.A = L; g.rsA = n; g.csA = 1; g.bsA = (long long)n * n; g.conjA = 1;
+    g.B = L; g.rsB = 1; g.csB = n; g.bsB = (long long)n * n;
+    g.C = M; g.rsC = r; g.csC = 1; g.bsC = (long long)r * r;
+    g.alpha_re = 1;
+    if ((err = gemm(C128_F64ACC, g, st)) != hipSuccess) return err;
+  }
+  int sw = 0;
+  if ((err = eigh(r, batch, M, r, (long long)r * r, wm, r, U, r, (long long)r * r, ws_inner, st, &sw)) != hipSuccess)
+    return err;
+  if (sweeps_out) *sweeps_out = sw;
+  // Y = L[:, :r] U  (n x r), then V = Y Lambda^{-1/2} (zero past r and below the floor)
+  {
+    GemmArgs g{};
+    g.M = n; g.N = r; g.K = r; g.batch = batch;
+    g.A = L; g.rsA = 1; g.csA = n; g.bsA = (long long)n * n;
+    g.B = U; g.rsB = r; g.csB = 1; g.bsB = (long long)r * r;
+    g.C = Y; g.rsC = r; g.csC = 1; g.bsC = (long long)n * r;
+    g.alpha_re = 1;
+    if ((err = gemm(C128_F64ACC, g, st)) != hipSuccess) return err;
+  }
+  if ((err = factor64_b(Y, r, (long long)n * r, wm, r, n, r, n, -0.5, tol, 0, (double2*)V, ldv, bsV, batch, st)) !=
+      hipSuccess)
+    return err;
+  hipLaunchKernelGGL(pad_eigs_kernel, dim3(batch), dim3(256), 0, st, wm, r, w, ldw, n);
   return hipGetLastError();
 }
 

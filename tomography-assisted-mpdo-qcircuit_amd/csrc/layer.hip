@@ -26,6 +26,7 @@
 #include "site_ops.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -279,6 +280,16 @@ hipError_t gram_rows_b_(int line, const float2* X, int n, int cols, long long bs
 #define gram_cols_b(...) gram_cols_b_(__LINE__, __VA_ARGS__)
 #define gram_rows_b(...) gram_rows_b_(__LINE__, __VA_ARGS__)
 
+// The pivoted-Cholesky-preconditioned solver (eigh_gram) for the layer's Gram matrices is
+// selected with MPDO_LAYER_GRAM_EIG=1; the plain Jacobi solver otherwise.
+bool use_gram_eig() {
+  static const bool on = [] {
+    const char* v = getenv("MPDO_LAYER_GRAM_EIG");
+    return v && v[0] == '1';
+  }();
+  return on;
+}
+
 struct EigB {
   double2* V;
   double* w;
@@ -287,7 +298,7 @@ struct EigB {
 void carve_eig(Carver& c, int n, int batch, EigB& e) {
   e.V = c.get<double2>((size_t)n * n * batch);
   e.w = c.get<double>((size_t)n * batch);
-  e.ws = c.get<char>(eigh_workspace_bytes(n, batch));
+  e.ws = c.get<char>(eigh_gram_workspace_bytes(n, batch));
 }
 mpdo_status eig_b_(int line, int n, int batch, double2* G, EigB& e, hipStream_t s) {
   int sweeps = 0;
@@ -295,11 +306,14 @@ mpdo_status eig_b_(int line, int n, int batch, double2* G, EigB& e, hipStream_t 
   if (!fits(G, batch * nn, sizeof(double2), "eigensolver input", line) ||
       !fits(e.V, batch * nn, sizeof(double2), "eigenvectors", line) ||
       !fits(e.w, (long long)batch * n, sizeof(double), "eigenvalues", line) ||
-      !fits(e.ws, (long long)eigh_workspace_bytes(n, batch), 1, "eigensolver workspace", line)) {
+      !fits(e.ws, (long long)eigh_gram_workspace_bytes(n, batch), 1, "eigensolver workspace", line)) {
     return layer_fail(MPDO_EINVAL, t_oob.c_str());
   }
   if (t_dry) return MPDO_OK;
-  LCK(eigh(n, batch, G, n, (long long)n * n, e.w, n, e.V, n, (long long)n * n, e.ws, s, &sweeps));
+  if (use_gram_eig())
+    LCK(eigh_gram(n, batch, G, n, (long long)n * n, e.w, n, e.V, n, (long long)n * n, e.ws, s, &sweeps));
+  else
+    LCK(eigh(n, batch, G, n, (long long)n * n, e.w, n, e.V, n, (long long)n * n, e.ws, s, &sweeps));
   if (sweeps > 60) {
     char b[160];
     snprintf(b, sizeof b, "Jacobi eigensolver did not converge (layer.hip:%d, order %d, batch %d)", line, n, batch);

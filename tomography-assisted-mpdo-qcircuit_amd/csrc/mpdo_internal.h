@@ -41,4 +41,13 @@ hipError_t eigh(int n, int batch, void* A, long long lda, long long bsA,
                 double* w, long long ldw, void* V, long long ldv, long long bsV,
                 void* ws, hipStream_t s, int* sweeps_out);
 
+// The same for positive semidefinite Gram matrices, preconditioned by pivoted Cholesky and
+// solved at their numerical rank: eigenpairs below the Jacobi floor (and past the rank) come
+// back as zero eigenvalues with zero vectors. Synchronises `s` once (the rank). `ws` must hold
+// eigh_gram_workspace_bytes(n, batch) bytes.
+size_t eigh_gram_workspace_bytes(int n, int batch);
+hipError_t eigh_gram(int n, int batch, void* A, long long lda, long long bsA,
+                     double* w, long long ldw, void* V, long long ldv, long long bsV,
+                     void* ws, hipStream_t s, int* sweeps_out);
+
 }  // namespace mpdo

@@ -30,7 +30,7 @@ std::vector<int> g_eig;   // (n, batch, sweeps) triples
 // fold finished records into the totals (waits for the last one) and recycle their events
 void drain_locked() {
   if (g_recs.empty()) return;
-  hipEventSynchronize(g_recs.back().e1);
+  (void)hipEventSynchronize(g_recs.back().e1);   // profiling only: a failed wait shows as a missing time
   for (const Rec& r : g_recs) {
     float ms = 0;
     if (hipEventElapsedTime(&ms, r.e0, r.e1) == hipSuccess) {
@@ -53,7 +53,7 @@ hipEvent_t take_locked() {
     return e;
   }
   hipEvent_t e = nullptr;
-  hipEventCreate(&e);
+  (void)hipEventCreate(&e);
   return e;
 }
 
@@ -74,13 +74,13 @@ Scope::Scope(int cls, hipStream_t s, double flops, double bytes) : cls_(cls), s_
   std::lock_guard<std::mutex> lk(g_mu);
   e0_ = take_locked();
   e1_ = take_locked();
-  hipEventRecord(e0_, s_);
+  (void)hipEventRecord(e0_, s_);
 }
 
 Scope::~Scope() {
   if (!e0_) return;
   std::lock_guard<std::mutex> lk(g_mu);
-  hipEventRecord(e1_, s_);
+  (void)hipEventRecord(e1_, s_);
   g_recs.push_back(Rec{cls_, e0_, e1_, flops_, bytes_});
   if (g_recs.size() >= 4096) drain_locked();
 }

@@ -10,6 +10,7 @@ import ctypes
 import os
 from typing import List, Optional, Sequence, Tuple
 
+import numpy as np
 import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -53,6 +54,9 @@ def load() -> ctypes.CDLL:
         'mpdo_reduced_dm': (I, [I, SP, P(I), VP, VP]),
         'mpdo_trace_rho_rho': (I, [I, SP, SP, P(D), VP]),
         'mpdo_contract_state': (I, [I, SP, VP, VP]),
+        'mpdo_env_right_elems': (ctypes.c_longlong, [I, SP]),
+        'mpdo_env_right': (I, [I, SP, VP, VP]),
+        'mpdo_sample_step': (I, [SP, I, VP, I, P(I), I, VP, VP, P(D), VP]),
         'mpdo_layer_dims': (I, [I, SP, I, I, I, I, I, SP]),
         'mpdo_layer_batched': (I, [I, I, SP, P(ctypes.c_float), I, I, P(ctypes.c_float), I, I, I, SP, P(D), VP]),
         'mpdo_debug_gemm': (I, [I, I, I, I, I, VP, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, I,
@@ -60,6 +64,7 @@ def load() -> ctypes.CDLL:
                                 VP, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong,
                                 D, D, D, D, VP]),
         'mpdo_debug_eigh': (I, [I, I, VP, P(D), VP, P(I), VP]),
+        'mpdo_debug_eigh_gram': (I, [I, I, VP, P(D), VP, P(I), VP]),
         'mpdo_debug_layer_dry_run': (I, [I, I, SP, I, I, I, I, I, SP, P(ctypes.c_ulonglong)]),
         'mpdo_debug_prof_enable': (I, [I]),
         'mpdo_debug_layer_kept': (I, [P(I), I]),
@@ -280,6 +285,54 @@ def trace_rho_rho(a: Sequence[torch.Tensor], b: Sequence[torch.Tensor]) -> float
     return out.value
 
 
+def env_right(sites: Sequence[torch.Tensor]) -> List[torch.Tensor]:
+    """Traced tail environments R_0..R_n (complex128, R_i of shape (l_i, l_i), R_n = [[1]])."""
+    lib = load()
+    n = len(sites)
+    arr = _sites(sites)
+    dev = sites[0].device
+    total = lib.mpdo_env_right_elems(n, arr)
+    buf = torch.empty(total, dtype=torch.complex128, device=dev)
+    check(lib.mpdo_env_right(n, arr, ctypes.c_void_p(buf.data_ptr()), _stream(dev)), 'mpdo_env_right')
+    out, off = [], 0
+    for t in sites:
+        m = t.shape[0]
+        out.append(buf[off:off + m * m].view(m, m))
+        off += m * m
+    out.append(buf[off:off + 1].view(1, 1))
+    return out
+
+
+def sample_step(site: torch.Tensor, prev: torch.Tensor, sel: Optional[Sequence[int]], sampled: bool,
+                R_next: Optional[torch.Tensor] = None):
+    """One site of the conditional sweep. prev: (m, l, l) complex128 environments; sel picks one
+    per group (None: group g = prev[g]). Returns (out, probs): out (G, 2, r, r) and probs (G, 2)
+    numpy float64 when sampled, else out (G, r, r) and None."""
+    lib = load()
+    if prev.dtype != torch.complex128 or prev.dim() != 3 or not prev.is_contiguous():
+        raise ValueError('environments must be a contiguous (m, l, l) complex128 tensor')
+    l, r = site.shape[0], site.shape[3]
+    if prev.shape[1] != l or prev.shape[2] != l:
+        raise ValueError(f'environment shape {tuple(prev.shape[1:])} does not match the site bond {l}')
+    G = prev.shape[0] if sel is None else len(sel)
+    dev = site.device
+    if sampled:
+        if R_next is None or tuple(R_next.shape) != (r, r) or R_next.dtype != torch.complex128:
+            raise ValueError('a sampled site needs its (r, r) complex128 tail environment')
+        out = torch.empty(G, 2, r, r, dtype=torch.complex128, device=dev)
+    else:
+        out = torch.empty(G, r, r, dtype=torch.complex128, device=dev)
+    probs = np.zeros((G, 2), dtype=np.float64)
+    csel = None if sel is None else (ctypes.c_int * G)(*[int(x) for x in sel])
+    check(lib.mpdo_sample_step(ctypes.byref(site_desc(site)), G, ctypes.c_void_p(prev.data_ptr()), prev.shape[0],
+                               csel, 1 if sampled else 0,
+                               ctypes.c_void_p(R_next.contiguous().data_ptr()) if sampled else None,
+                               ctypes.c_void_p(out.data_ptr()),
+                               probs.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if sampled else None,
+                               _stream(dev)), 'mpdo_sample_step')
+    return out, (probs if sampled else None)
+
+
 def layer_dims(dims: Sequence[Tuple[int, int, int]], K1: int, parity: int, K2: int, chi: Optional[int],
                kappa: Optional[int]):
     """Output (l, a, r) of mpdo_layer_batched, or None with the reason when the layer does not
@@ -301,7 +354,6 @@ def layer_batched(sites: Sequence[torch.Tensor], kraus1: torch.Tensor, parity: i
     """One fused brickwork layer. sites[i]: (B, l, 2, a, r) complex64 on the device;
     kraus1: (B, n, K1, 2, 2); gates2: (npairs, 2, 2, 2, 2, K2). Returns (new sites, discarded
     (B, 3n) numpy: chi steps by bond, kappa cuts by site, gate-time splits by pair min site)."""
-    import numpy as np
     lib = load()
     n = len(sites)
     B = sites[0].shape[0]
@@ -354,14 +406,16 @@ def debug_gemm(prec: int, A: torch.Tensor, B: torch.Tensor, C: torch.Tensor, M: 
     check(st, 'mpdo_debug_gemm')
 
 
-def debug_eigh(A: torch.Tensor):
-    """A: (batch, n, n) complex128 Hermitian on the device (destroyed). Returns (w, V, sweeps)."""
+def debug_eigh(A: torch.Tensor, gram: bool = False):
+    """A: (batch, n, n) complex128 Hermitian on the device (destroyed). Returns (w, V, sweeps).
+    gram=True: the positive-semidefinite solver of the fused layer (zero vectors past the rank)."""
     lib = load()
     b, n, _ = A.shape
     w = torch.empty(b, n, dtype=torch.float64, device=A.device)
     V = torch.empty(b, n, n, dtype=torch.complex128, device=A.device)
     sw = ctypes.c_int(0)
-    check(lib.mpdo_debug_eigh(n, b, ctypes.c_void_p(A.data_ptr()),
+    fn = lib.mpdo_debug_eigh_gram if gram else lib.mpdo_debug_eigh
+    check(fn(n, b, ctypes.c_void_p(A.data_ptr()),
                               ctypes.cast(w.data_ptr(), ctypes.POINTER(ctypes.c_double)),
                               ctypes.c_void_p(V.data_ptr()), ctypes.byref(sw), _stream(A.device)),
           'mpdo_debug_eigh')

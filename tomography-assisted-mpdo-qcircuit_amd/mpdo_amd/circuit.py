@@ -31,7 +31,8 @@ from .dmOperations import DMNodes
 from .node import SiteNode, replicate_nodes
 from .noise import NoiseChannel, choi_compress, compose_kraus
 from .realnoise import cpExp_channel, czExp_channel
-from .Tools import select_device
+from . import sampling as _sampling
+from .Tools import count_item, select_device
 
 __all__ = ['QuantumCircuit', 'TensorCircuit', 'GLOBAL_MINIMUM', 'MAX_ATTEMPTS']
 
@@ -554,3 +555,75 @@ class TensorCircuit(QuantumCircuit):
         st = self._state_or_raise()
         self._vector = _lib.contract_state([s.tensor.contiguous() for s in st]).reshape(2 ** self.qnumber, 1)
         return self._vector
+
+    # -- shot sampling (Circuit.py:297-467) -------------------------------------------------------
+    _projectors_string = ['X', 'Y', 'Z']     # AbstractCircuit.py:80
+
+    def _conditional_prb(self, _history: List[Union[int, bool]]) -> float:
+        """Circuit.py:297-332: probability that qubit ``_indices4samples[len(_history)]`` reads 1
+        given the outcomes ``_history`` of the qubits before it (later qubits traced)."""
+        if self._nodes4samples is None:
+            raise ValueError('call sample() first')
+        return _sampling.normalise_prb(*self._nodes4samples.probs_of(_history))
+
+    def _conditional_sample(self, shots: int, _sampleLength: int, _tqdm_disable: bool = False) -> List[List[int]]:
+        """Circuit.py:334-342."""
+        return _sampling.sequential_sample(self._nodes4samples, shots)
+
+    def _conditional_batch_sample(self, shots: int, _sampleLength: int, _bool: bool = False,
+                                  _tqdm_disable: bool = False) -> List[List[int]]:
+        """Circuit.py:344-391."""
+        return _sampling.batch_sample(self._nodes4samples, shots, bool_result=_bool)
+
+    def sample(self, shots: Optional[int] = None, orientation: Optional[List[int]] = None,
+               reduced: Optional[List[int]] = None, sample_string: bool = True, _tqdm_disable: bool = False,
+               _require_sequential_sample: bool = False, _require_bool_result: bool = False,
+               _require_counts: bool = True, _stateNodes4Sample: Optional[List[SiteNode]] = None):
+        """Circuit.py:393-445. ``orientation[i]`` 0/1/2 = X/Y/Z basis; as in the reference the
+        MeasureX/MeasureY rotation of entry i is applied to qubit i (the position in the list,
+        which is the i-th sampled qubit only when no earlier qubit is reduced)."""
+        shots = shots or 1024
+        reduced = reduced or []
+        _ori_list = [num for num in range(self.qnumber) if num not in reduced]
+        _sampleLength = len(_ori_list)
+        orientation = orientation or [2] * _sampleLength
+        if len(orientation) != _sampleLength:
+            raise ValueError('Length of orientation must match the sample length. Check reduced or unmeasured qubits.')
+        src = self._state_or_raise() if _stateNodes4Sample is None else _stateNodes4Sample
+        tensors = [s.tensor.contiguous() for s in src]
+        for _ori_val, cls in [(0, G.MeasureX), (1, G.MeasureY)]:
+            _indices = [i for i, value in enumerate(orientation) if value == _ori_val]
+            if _indices:
+                if max(_indices) >= self.qnumber:
+                    raise ValueError(f'Qubit index out of range, max index is Q{max(_indices)}.')
+                M = cls(dtype=self.dtype, device=self.device).tensor.to(torch.complex128)
+                outs = _lib.apply_1q([tensors[i] for i in _indices], [M] * len(_indices))
+                for i, t in zip(_indices, outs):
+                    tensors[i] = t
+        self._nodes4samples = _sampling.ConditionalSweep(tensors, _ori_list)
+        self._indices4samples = _ori_list
+        if not _tqdm_disable:
+            print('Sample Direction:'
+                  f'\n (scheme-{"".join([self._projectors_string[num] for num in orientation])})')
+        if _require_sequential_sample:
+            _bitStrings = self._conditional_sample(shots, _sampleLength, _tqdm_disable)
+        else:
+            _bitStrings = self._conditional_batch_sample(shots, _sampleLength, _require_bool_result, _tqdm_disable)
+        if sample_string and not _require_bool_result:
+            _bitStrings = [''.join(map(str, _stringList)) for _stringList in _bitStrings]
+        self._samples = _bitStrings
+        if _require_counts:
+            self._counts = count_item(_bitStrings)
+            return self._samples, self._counts
+        return self._samples
+
+    def randomSample(self, measurement_schemes: List[List[int]], shots_per_scheme: int = 1024,
+                     reduced: Optional[List[int]] = None, _tqdm_disable: bool = False,
+                     _require_sequential_sample: bool = False, _require_bool_result: bool = False,
+                     _stateNodes4Sample: Optional[List[SiteNode]] = None) -> List[List[List[Union[int, bool]]]]:
+        """Circuit.py:447-467: one sample() per measurement scheme, bit lists, no counts."""
+        return [self.sample(shots=shots_per_scheme, orientation=scheme, reduced=reduced, sample_string=False,
+                            _tqdm_disable=True, _require_sequential_sample=_require_sequential_sample,
+                            _require_bool_result=_require_bool_result, _require_counts=False,
+                            _stateNodes4Sample=_stateNodes4Sample)
+                for scheme in measurement_schemes]

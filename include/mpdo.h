@@ -130,10 +130,15 @@ mpdo_status mpdo_pauli_expect(int n, const mpdo_site* sites, int nstrings, const
 mpdo_status mpdo_reduced_dm(int n, const mpdo_site* sites, const int* keep, void* out, void* stream);
 
 /*
- * Re Tr(rho0 rho1) of two MPDOs over the same qubits. Replaces dmOperations.py:51-66
- * (trace_rho_rho). out: host double. Synchronises once.
+ * Re Tr(rho0 rho1) of two MPDOs over the same qubits, by a transfer sweep over the four open
+ * bonds (any n; bonds up to chi0 * chi1 ~ 2^15). Replaces dmOperations.py:51-66 (trace_rho_rho).
+ * mpdo_trace_rho_rho_masked: qubits with keep[i] == 0 are traced within each state first (both
+ * states reduced on the same qubits, reduce_dmNodes, dmOperations.py:18-37); keep NULL = all kept.
+ * out: host double. Synchronises once.
  */
 mpdo_status mpdo_trace_rho_rho(int n, const mpdo_site* a, const mpdo_site* b, double* out, void* stream);
+mpdo_status mpdo_trace_rho_rho_masked(int n, const mpdo_site* a, const mpdo_site* b, const int* keep, double* out,
+                                      void* stream);
 
 /*
  * One brickwork circuit-layer for `batch` chains with identical shapes: the single-qubit

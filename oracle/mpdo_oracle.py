@@ -762,14 +762,26 @@ def trace_rho(st: MPDO) -> float:
     return float(np.real(env[0, 0]))
 
 
-def trace_rho_rho(st0: MPDO, st1: Optional[MPDO] = None) -> float:
-    """dmOperations.py:51-66: Re Tr(rho0 rho1)."""
+def trace_rho_rho(st0: MPDO, st1: Optional[MPDO] = None, reduced: Sequence[int] = ()) -> float:
+    """dmOperations.py:51-66: Re Tr(rho0 rho1); the qubits in ``reduced`` are traced within each
+    state first (their physical edges are no longer dangling, so the reference leaves them)."""
     st1 = st0 if st1 is None else st1
+    red = set(reduced)
     env = np.ones((1, 1, 1, 1), dtype=np.complex128)
-    for A, B in zip(st0.sites, st1.sites):
-        # rho0[s, s'] rho1[s', s]: ket0 s with bra1 s, bra0 s' with ket1 s'
-        env = np.einsum('abcd,asxe,btxf,ctyg,dsyh->efgh', env, A, A.conj(), B, B.conj(), optimize=True)
+    for i, (A, B) in enumerate(zip(st0.sites, st1.sites)):
+        if i in red:
+            env = np.einsum('abcd,asxe,bsxf,ctyg,dtyh->efgh', env, A, A.conj(), B, B.conj(), optimize=True)
+        else:
+            # rho0[s, s'] rho1[s', s]: ket0 s with bra1 s, bra0 s' with ket1 s'
+            env = np.einsum('abcd,asxe,btxf,ctyg,dsyh->efgh', env, A, A.conj(), B, B.conj(), optimize=True)
     return float(np.real(env[0, 0, 0, 0]))
+
+
+def trace_composited_rho2(states: Sequence[MPDO], reduced: Sequence[int] = ()) -> float:
+    """dmOperations.py:91-132: Tr(rho^2) of rho = sum_i rho_i / S (diagonal and cross terms)."""
+    S = len(states)
+    tot = sum(trace_rho_rho(states[i], states[j], reduced) for i in range(S) for j in range(S))
+    return tot / S ** 2
 
 
 # ----------------------------------------------------------------------------------------------

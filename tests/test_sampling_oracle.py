@@ -124,3 +124,18 @@ def test_conditional_prb_normalises_in_float32_with_the_floor():
     assert O.normalise_prb([0.25, 0.75]) == np.float32(0.75)
     with pytest.raises(RuntimeError):
         O.normalise_prb([0.5, -1e-3])
+
+
+@pytest.mark.parametrize('reduced', [[], [1], [0, 3], [1, 2, 3]])
+def test_trace_rho_rho_with_reduced_qubits_is_the_dense_trace(reduced):
+    """dmOperations.py:51-66 on reduced states: Tr(rho0_k rho1_k) of the dense partial traces;
+    trace_composited_rho2 as Tr(((rho0 + rho1) / 2)^2)."""
+    n = 4
+    st0, r0 = _state(n, 21)
+    st1, r1 = _state(n, 22)
+    keep = [q for q in range(n) if q not in reduced]
+    d0, d1 = D.partial_trace(r0, keep, n), D.partial_trace(r1, keep, n)
+    assert np.isclose(O.trace_rho_rho(st0, st1, reduced), np.real(np.trace(d0 @ d1)), rtol=1e-12)
+    assert np.isclose(O.trace_rho_rho(st1, st1, reduced), np.real(np.trace(d1 @ d1)), rtol=1e-12)
+    m = 0.5 * (d0 + d1)
+    assert np.isclose(O.trace_composited_rho2([st0, st1], reduced), np.real(np.trace(m @ m)), rtol=1e-12)

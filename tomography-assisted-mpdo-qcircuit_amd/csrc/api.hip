@@ -884,32 +884,7 @@ mpdo_status mpdo_reduced_dm(int n, const mpdo_site* sites, const int* keep, void
 }
 
 mpdo_status mpdo_trace_rho_rho(int n, const mpdo_site* a, const mpdo_site* b, double* out, void* stream) {
-  // Tr(rho0 rho1) from the two full density matrices (dense contraction, n <= 12), the same
-  // network dmOperations.py:51-66 hands to contractors.auto.
-  if (n <= 0) return MPDO_OK;
-  if (!a || !b || !out) return fail(MPDO_EINVAL, "null argument");
-  if (n > 12) return fail(MPDO_EINVAL, "trace_rho_rho supports n <= 12 qubits");
-  hipStream_t s = S(stream);
-  const size_t dim = (size_t)1 << n;
-  // the two reduced_dm calls use the workspace themselves: keep the matrices in a separate
-  // allocation owned by this call
-  float2* r = nullptr;
-  CK(hipMallocAsync((void**)&r, 2 * dim * dim * sizeof(float2) + 256, s));
-  std::vector<int> keep(n, 1);
-  mpdo_status st = mpdo_reduced_dm(n, a, keep.data(), r, stream);
-  if (st == MPDO_OK) st = mpdo_reduced_dm(n, b, keep.data(), r + dim * dim, stream);
-  if (st == MPDO_OK) {
-    double2* res = (double2*)(r + 2 * dim * dim);
-    if (trace_prod(r, r + dim * dim, (int)dim, res, s) != hipSuccess) st = fail(MPDO_EHIP, "trace_prod");
-    double2 h = make_double2(0, 0);
-    if (st == MPDO_OK && hipMemcpyAsync(&h, res, sizeof(double2), hipMemcpyDeviceToHost, s) != hipSuccess)
-      st = fail(MPDO_EHIP, "copy");
-    (void)hipStreamSynchronize(s);
-    *out = h.x;
-  }
-  (void)hipFreeAsync(r, s);
-  (void)hipStreamSynchronize(s);
-  return st;
+  return mpdo_trace_rho_rho_masked(n, a, b, nullptr, out, stream);
 }
 
 mpdo_status mpdo_contract_state(int n, const mpdo_site* sites, void* out, void* stream) {

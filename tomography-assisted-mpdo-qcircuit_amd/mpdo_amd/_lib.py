@@ -53,6 +53,7 @@ def load() -> ctypes.CDLL:
         'mpdo_pauli_expect': (I, [I, SP, I, P(I), P(D), VP]),
         'mpdo_reduced_dm': (I, [I, SP, P(I), VP, VP]),
         'mpdo_trace_rho_rho': (I, [I, SP, SP, P(D), VP]),
+        'mpdo_trace_rho_rho_masked': (I, [I, SP, SP, P(I), P(D), VP]),
         'mpdo_contract_state': (I, [I, SP, VP, VP]),
         'mpdo_env_right_elems': (ctypes.c_longlong, [I, SP]),
         'mpdo_env_right': (I, [I, SP, VP, VP]),
@@ -277,11 +278,19 @@ def reduced_dm(sites: Sequence[torch.Tensor], keep: Sequence[bool]) -> torch.Ten
     return out
 
 
-def trace_rho_rho(a: Sequence[torch.Tensor], b: Sequence[torch.Tensor]) -> float:
+def trace_rho_rho(a: Sequence[torch.Tensor], b: Sequence[torch.Tensor], keep: Optional[Sequence[bool]] = None) -> float:
+    """Re Tr(rho0 rho1); qubits with keep[i] False are traced within each state first."""
     lib = load()
+    if len(a) != len(b):
+        raise ValueError('Density matrices must have the same number of nodes.')
     out = ctypes.c_double(0.0)
-    check(lib.mpdo_trace_rho_rho(len(a), _sites(a), _sites(b), ctypes.byref(out), _stream(a[0].device)),
-          'mpdo_trace_rho_rho')
+    if keep is None:
+        check(lib.mpdo_trace_rho_rho(len(a), _sites(a), _sites(b), ctypes.byref(out), _stream(a[0].device)),
+              'mpdo_trace_rho_rho')
+    else:
+        kf = (ctypes.c_int * len(a))(*[1 if k else 0 for k in keep])
+        check(lib.mpdo_trace_rho_rho_masked(len(a), _sites(a), _sites(b), kf, ctypes.byref(out),
+                                            _stream(a[0].device)), 'mpdo_trace_rho_rho_masked')
     return out.value
 
 

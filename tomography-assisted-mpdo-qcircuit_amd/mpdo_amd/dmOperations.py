@@ -15,8 +15,8 @@ import torch
 from . import _lib
 from .node import SiteNode
 
-__all__ = ['PAULI_DICT', 'DMNodes', 'reduce_dmNodes', 'trace_rho', 'trace_rho_rho', 'trace_rho2', 'expect',
-           'pauli_expect']
+__all__ = ['PAULI_DICT', 'DMNodes', 'reduce_dmNodes', 'trace_rho', 'trace_rho_rho', 'trace_rho2',
+           'trace_composited_rho', 'trace_composited_rho2', 'expect', 'pauli_expect']
 
 PAULI_DICT = {
     0: torch.tensor([[0, 1], [1, 0]]),
@@ -89,25 +89,44 @@ def _dense_rho(dm: DMNodes, keep: Sequence[bool]) -> torch.Tensor:
 
 
 def trace_rho_rho(dmNodes_0: DMNodes, dmNodes_1: Optional[DMNodes] = None) -> torch.Tensor:
-    """dmOperations.py:51-66: Re Tr(rho0 rho1) over the qubits neither state traces."""
+    """dmOperations.py:51-66: Re Tr(rho0 rho1) over the qubits neither state traces (a transfer
+    sweep on the device, any number of qubits). Both states must trace the same qubits: the
+    reference leaves the other state's physical edges open there and returns no scalar."""
     dm1 = dmNodes_0 if dmNodes_1 is None else dmNodes_1
     n = dmNodes_0.qnumber
     if dm1.qnumber != n:
         raise ValueError('Density matrices must have the same number of nodes.')
     if dmNodes_0.reduced != dm1.reduced:
         raise ValueError('both density matrices must trace the same qubits')
-    if not dmNodes_0.reduced:
-        return torch.tensor(_lib.trace_rho_rho(dmNodes_0.tensors(), dm1.tensors()), dtype=torch.float64)
-    keep = [i not in dmNodes_0.reduced for i in range(n)]
-    r0 = _dense_rho(dmNodes_0, keep)
-    r1 = r0 if dmNodes_1 is None else _dense_rho(dm1, keep)
-    r0, r1 = r0.cpu().to(torch.complex128), r1.cpu().to(torch.complex128)
-    return torch.trace(r0 @ r1).real
+    keep = None if not dmNodes_0.reduced else [i not in dmNodes_0.reduced for i in range(n)]
+    return torch.tensor(_lib.trace_rho_rho(dmNodes_0.tensors(), dm1.tensors(), keep), dtype=torch.float64)
 
 
 def trace_rho2(dmNodes: DMNodes) -> torch.Tensor:
     """dmOperations.py:69-70 (purity Tr rho~^2)."""
     return trace_rho_rho(dmNodes)
+
+
+def trace_composited_rho(*dmNodes: DMNodes) -> torch.Tensor:
+    """dmOperations.py:73-88: the sum of trace_rho_rho(rho_i) over the slices (the reference sums
+    the purities of the slices, whatever its docstring says)."""
+    return sum((trace_rho_rho(d) for d in dmNodes), torch.tensor(0.0, dtype=torch.float64))
+
+
+def trace_composited_rho2(*dmNodes: DMNodes) -> torch.Tensor:
+    """dmOperations.py:91-132: Tr(rho^2) of rho = (rho_0 + ... + rho_{S-1}) / S as
+    (sum_i Tr rho_i^2 + 2 sum_{i<j} Tr rho_i rho_j) / S^2. The reference takes the same sum from a
+    dense 2^k matrix when at most 12 qubits stay untraced; the value is the same."""
+    S = len(dmNodes)
+    if S == 0:
+        raise ValueError('no density matrices given')
+    kept = {d.qnumber - len(d.reduced) for d in dmNodes}
+    if len(kept) != 1:
+        raise ValueError('Density matrices should have same number of un-traced nodes.')
+    diag = sum((trace_rho2(d) for d in dmNodes), torch.tensor(0.0, dtype=torch.float64))
+    cross = sum((trace_rho_rho(dmNodes[i], dmNodes[j]) for i in range(S) for j in range(i + 1, S)),
+                torch.tensor(0.0, dtype=torch.float64))
+    return (diag + 2 * cross) / (S ** 2)
 
 
 def expect(dmNodes: DMNodes, observables: Union[torch.Tensor, List[torch.Tensor]],

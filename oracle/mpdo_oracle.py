@@ -767,13 +767,19 @@ def trace_rho_rho(st0: MPDO, st1: Optional[MPDO] = None, reduced: Sequence[int] 
     state first (their physical edges are no longer dangling, so the reference leaves them)."""
     st1 = st0 if st1 is None else st1
     red = set(reduced)
-    env = np.ones((1, 1, 1, 1), dtype=np.complex128)
+    env = np.ones((1, 1, 1, 1), dtype=np.complex128)      # [ket0, bra0, ket1, bra1]
+    o = dict(optimize=True)
     for i, (A, B) in enumerate(zip(st0.sites, st1.sites)):
-        if i in red:
-            env = np.einsum('abcd,asxe,bsxf,ctyg,dtyh->efgh', env, A, A.conj(), B, B.conj(), optimize=True)
-        else:
-            # rho0[s, s'] rho1[s', s]: ket0 s with bra1 s, bra0 s' with ket1 s'
-            env = np.einsum('abcd,asxe,btxf,ctyg,dsyh->efgh', env, A, A.conj(), B, B.conj(), optimize=True)
+        if i in red:     # each state's physical index meets its own bra
+            t = np.einsum('abcd,asxe->bcdsxe', env, A, **o)
+            t = np.einsum('bcdsxe,bsxf->cdef', t, A.conj(), **o)
+            t = np.einsum('cdef,ctyg->deftyg', t, B, **o)
+            env = np.einsum('deftyg,dtyh->efgh', t, B.conj(), **o)
+        else:            # rho0[s, s'] rho1[s', s]: ket0 s with bra1 s, bra0 s' with ket1 s'
+            t = np.einsum('abcd,asxe->bcdsxe', env, A, **o)
+            t = np.einsum('bcdsxe,dsyh->bcxeyh', t, B.conj(), **o)
+            t = np.einsum('bcxeyh,btxf->ceyhtf', t, A.conj(), **o)
+            env = np.einsum('ceyhtf,ctyg->efgh', t, B, **o)
     return float(np.real(env[0, 0, 0, 0]))
 
 

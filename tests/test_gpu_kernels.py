@@ -80,3 +80,41 @@ def test_eigh_rank_deficient(cuda, n, rank):
         top = max(ref[0], np.abs(H[b]).max())
         assert np.abs(w[b] - ref).max() <= 1e-12 * top
         assert np.abs(V[b].conj().T @ V[b] - np.eye(n)).max() <= 1e-11
+
+
+@pytest.mark.parametrize('n,rank', [(100, 100), (256, 256), (512, 300), (1024, 450), (200, 40)])
+def test_eigh_gram(cuda, n, rank):
+    """The pivoted-Cholesky-preconditioned solver on graded Gram matrices (12 decades) of full and
+    deficient rank: eigenvalues above its floor as numpy's; eigenvectors with residual within the
+    Jacobi floor on L^H L times sqrt(largest / own eigenvalue), orthonormality within it times the
+    product of two such factors; zero vectors for zero eigenvalues."""
+    from mpdo_amd import _lib
+    rng = np.random.default_rng(n + rank)
+    X = _crand(rng, 3, n, rank) * np.logspace(0, -6, rank)[None, None, :]
+    H = X @ np.swapaxes(X.conj(), 1, 2)
+    H = 0.5 * (H + np.swapaxes(H.conj(), 1, 2))
+    w, V, sweeps = _lib.debug_eigh(torch.from_numpy(H.copy()).to(cuda), gram=True)
+    w, V = w.cpu().numpy(), V.cpu().numpy()
+    assert sweeps < 40
+    floor = 1e-14 * max(1, ((n + 63) // 64 * 64) // 64)
+    for b in range(3):
+        ref = np.sort(np.linalg.eigvalsh(H[b]))[::-1]
+        top = ref[0]
+        big = ref > 100 * floor * top
+        assert np.all(np.diff(w[b]) <= 1e-12 * top)
+        assert np.abs(w[b][big] - ref[big]).max() <= 1e-12 * top
+        assert np.all(np.abs(w[b]) <= ref[0] * (1 + 1e-12))
+        keep = w[b] > 1e-9 * top
+        Vk, wk = V[b][:, keep], w[b][keep]
+        # V = L U Lambda^{-1/2}: the Jacobi residual on L^H L is amplified by sqrt(top / w_j)
+        amp = np.sqrt(top / wk)
+        scale = max(1, ((n + 63) // 64 * 64) // 64)
+        res = np.abs(H[b] @ Vk - Vk * wk[None, :]).max(axis=0)
+        assert np.all(res <= 1e-13 * scale * top * amp), (res / (top * amp)).max()
+        # V_i^H V_j = (U_i^H M U_j) / sqrt(w_i w_j): the floor times amp_i amp_j
+        gram = np.abs(Vk.conj().T @ Vk - np.eye(Vk.shape[1]))
+        assert np.all(gram <= 1e-12 * scale * np.outer(amp, amp)), (gram / np.outer(amp, amp)).max()
+        zero = w[b] == 0
+        assert np.all(V[b][:, zero] == 0)
+        assert np.all(w[b][rank:] <= 1e-12 * top)
+

@@ -565,7 +565,7 @@ hipError_t eigh(int n, int batch, void* A, long long lda, long long bsA, double*
 // =======================================================================================
 // Eigen-decomposition of positive semidefinite Gram matrices, preconditioned by pivoted
 // Cholesky: G = P L L^H P^T stopped at the numerical rank r (largest remaining diagonal at
-// or below the Jacobi floor), the Jacobi method on the r x r matrix M = L^H L (same nonzero
+// or below 1e-14 of the largest diagonal), the Jacobi method on the r x r matrix M = L^H L (same nonzero
 // eigenvalues, strongly diagonally dominant for graded spectra, so it converges in a few sweeps),
 // eigenvectors L U Lambda^{-1/2}. Directions beyond the rank (and eigenvalues below the floor)
 // come back as zero vectors with zero eigenvalues.
@@ -684,8 +684,9 @@ hipError_t eigh_gram(int n, int batch, void* A, long long lda, long long bsA, do
   double* wm = (double*)p; p += align256((size_t)n * sizeof(double) * batch);
   int* rank = (int*)p; p += align256(sizeof(int) * batch);
   void* ws_inner = p;
-  const int N = ((n + 63) / 64) * 64;
-  const double tol = ABS_TOL * (double)(N / NS);
+  // pivots stop at 1e-14 of the largest diagonal: the Gram mass left out is at most
+  // (n - rank) * 1e-14 * max G_ii, the level of the fp64 rounding of the Schur complement
+  const double tol = 1e-14;
   {
     prof::Scope ps(prof::EIG_SMALL, st, 0.0, 16.0 * n * n * batch);
     hipLaunchKernelGGL(pchol_kernel, dim3(batch), dim3(256), 0, st, (const double2*)A, n, lda, bsA, tol, L, rank);

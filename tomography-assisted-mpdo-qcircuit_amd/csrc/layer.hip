@@ -280,12 +280,13 @@ hipError_t gram_rows_b_(int line, const float2* X, int n, int cols, long long bs
 #define gram_cols_b(...) gram_cols_b_(__LINE__, __VA_ARGS__)
 #define gram_rows_b(...) gram_rows_b_(__LINE__, __VA_ARGS__)
 
-// The pivoted-Cholesky-preconditioned solver (eigh_gram) for the layer's Gram matrices is
-// selected with MPDO_LAYER_GRAM_EIG=1; the plain Jacobi solver otherwise.
+// The two cores whose consumers read only the leading eigenvectors and the eigenvalue sums (the
+// gate-time cut and the chi step) use the pivoted-Cholesky-preconditioned solver (eigh_gram);
+// MPDO_LAYER_GRAM_EIG=0 selects the plain Jacobi solver for them.
 bool use_gram_eig() {
   static const bool on = [] {
     const char* v = getenv("MPDO_LAYER_GRAM_EIG");
-    return v && v[0] == '1';
+    return !(v && v[0] == '0');
   }();
   return on;
 }
@@ -300,7 +301,7 @@ void carve_eig(Carver& c, int n, int batch, EigB& e) {
   e.w = c.get<double>((size_t)n * batch);
   e.ws = c.get<char>(eigh_gram_workspace_bytes(n, batch));
 }
-mpdo_status eig_b_(int line, int n, int batch, double2* G, EigB& e, hipStream_t s) {
+mpdo_status eig_b_(int line, int n, int batch, double2* G, EigB& e, hipStream_t s, bool gram = false) {
   int sweeps = 0;
   const long long nn = (long long)n * n;
   if (!fits(G, batch * nn, sizeof(double2), "eigensolver input", line) ||
@@ -310,7 +311,7 @@ mpdo_status eig_b_(int line, int n, int batch, double2* G, EigB& e, hipStream_t 
     return layer_fail(MPDO_EINVAL, t_oob.c_str());
   }
   if (t_dry) return MPDO_OK;
-  if (use_gram_eig())
+  if (gram && use_gram_eig())
     LCK(eigh_gram(n, batch, G, n, (long long)n * n, e.w, n, e.V, n, (long long)n * n, e.ws, s, &sweeps));
   else
     LCK(eigh(n, batch, G, n, (long long)n * n, e.w, n, e.V, n, (long long)n * n, e.ws, s, &sweeps));
@@ -859,7 +860,7 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
       const int nT = 2 * p1t;
       const int len_gate = (int)std::min(2LL * l * a1, 2LL * a2 * K2 * r);
       LCK(gram_rows_b(Cs, nT, (int)(2LL * K2 * ps.p2), szCt, G, B, s));
-      LST(eig_b(nT, B, G, eE, s));
+      LST(eig_b(nT, B, G, eE, s, true));
       LCK(K_KEEP(eE.w, nT, nT, len_gate, -1, GATE_EPS, 0, dk, ddisc + (size_t)B * (nsteps + n + i), B, s));
       // the projector onto the kept directions is applied only when some chain drops one; the
       // core then lives in the mq-dimensional span of the kept directions (mq = max over chains)
@@ -1065,7 +1066,7 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
       g.alpha_re = 1;
       LCK(K_GEMM(C128_F64ACC, g, s));
     }
-    LST(eig_b(nC, B, G, eE, s));
+    LST(eig_b(nC, B, G, eE, s, true));
     LCK(K_KEEP(eE.w, nC, nC, P.step_len[i], kp, -1.0, 1, dk, ddisc + (size_t)B * i, B, s));
     // new min site: (Q1 (x) I) U sqrt(S) -> (kprev, a1, s1, kp) -> (kprev, s1, a1, kp)
     LCK(K_FACTOR(eE.V, nC, (long long)nC * nC, eE.w, nC, nC, nC, kp, 0.25, SPLIT_FLOOR, 0, FL, kp, (long long)nC * kp, B, s));

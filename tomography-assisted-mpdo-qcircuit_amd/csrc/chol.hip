@@ -1,0 +1,318 @@
+// Pivoted Cholesky of batched positive semidefinite Gram matrices (complex128) and the inverse of
+// its pivot-ordered triangular factor.
+//
+// pchol_b: G = L L^H with L stored by columns (L[k n + i] = L_ik in the original row order), the
+// pivot of column k the row with the largest remaining diagonal, stopped once that diagonal is at
+// or below tol * max_i G_ii (the rank). Blocked: a panel kernel produces PC_NB pivot columns
+// (left-looking within the panel, one workgroup per matrix), then one fp64 MFMA GEMM subtracts the
+// panel's outer products from the trailing matrix, so the sequential part reads only the panel.
+//
+// pchol_inverse_b: with Lt[t][u] = L[piv[t]][u] (lower triangular in pivot order, t, u < rank),
+// the inverse Lt^{-1} by 64 x 64 diagonal-block inversions in LDS and block merges
+// inv([A 0; C B]) = [A^-1 0; -B^-1 C A^-1  B^-1] on the matrix cores, scattered back to the
+// original index order as the factor of a Cholesky QR: Q = M X with X[piv[t]][u] = conj(Lt^-1[u][t])
+// (M^H M = L L^H) or Qt = X M with X[u][piv[t]] = Lt^-1[u][t] (M M^H = L L^H).
+#include "mpdo_internal.h"
+#include "prof.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace mpdo {
+namespace {
+
+inline size_t a256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+constexpr int PC_NB = 32;      // columns per panel; the trailing update is one GEMM per panel
+constexpr int PC_THREADS = 256;
+
+// Per-matrix state of the blocked pivoted Cholesky, kept in global memory across panels.
+struct PcState {
+  double* d;             // [batch][n] remaining diagonal A_ii - sum_t |L_it|^2
+  unsigned char* used;   // [batch][n] 1 once row i has been a pivot
+  int* piv;              // [batch][n] pivot row of column k
+  int* rank;             // [batch] columns produced so far (final: the numerical rank)
+  int* stop;             // [batch] 1 once the largest remaining diagonal is at or below thr
+  double* thr;           // [batch] tol * max_i A_ii
+};
+
+__global__ __launch_bounds__(PC_THREADS) void pchol_init_kernel(const double2* A, int n, long long lda, long long bsA,
+                                                                 double tol_rel, double2* L, PcState st) {
+  __shared__ double red[PC_THREADS];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  A += (long long)b * bsA;
+  double m = 0;
+  for (int i = tid; i < n; i += PC_THREADS) {
+    const double v = A[(long long)i * lda + i].x;
+    st.d[(long long)b * n + i] = v;
+    st.used[(long long)b * n + i] = 0;
+    st.piv[(long long)b * n + i] = 0;
+    m = fmax(m, v);
+  }
+  double2* Lb = L + (long long)b * n * n;
+  for (long long e = tid; e < (long long)n * n; e += PC_THREADS) Lb[e] = make_double2(0.0, 0.0);
+  red[tid] = m;
+  __syncthreads();
+  for (int s2 = PC_THREADS / 2; s2 > 0; s2 >>= 1) {
+    if (tid < s2) red[tid] = fmax(red[tid], red[tid + s2]);
+    __syncthreads();
+  }
+  if (tid == 0) {
+    st.thr[b] = tol_rel * red[0];
+    st.rank[b] = 0;
+    st.stop[b] = 0;
+  }
+}
+
+// One panel of up to PC_NB pivot columns of matrix b (left-looking within the panel). A holds
+// the Gram matrix minus the outer products of every earlier panel (the trailing updates), so a
+// column needs only the panel's own earlier columns: L_ik = (A_ip - sum_{t in panel, t<k}
+// L_it conj(L_pt)) / sqrt(d_p). Rows that were pivots get 0. Two barriers per column: the
+// argmax candidates of the waves, then the pivot row's panel entries.
+__global__ __launch_bounds__(PC_THREADS) void pchol_panel_kernel(const double2* A, int n, long long lda, long long bsA,
+                                                                  int j0, double2* L, PcState st) {
+  __shared__ double wv[PC_THREADS / 64];
+  __shared__ int wi[PC_THREADS / 64];
+  __shared__ double2 Lp[PC_NB];
+  const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (st.stop[b]) return;
+  A += (long long)b * bsA;
+  double2* Lb = L + (long long)b * n * n;
+  double* d = st.d + (long long)b * n;
+  unsigned char* used = st.used + (long long)b * n;
+  const double thr = st.thr[b];
+  int k = j0;
+  const int kend = min(j0 + PC_NB, n);
+  for (; k < kend; ++k) {
+    // argmax of the remaining diagonal (ties: lowest index); each thread scans the rows it owns
+    double best = -1.0;
+    int bi = n;
+    for (int i = tid; i < n; i += PC_THREADS)
+      if (!used[i] && d[i] > best) { best = d[i]; bi = i; }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ov = __shfl_xor(best, off);
+      const int oi = __shfl_xor(bi, off);
+      if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+    }
+    if (lane == 0) { wv[wave] = best; wi[wave] = bi; }
+    __syncthreads();
+    // every thread reduces the per-wave candidates itself: no broadcast barrier needed
+    double bv = wv[0];
+    int bx = wi[0];
+    for (int w = 1; w < PC_THREADS / 64; ++w)
+      if (wv[w] > bv || (wv[w] == bv && wi[w] < bx)) { bv = wv[w]; bx = wi[w]; }
+    const int p = (bv > thr && bx < n) ? bx : -1;
+    if (p < 0) break;
+    if (tid == 0) {
+      used[p] = 1;
+      st.piv[(long long)b * n + k] = p;
+    }
+    for (int t = tid; t < k - j0; t += PC_THREADS) Lp[t] = Lb[(long long)(j0 + t) * n + p];
+    __syncthreads();
+    const double piv = sqrt(bv);
+    const double2* Ap = A + (long long)p * lda;   // row p: A_ip = conj(A_pi)
+    for (int i = tid; i < n; i += PC_THREADS) {
+      double2 v;
+      if (i == p) {
+        v = make_double2(piv, 0.0);
+      } else if (used[i]) {
+        v = make_double2(0.0, 0.0);
+      } else {
+        const double2 api = Ap[i];
+        double2 acc = make_double2(api.x, -api.y);
+        for (int t = 0; t < k - j0; ++t) {   // acc -= L_it conj(L_pt)
+          const double2 x = Lb[(long long)(j0 + t) * n + i], y = Lp[t];
+          acc.x -= x.x * y.x + x.y * y.y;
+          acc.y -= x.y * y.x - x.x * y.y;
+        }
+        v = make_double2(acc.x / piv, acc.y / piv);
+        d[i] -= v.x * v.x + v.y * v.y;
+      }
+      Lb[(long long)k * n + i] = v;
+    }
+  }
+  if (tid == 0) {
+    st.rank[b] = k;
+    if (k < kend || k == n) st.stop[b] = 1;
+  }
+}
+
+// Lt[b] (R x R) = the pivot-ordered factor, the identity past the matrix's own rank
+__global__ void gather_lt_kernel(const double2* L, const int* piv, const int* rank, int n, int R, double2* Lt) {
+  const int b = blockIdx.y;
+  const int rb = rank[b];
+  const double2* Lb = L + (long long)b * n * n;
+  const int* pb = piv + (long long)b * n;
+  double2* T = Lt + (long long)b * R * R;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < (long long)R * R;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int t = (int)(e / R), u = (int)(e % R);
+    double2 v = make_double2(t == u ? 1.0 : 0.0, 0.0);
+    if (t < rb && u < rb) v = u <= t ? Lb[(long long)u * n + pb[t]] : make_double2(0.0, 0.0);
+    T[e] = v;
+  }
+}
+
+// In-place inverse of the 64 x 64 lower-triangular diagonal blocks of Lt (one wave per block):
+// lane j solves Lt_blk x = e_j by forward substitution.
+__global__ __launch_bounds__(64) void inv64_kernel(double2* Lt, int R) {
+  __shared__ double2 A[64][65];
+  __shared__ double2 X[64][65];
+  const int blk = blockIdx.x, b = blockIdx.y, j = threadIdx.x;
+  double2* T = Lt + (long long)b * R * R + (long long)blk * 64 * R + blk * 64;
+  for (int r = 0; r < 64; ++r) A[r][j] = T[(long long)r * R + j];
+  __syncthreads();
+  for (int i = 0; i < 64; ++i) {
+    double2 v = make_double2(0.0, 0.0);
+    if (i == j) {
+      const double2 a = A[i][i];
+      const double m = a.x * a.x + a.y * a.y;
+      v = make_double2(a.x / m, -a.y / m);
+    } else if (i > j) {
+      double2 acc = make_double2(0.0, 0.0);
+      for (int k = j; k < i; ++k) {   // acc += A_ik X_kj
+        const double2 a = A[i][k], x = X[k][j];
+        acc.x += a.x * x.x - a.y * x.y;
+        acc.y += a.x * x.y + a.y * x.x;
+      }
+      const double2 a = A[i][i];
+      const double m = a.x * a.x + a.y * a.y;
+      // v = -acc / a_ii
+      v = make_double2(-(acc.x * a.x + acc.y * a.y) / m, -(acc.y * a.x - acc.x * a.y) / m);
+    }
+    X[i][j] = v;
+  }
+  for (int r = 0; r < 64; ++r) T[(long long)r * R + j] = X[r][j];
+}
+
+__global__ void zero_c128_kernel(double2* X, long long per) {
+  double2* Xb = X + (long long)blockIdx.y * per;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < per; e += (long long)gridDim.x * blockDim.x)
+    Xb[e] = make_double2(0.0, 0.0);
+}
+
+// X (n x n complex128, zero-filled before) from Linv = Lt^-1 (R x R): mode 0
+// X[piv[t]][u] = conj(Linv[u][t]), mode 1 X[u][piv[t]] = Linv[u][t], for t, u < rank
+__global__ void scatter_x_kernel(const double2* Linv, const int* piv, const int* rank, int n, int R, int mode,
+                                 double2* X) {
+  const int b = blockIdx.y;
+  const int rb = rank[b];
+  const int* pb = piv + (long long)b * n;
+  const double2* Li = Linv + (long long)b * R * R;
+  double2* Xb = X + (long long)b * n * n;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < (long long)rb * rb;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int t = (int)(e / rb), u = (int)(e % rb);
+    const double2 v = Li[(long long)u * R + t];
+    if (mode == 0) Xb[(long long)pb[t] * n + u] = make_double2(v.x, -v.y);
+    else Xb[(long long)u * n + pb[t]] = v;
+  }
+}
+
+int grid_for(long long n) { return (int)std::max<long long>(1, std::min<long long>((n + 255) / 256, 2048)); }
+
+}  // namespace
+
+size_t pchol_workspace_bytes(int n, int batch) {
+  return a256((size_t)n * sizeof(double) * batch) + a256((size_t)n * batch) + 3 * a256(sizeof(double) * batch);
+}
+
+hipError_t pchol_b(int n, int batch, void* A, long long lda, long long bsA, double tol_rel, double2* L, int* piv,
+                   int* rank, void* ws, hipStream_t st) {
+  if (n <= 0 || batch <= 0) return hipSuccess;
+  hipError_t err;
+  char* p = (char*)ws;
+  PcState pc;
+  pc.d = (double*)p; p += a256((size_t)n * sizeof(double) * batch);
+  pc.used = (unsigned char*)p; p += a256((size_t)n * batch);
+  pc.stop = (int*)p; p += a256(sizeof(double) * batch);
+  pc.thr = (double*)p; p += a256(sizeof(double) * batch);
+  pc.piv = piv;
+  pc.rank = rank;
+  prof::Scope ps(prof::EIG_SMALL, st, 0.0, 16.0 * n * n * batch);
+  hipLaunchKernelGGL(pchol_init_kernel, dim3(batch), dim3(PC_THREADS), 0, st, (const double2*)A, n, lda, bsA, tol_rel,
+                     L, pc);
+  if ((err = hipGetLastError()) != hipSuccess) return err;
+  std::vector<int> hs(batch);
+  for (int j0 = 0; j0 < n; j0 += PC_NB) {
+    hipLaunchKernelGGL(pchol_panel_kernel, dim3(batch), dim3(PC_THREADS), 0, st, (const double2*)A, n, lda, bsA, j0, L,
+                       pc);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+    const int nb = std::min(PC_NB, n - j0);
+    if (j0 + nb >= n) break;
+    // trailing update A <- A - L[:, j0:j0+nb] L[:, j0:j0+nb]^H (zero columns of stopped matrices add nothing)
+    GemmArgs g{};
+    g.M = n; g.N = n; g.K = nb; g.batch = batch;
+    g.A = L + (long long)j0 * n; g.rsA = 1; g.csA = n; g.bsA = (long long)n * n;
+    g.B = L + (long long)j0 * n; g.rsB = n; g.csB = 1; g.bsB = (long long)n * n; g.conjB = 1;
+    g.C = A; g.rsC = lda; g.csC = 1; g.bsC = bsA;
+    g.alpha_re = -1.0; g.beta_re = 1.0;
+    if ((err = gemm(C128_F64ACC, g, st)) != hipSuccess) return err;
+    if ((j0 / PC_NB) % 4 == 3) {   // every 4 panels: stop once every matrix has stopped
+      if ((err = hipMemcpyAsync(hs.data(), pc.stop, sizeof(int) * batch, hipMemcpyDeviceToHost, st)) != hipSuccess)
+        return err;
+      if ((err = hipStreamSynchronize(st)) != hipSuccess) return err;
+      bool all = true;
+      for (int b = 0; b < batch; ++b) all = all && hs[b];
+      if (all) break;
+    }
+  }
+  return hipSuccess;
+}
+
+int pinv_order(int n) {
+  int R = 64;
+  while (R < n) R *= 2;
+  return R;
+}
+
+size_t pchol_inverse_workspace_bytes(int n, int batch) {
+  const size_t R = (size_t)pinv_order(n);
+  return a256(R * R * sizeof(double2) * batch) + a256(R * R / 4 * sizeof(double2) * batch);
+}
+
+hipError_t pchol_inverse_b(int n, int batch, const double2* L, const int* piv, const int* rank, int mode, double2* X,
+                           void* ws, hipStream_t st) {
+  if (n <= 0 || batch <= 0) return hipSuccess;
+  hipError_t err;
+  const int R = pinv_order(n);
+  double2* Lt = (double2*)ws;
+  double2* T = (double2*)((char*)ws + a256((size_t)R * R * sizeof(double2) * batch));
+  prof::Scope ps(prof::EIG_SMALL, st, 2.7 * R * R * R * batch, 16.0 * R * R * batch);
+  hipLaunchKernelGGL(gather_lt_kernel, dim3(grid_for((long long)R * R), batch), dim3(256), 0, st, L, piv, rank, n, R,
+                     Lt);
+  if ((err = hipGetLastError()) != hipSuccess) return err;
+  hipLaunchKernelGGL(inv64_kernel, dim3(R / 64, batch), dim3(64), 0, st, Lt, R);
+  if ((err = hipGetLastError()) != hipSuccess) return err;
+  for (int s = 64; s < R; s *= 2) {
+    for (int m = 0; m < R / (2 * s); ++m) {
+      double2* Ainv = Lt + (long long)(2 * m) * s * (R + 1);
+      double2* Binv = Lt + (long long)(2 * m + 1) * s * (R + 1);
+      double2* C = Lt + (long long)(2 * m + 1) * s * R + (long long)(2 * m) * s;
+      // T = C Ainv
+      GemmArgs g{};
+      g.M = s; g.N = s; g.K = s; g.batch = batch;
+      g.A = C; g.rsA = R; g.csA = 1; g.bsA = (long long)R * R;
+      g.B = Ainv; g.rsB = R; g.csB = 1; g.bsB = (long long)R * R;
+      g.C = T; g.rsC = s; g.csC = 1; g.bsC = (long long)R * R / 4;
+      g.alpha_re = 1.0;
+      if ((err = gemm(C128_F64ACC, g, st)) != hipSuccess) return err;
+      // C <- -Binv T
+      GemmArgs h{};
+      h.M = s; h.N = s; h.K = s; h.batch = batch;
+      h.A = Binv; h.rsA = R; h.csA = 1; h.bsA = (long long)R * R;
+      h.B = T; h.rsB = s; h.csB = 1; h.bsB = (long long)R * R / 4;
+      h.C = C; h.rsC = R; h.csC = 1; h.bsC = (long long)R * R;
+      h.alpha_re = -1.0;
+      if ((err = gemm(C128_F64ACC, h, st)) != hipSuccess) return err;
+    }
+  }
+  const long long nn = (long long)n * n;
+  hipLaunchKernelGGL(zero_c128_kernel, dim3(grid_for(nn), batch), dim3(256), 0, st, X, nn);
+  if ((err = hipGetLastError()) != hipSuccess) return err;
+  hipLaunchKernelGGL(scatter_x_kernel, dim3(grid_for(nn), batch), dim3(256), 0, st, (const double2*)Lt, piv, rank, n, R,
+                     mode, X);
+  return hipGetLastError();
+}
+
+}  // namespace mpdo

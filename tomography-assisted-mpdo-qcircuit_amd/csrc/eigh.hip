@@ -574,114 +574,6 @@ namespace mpdo {
 namespace {
 
 constexpr int PC_MAX = 1024;
-constexpr int PC_NB = 32;      // columns per panel; the trailing update is one GEMM per panel
-constexpr int PC_THREADS = 256;
-
-// Per-matrix state of the blocked pivoted Cholesky, kept in global memory across panels.
-struct PcState {
-  double* d;             // [batch][n] remaining diagonal A_ii - sum_t |L_it|^2
-  unsigned char* used;   // [batch][n] 1 once row i has been a pivot
-  int* rank;             // [batch] columns produced so far (final: the numerical rank)
-  int* stop;             // [batch] 1 once the largest remaining diagonal is at or below thr
-  double* thr;           // [batch] tol * max_i A_ii
-};
-
-__global__ __launch_bounds__(PC_THREADS) void pchol_init_kernel(const double2* A, int n, long long lda, long long bsA,
-                                                                 double tol_rel, double2* L, PcState st) {
-  __shared__ double red[PC_THREADS];
-  const int b = blockIdx.x, tid = threadIdx.x;
-  A += (long long)b * bsA;
-  double m = 0;
-  for (int i = tid; i < n; i += PC_THREADS) {
-    const double v = A[(long long)i * lda + i].x;
-    st.d[(long long)b * n + i] = v;
-    st.used[(long long)b * n + i] = 0;
-    m = fmax(m, v);
-  }
-  double2* Lb = L + (long long)b * n * n;
-  for (long long e = tid; e < (long long)n * n; e += PC_THREADS) Lb[e] = make_double2(0.0, 0.0);
-  red[tid] = m;
-  __syncthreads();
-  for (int s2 = PC_THREADS / 2; s2 > 0; s2 >>= 1) {
-    if (tid < s2) red[tid] = fmax(red[tid], red[tid + s2]);
-    __syncthreads();
-  }
-  if (tid == 0) {
-    st.thr[b] = tol_rel * red[0];
-    st.rank[b] = 0;
-    st.stop[b] = 0;
-  }
-}
-
-// One panel of up to PC_NB pivot columns of matrix b (left-looking within the panel). A holds
-// the Gram matrix minus the outer products of every earlier panel (the trailing updates), so a
-// column needs only the panel's own earlier columns: L_ik = (A_ip - sum_{t in panel, t<k}
-// L_it conj(L_pt)) / sqrt(d_p). Rows that were pivots get 0; L is stored by columns.
-__global__ __launch_bounds__(PC_THREADS) void pchol_panel_kernel(const double2* A, int n, long long lda, long long bsA,
-                                                                  int j0, double2* L, PcState st) {
-  __shared__ double wv[PC_THREADS / 64];
-  __shared__ int wi[PC_THREADS / 64];
-  __shared__ double2 Lp[PC_NB];
-  const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  if (st.stop[b]) return;
-  A += (long long)b * bsA;
-  double2* Lb = L + (long long)b * n * n;
-  double* d = st.d + (long long)b * n;
-  unsigned char* used = st.used + (long long)b * n;
-  const double thr = st.thr[b];
-  int k = j0;
-  const int kend = min(j0 + PC_NB, n);
-  for (; k < kend; ++k) {
-    // argmax of the remaining diagonal (ties: lowest index)
-    double best = -1.0;
-    int bi = n;
-    for (int i = tid; i < n; i += PC_THREADS)
-      if (!used[i] && d[i] > best) { best = d[i]; bi = i; }
-    for (int off = 32; off > 0; off >>= 1) {
-      const double ov = __shfl_xor(best, off);
-      const int oi = __shfl_xor(bi, off);
-      if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-    }
-    if (lane == 0) { wv[wave] = best; wi[wave] = bi; }
-    __syncthreads();
-    // every thread reduces the per-wave candidates itself: no broadcast barrier needed
-    double bv = wv[0];
-    int bx = wi[0];
-    for (int w = 1; w < PC_THREADS / 64; ++w)
-      if (wv[w] > bv || (wv[w] == bv && wi[w] < bx)) { bv = wv[w]; bx = wi[w]; }
-    const int p = (bv > thr && bx < n) ? bx : -1;
-    if (p < 0) break;
-    if (tid == 0) used[p] = 1;
-    for (int t = tid; t < k - j0; t += PC_THREADS) Lp[t] = Lb[(long long)(j0 + t) * n + p];
-    __syncthreads();
-    const double piv = sqrt(bv);
-    const double2* Ap = A + (long long)p * lda;   // row p: A_ip = conj(A_pi)
-    for (int i = tid; i < n; i += PC_THREADS) {
-      double2 v;
-      if (i == p) {
-        v = make_double2(piv, 0.0);
-      } else if (used[i]) {
-        v = make_double2(0.0, 0.0);
-      } else {
-        const double2 api = Ap[i];
-        double2 acc = make_double2(api.x, -api.y);
-        for (int t = 0; t < k - j0; ++t) {   // acc -= L_it conj(L_pt)
-          const double2 x = Lb[(long long)(j0 + t) * n + i], y = Lp[t];
-          acc.x -= x.x * y.x + x.y * y.y;
-          acc.y -= x.y * y.x - x.x * y.y;
-        }
-        v = make_double2(acc.x / piv, acc.y / piv);
-        d[i] -= v.x * v.x + v.y * v.y;
-      }
-      Lb[(long long)k * n + i] = v;
-    }
-  }
-  if (tid == 0) {
-    st.rank[b] = k;
-    if (k < kend || k == n) st.stop[b] = 1;
-  }
-}
-
 // w[b][j] <- w_M[b][j] for j < r, 0 beyond
 __global__ void pad_eigs_kernel(const double* wm, int r, double* w, long long ldw, int n) {
   const int b = blockIdx.x;
@@ -693,8 +585,8 @@ __global__ void pad_eigs_kernel(const double* wm, int r, double* w, long long ld
 size_t eigh_gram_workspace_bytes(int n, int batch) {
   const size_t mat = align256((size_t)n * n * sizeof(double2) * batch);
   const size_t vec = align256((size_t)n * sizeof(double) * batch);
-  return 4 * mat + 2 * vec + align256((size_t)n * batch) + 3 * align256(sizeof(double) * batch) +
-         eigh_workspace_bytes(n, batch) + 256;
+  return 4 * mat + vec + align256((size_t)n * sizeof(int) * batch) + align256(sizeof(int) * batch) +
+         pchol_workspace_bytes(n, batch) + eigh_workspace_bytes(n, batch) + 256;
 }
 
 hipError_t eigh_gram(int n, int batch, void* A, long long lda, long long bsA, double* w, long long ldw, void* V,
@@ -704,53 +596,21 @@ hipError_t eigh_gram(int n, int batch, void* A, long long lda, long long bsA, do
   hipError_t err;
   char* p = (char*)ws;
   const size_t mat = align256((size_t)n * n * sizeof(double2) * batch);
-  const size_t vec = align256((size_t)n * sizeof(double) * batch);
   double2* L = (double2*)p; p += mat;
   double2* M = (double2*)p; p += mat;
   double2* U = (double2*)p; p += mat;
   double2* Y = (double2*)p; p += mat;
-  double* wm = (double*)p; p += vec;
-  PcState pc;
-  pc.d = (double*)p; p += vec;
-  pc.used = (unsigned char*)p; p += align256((size_t)n * batch);
-  pc.rank = (int*)p; p += align256(sizeof(double) * batch);
-  pc.stop = (int*)p; p += align256(sizeof(double) * batch);
-  pc.thr = (double*)p; p += align256(sizeof(double) * batch);
+  double* wm = (double*)p; p += align256((size_t)n * sizeof(double) * batch);
+  int* piv = (int*)p; p += align256((size_t)n * sizeof(int) * batch);
+  int* rank = (int*)p; p += align256(sizeof(int) * batch);
+  void* pws = p; p += pchol_workspace_bytes(n, batch);
   void* ws_inner = p;
   // pivots stop at 1e-14 of the largest diagonal: the Gram mass left out is at most
   // (n - rank) * 1e-14 * max G_ii, the level of the fp64 rounding of the Schur complement
   const double tol = 1e-14;
+  if ((err = pchol_b(n, batch, A, lda, bsA, tol, L, piv, rank, pws, st)) != hipSuccess) return err;
   std::vector<int> hr(batch);
-  {
-    prof::Scope ps(prof::EIG_SMALL, st, 0.0, 16.0 * n * n * batch);
-    hipLaunchKernelGGL(pchol_init_kernel, dim3(batch), dim3(PC_THREADS), 0, st, (const double2*)A, n, lda, bsA, tol, L,
-                       pc);
-    if ((err = hipGetLastError()) != hipSuccess) return err;
-    for (int j0 = 0; j0 < n; j0 += PC_NB) {
-      hipLaunchKernelGGL(pchol_panel_kernel, dim3(batch), dim3(PC_THREADS), 0, st, (const double2*)A, n, lda, bsA, j0,
-                         L, pc);
-      if ((err = hipGetLastError()) != hipSuccess) return err;
-      const int nb = std::min(PC_NB, n - j0);
-      if (j0 + nb >= n) break;
-      // trailing update A <- A - L[:, j0:j0+nb] L[:, j0:j0+nb]^H (zero columns of stopped matrices add nothing)
-      GemmArgs g{};
-      g.M = n; g.N = n; g.K = nb; g.batch = batch;
-      g.A = L + (long long)j0 * n; g.rsA = 1; g.csA = n; g.bsA = (long long)n * n;
-      g.B = L + (long long)j0 * n; g.rsB = n; g.csB = 1; g.bsB = (long long)n * n; g.conjB = 1;
-      g.C = A; g.rsC = lda; g.csC = 1; g.bsC = bsA;
-      g.alpha_re = -1.0; g.beta_re = 1.0;
-      if ((err = gemm(C128_F64ACC, g, st)) != hipSuccess) return err;
-      if ((j0 / PC_NB) % 4 == 3) {   // every 4 panels: stop once every matrix has stopped
-        if ((err = hipMemcpyAsync(hr.data(), pc.stop, sizeof(int) * batch, hipMemcpyDeviceToHost, st)) != hipSuccess)
-          return err;
-        if ((err = hipStreamSynchronize(st)) != hipSuccess) return err;
-        bool all = true;
-        for (int b = 0; b < batch; ++b) all = all && hr[b];
-        if (all) break;
-      }
-    }
-  }
-  if ((err = hipMemcpyAsync(hr.data(), pc.rank, sizeof(int) * batch, hipMemcpyDeviceToHost, st)) != hipSuccess)
+  if ((err = hipMemcpyAsync(hr.data(), rank, sizeof(int) * batch, hipMemcpyDeviceToHost, st)) != hipSuccess)
     return err;
   if ((err = hipStreamSynchronize(st)) != hipSuccess) return err;
   int r = 0;

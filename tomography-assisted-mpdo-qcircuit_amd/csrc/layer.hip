@@ -281,7 +281,8 @@ hipError_t gram_rows_b_(int line, const float2* X, int n, int cols, long long bs
 #define gram_rows_b(...) gram_rows_b_(__LINE__, __VA_ARGS__)
 
 // The two cores whose consumers read only the leading eigenvectors and the eigenvalue sums (the
-// gate-time cut and the chi step) use the pivoted-Cholesky-preconditioned solver (eigh_gram);
+// gate-time cut and the chi step) and the first pass of every eigen-QR/LQ (whose second pass
+// re-orthonormalises) use the pivoted-Cholesky-preconditioned solver (eigh_gram);
 // MPDO_LAYER_GRAM_EIG=0 selects the plain Jacobi solver for them.
 bool use_gram_eig() {
   static const bool on = [] {
@@ -324,53 +325,106 @@ mpdo_status eig_b_(int line, int n, int batch, double2* G, EigB& e, hipStream_t 
 }
 #define eig_b(...) eig_b_(__LINE__, __VA_ARGS__)
 
-// Two-pass eigen-QR of a batch of tall matrices M (rows x cols, rows > cols):
-// Q (rows x cols) with orthonormal columns, Rf (cols x cols) complex64 with M = Q Rf.
+// Cholesky QR with column pivoting, twice (the second pass restores orthonormality lost to
+// the conditioning of the first): the Gram matrix of the input, its pivoted Cholesky factor and
+// the inverse of that factor's pivot-ordered triangle (pchol_b, pchol_inverse_b), then one GEMM.
+// Directions whose pivot falls below FLOOR of the largest diagonal get zero columns of Q and zero
+// rows of R, as the zero-filled null directions of the reference's QR.
 struct QRWS {
-  double2 *G, *P1, *P2;
-  float2 *F, *A1;
-  EigB e1, e2;
+  double2 *G, *L1, *L2, *F, *W;
+  int *piv1, *piv2, *rank1, *rank2;
+  void *pws, *iws;
+  float2* A1;
 };
 void carve_qr(Carver& c, int rows, int cols, int batch, QRWS& w) {
-  w.G = c.get<double2>((size_t)cols * cols * batch);
-  w.P1 = c.get<double2>((size_t)cols * cols * batch);
-  w.P2 = c.get<double2>((size_t)cols * cols * batch);
-  w.F = c.get<float2>((size_t)cols * cols * batch);
+  const size_t n = (size_t)cols;
+  w.G = c.get<double2>(n * n * batch);
+  w.L1 = c.get<double2>(n * n * batch);
+  w.L2 = c.get<double2>(n * n * batch);
+  w.piv1 = c.get<int>(n * batch);
+  w.piv2 = c.get<int>(n * batch);
+  w.rank1 = c.get<int>((size_t)batch);
+  w.rank2 = c.get<int>((size_t)batch);
+  w.pws = c.get<char>(pchol_workspace_bytes(cols, batch));
+  w.iws = c.get<char>(pchol_inverse_workspace_bytes(cols, batch));
+  w.F = c.get<double2>(n * n * batch);
+  w.W = c.get<double2>((size_t)rows * cols * batch);
   w.A1 = c.get<float2>((size_t)rows * cols * batch);
-  carve_eig(c, cols, batch, w.e1);
-  carve_eig(c, cols, batch, w.e2);
 }
+hipError_t k_pchol(int n, int batch, double2* G, double tol, double2* L, int* piv, int* rank, QRWS& w, hipStream_t s,
+                   int line) {
+  const long long nn = (long long)n * n;
+  if (!fits(G, batch * nn, sizeof(double2), "pchol input", line) ||
+      !fits(L, batch * nn, sizeof(double2), "pchol factor", line) ||
+      !fits(piv, (long long)batch * n, sizeof(int), "pchol pivots", line) ||
+      !fits(rank, batch, sizeof(int), "pchol ranks", line) ||
+      !fits(w.pws, (long long)pchol_workspace_bytes(n, batch), 1, "pchol workspace", line))
+    return hipErrorInvalidValue;
+  if (t_dry) return hipSuccess;
+  return pchol_b(n, batch, G, n, nn, tol, L, piv, rank, w.pws, s);
+}
+hipError_t k_pinv(int n, int batch, const double2* L, const int* piv, const int* rank, int mode, double2* X, QRWS& w,
+                  hipStream_t s, int line) {
+  const long long nn = (long long)n * n;
+  if (!fits(L, batch * nn, sizeof(double2), "pivoted inverse factor", line) ||
+      !fits(piv, (long long)batch * n, sizeof(int), "pivoted inverse pivots", line) ||
+      !fits(rank, batch, sizeof(int), "pivoted inverse ranks", line) ||
+      !fits(X, batch * nn, sizeof(double2), "pivoted inverse output", line) ||
+      !fits(w.iws, (long long)pchol_inverse_workspace_bytes(n, batch), 1, "pivoted inverse workspace", line))
+    return hipErrorInvalidValue;
+  if (t_dry) return hipSuccess;
+  return pchol_inverse_b(n, batch, L, piv, rank, mode, X, w.iws, s);
+}
+#define K_PCHOL(n, batch, G, tol, L, piv, rank, w, s) k_pchol(n, batch, G, tol, L, piv, rank, w, s, __LINE__)
+#define K_PINV(n, batch, L, piv, rank, mode, X, w, s) k_pinv(n, batch, L, piv, rank, mode, X, w, s, __LINE__)
+
+// Q (rows x cols) with orthonormal columns, Rf (cols x cols) complex64 with M = Q Rf; M is tall.
+// M = Q1 L1^H (Q1 = M X1), Q1 = Q L2^H (Q = Q1 X2), so Rf = L2^H L1^H. The products with the
+// inverse factors X (whose entries grow with the condition number) take complex128 operands.
 mpdo_status qr_b(const float2* M, int rows, int cols, float2* Q, float2* Rf, int batch, QRWS& w, hipStream_t s) {
   const long long cc = (long long)cols * cols, rc = (long long)rows * cols;
   LCK(gram_cols_b(M, rows, cols, rc, w.G, batch, s));
-  LST(eig_b(cols, batch, w.G, w.e1, s));
-  LCK(K_FACTOR(w.e1.V, cols, cc, w.e1.w, cols, cols, cols, cols, -0.5, FLOOR, 0, w.F, cols, cc, batch, s));
-  LCK(mmb(g_tensor_prec, rows, cols, cols, M, cols, rc, w.F, cols, cc, w.A1, cols, rc, batch, s));
+  LCK(K_PCHOL(cols, batch, w.G, FLOOR, w.L1, w.piv1, w.rank1, w, s));
+  LCK(K_PINV(cols, batch, w.L1, w.piv1, w.rank1, 0, w.F, w, s));
+  LCK(k_widen(M, w.W, rc, rc, rc, batch, s, __LINE__));
+  LCK(mmb(C128_IN_C64_OUT, rows, cols, cols, w.W, cols, rc, w.F, cols, cc, w.A1, cols, rc, batch, s));
   LCK(gram_cols_b(w.A1, rows, cols, rc, w.G, batch, s));
-  LST(eig_b(cols, batch, w.G, w.e2, s));
-  LCK(K_FACTOR(w.e2.V, cols, cc, w.e2.w, cols, cols, cols, cols, -0.5, FLOOR, 0, w.F, cols, cc, batch, s));
-  LCK(mmb(g_tensor_prec, rows, cols, cols, w.A1, cols, rc, w.F, cols, cc, Q, cols, rc, batch, s));
-  LCK(K_FACTOR64(w.e2.V, cols, cc, w.e2.w, cols, cols, cols, cols, 0.5, FLOOR, 1, w.P2, cols, cc, batch, s));
-  LCK(K_FACTOR64(w.e1.V, cols, cc, w.e1.w, cols, cols, cols, cols, 0.5, FLOOR, 1, w.P1, cols, cc, batch, s));
-  LCK(mmb(C128_IN_C64_OUT, cols, cols, cols, w.P2, cols, cc, w.P1, cols, cc, Rf, cols, cc, batch, s));
+  LCK(K_PCHOL(cols, batch, w.G, FLOOR, w.L2, w.piv2, w.rank2, w, s));
+  LCK(K_PINV(cols, batch, w.L2, w.piv2, w.rank2, 0, w.F, w, s));
+  LCK(k_widen(w.A1, w.W, rc, rc, rc, batch, s, __LINE__));
+  LCK(mmb(C128_IN_C64_OUT, rows, cols, cols, w.W, cols, rc, w.F, cols, cc, Q, cols, rc, batch, s));
+  // Rf[u][j] = sum_t conj(L2[t][u]) conj(L1[j][t]); L stored by columns: L[x][y] at L[y cols + x]
+  GemmArgs g{};
+  g.M = cols; g.N = cols; g.K = cols; g.batch = batch;
+  g.A = w.L2; g.rsA = cols; g.csA = 1; g.bsA = cc; g.conjA = 1;
+  g.B = w.L1; g.rsB = cols; g.csB = 1; g.bsB = cc; g.conjB = 1;
+  g.C = Rf; g.rsC = cols; g.csC = 1; g.bsC = cc;
+  g.alpha_re = 1;
+  LCK(K_GEMM(C128_IN_C64_OUT, g, s));
   return MPDO_OK;
 }
-// Two-pass eigen-LQ of a batch of wide matrices X (rows x cols, cols > rows):
-// Qt (rows x cols) with orthonormal rows, Lf (rows x rows) complex64 with X = Lf Qt.
+// Qt (rows x cols) with orthonormal rows, Lf (rows x rows) complex64 with X = Lf Qt; X is wide.
+// X = L1 Qt1 (Qt1 = X1 X), Qt1 = L2 Qt (Qt = X2 Qt1), so Lf = L1 L2.
 mpdo_status lq_b(const float2* X, int rows, int cols, float2* Qt, float2* Lf, int batch, QRWS& w, hipStream_t s) {
   const long long rr = (long long)rows * rows, rc = (long long)rows * cols;
   LCK(gram_rows_b(X, rows, cols, rc, w.G, batch, s));
-  LST(eig_b(rows, batch, w.G, w.e1, s));
-  LCK(K_FACTOR(w.e1.V, rows, rr, w.e1.w, rows, rows, rows, rows, -0.5, FLOOR, 1, w.F, rows, rr, batch, s));
-  LCK(mmb(g_tensor_prec, rows, cols, rows, w.F, rows, rr, X, cols, rc, w.A1, cols, rc, batch, s));
+  LCK(K_PCHOL(rows, batch, w.G, FLOOR, w.L1, w.piv1, w.rank1, w, s));
+  LCK(K_PINV(rows, batch, w.L1, w.piv1, w.rank1, 1, w.F, w, s));
+  LCK(k_widen(X, w.W, rc, rc, rc, batch, s, __LINE__));
+  LCK(mmb(C128_IN_C64_OUT, rows, cols, rows, w.F, rows, rr, w.W, cols, rc, w.A1, cols, rc, batch, s));
   LCK(gram_rows_b(w.A1, rows, cols, rc, w.G, batch, s));
-  LST(eig_b(rows, batch, w.G, w.e2, s));
-  LCK(K_FACTOR(w.e2.V, rows, rr, w.e2.w, rows, rows, rows, rows, -0.5, FLOOR, 1, w.F, rows, rr, batch, s));
-  LCK(mmb(g_tensor_prec, rows, cols, rows, w.F, rows, rr, w.A1, cols, rc, Qt, cols, rc, batch, s));
-  // X = (W1 L1^{1/2}) (W2 L2^{1/2}) Qt
-  LCK(K_FACTOR64(w.e1.V, rows, rr, w.e1.w, rows, rows, rows, rows, 0.5, FLOOR, 0, w.P1, rows, rr, batch, s));
-  LCK(K_FACTOR64(w.e2.V, rows, rr, w.e2.w, rows, rows, rows, rows, 0.5, FLOOR, 0, w.P2, rows, rr, batch, s));
-  LCK(mmb(C128_IN_C64_OUT, rows, rows, rows, w.P1, rows, rr, w.P2, rows, rr, Lf, rows, rr, batch, s));
+  LCK(K_PCHOL(rows, batch, w.G, FLOOR, w.L2, w.piv2, w.rank2, w, s));
+  LCK(K_PINV(rows, batch, w.L2, w.piv2, w.rank2, 1, w.F, w, s));
+  LCK(k_widen(w.A1, w.W, rc, rc, rc, batch, s, __LINE__));
+  LCK(mmb(C128_IN_C64_OUT, rows, cols, rows, w.F, rows, rr, w.W, cols, rc, Qt, cols, rc, batch, s));
+  // Lf[i][v] = sum_u L1[i][u] L2[u][v]; L stored by columns
+  GemmArgs g{};
+  g.M = rows; g.N = rows; g.K = rows; g.batch = batch;
+  g.A = w.L1; g.rsA = 1; g.csA = rows; g.bsA = rr;
+  g.B = w.L2; g.rsB = 1; g.csB = rows; g.bsB = rr;
+  g.C = Lf; g.rsC = rows; g.csC = 1; g.bsC = rr;
+  g.alpha_re = 1;
+  LCK(K_GEMM(C128_IN_C64_OUT, g, s));
   return MPDO_OK;
 }
 

@@ -50,4 +50,19 @@ hipError_t eigh_gram(int n, int batch, void* A, long long lda, long long bsA,
                      double* w, long long ldw, void* V, long long ldv, long long bsV,
                      void* ws, hipStream_t s, int* sweeps_out);
 
+// Pivoted Cholesky of positive semidefinite Gram matrices A[b] (n x n, leading dim lda, batch
+// stride bsA; destroyed): L[b] (n x n, stored by columns: L[b][k n + i] = L_ik) with A = L L^H,
+// piv[b][k] the row pivoted at column k, rank[b] the columns produced before the largest remaining
+// diagonal fell to tol_rel * max_i A_ii (later columns zero). `ws`: pchol_workspace_bytes.
+size_t pchol_workspace_bytes(int n, int batch);
+hipError_t pchol_b(int n, int batch, void* A, long long lda, long long bsA, double tol_rel, double2* L, int* piv,
+                   int* rank, void* ws, hipStream_t s);
+// The Cholesky-QR factor of a pivoted factor, as complex128 X[b] (n x n): with Lt[t][u] =
+// L[piv[t]][u] (t, u < rank), mode 0 gives X[piv[t]][u] = conj(Lt^-1[u][t]) (Q = M X has
+// orthonormal columns when M^H M = L L^H and M = Q L^H), mode 1 X[u][piv[t]] = Lt^-1[u][t]
+// (Qt = X M when M M^H = L L^H, M = L Qt); zero elsewhere. `ws`: pchol_inverse_workspace_bytes.
+size_t pchol_inverse_workspace_bytes(int n, int batch);
+hipError_t pchol_inverse_b(int n, int batch, const double2* L, const int* piv, const int* rank, int mode, double2* X,
+                           void* ws, hipStream_t s);
+
 }  // namespace mpdo

@@ -39,6 +39,10 @@ mpdo_status mpdo_debug_eigh_gram(int n, int batch, void* A, double* w, void* V, 
 mpdo_status mpdo_debug_layer_dry_run(int batch, int n, const mpdo_site* in, int K1, int parity, int K2, int chi,
                                      int kappa, const mpdo_site* out, unsigned long long* workspace_bytes);
 
+/* The GEMMs the last dry run of the fused layer would launch, in order: out [6 n] = source line,
+ * M, N, K, batch, precision mode; returns n (fills at most cap ints). */
+int mpdo_debug_layer_gemm_log(int* out, int cap);
+
 /* Per pair of the last mpdo_layer_batched call on this thread: the largest number of singular
  * values its gate-time split kept in any chain (2 p1t when none was dropped). Writes up to `cap`
  * values, returns the number of pairs. */
@@ -54,6 +58,10 @@ mpdo_status mpdo_debug_prof_enable(int on);
 mpdo_status mpdo_debug_prof_read(int cls, long long* launches, double* ms, double* flops, double* bytes);
 /* The blocked eigensolver calls (order > 64) made while timing was on: (order, batch, sweeps)
  * triples, up to cap ints written; returns the number of calls. */
+/* Per-shape totals of one timing class while timing is on: shapes [5 n] (for GEMMs: M, N, K,
+ * batch, precision mode), stats [3 n] (launches, ms, algorithmic flops); returns the number of
+ * distinct shapes (fills at most cap). */
+int mpdo_debug_prof_shapes(int cls, int* shapes, double* stats, int cap);
 int mpdo_debug_prof_eig_log(int* out, int cap);
 
 #ifdef __cplusplus

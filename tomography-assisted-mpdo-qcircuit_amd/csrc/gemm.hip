@@ -194,9 +194,11 @@ hipError_t gemm(Prec p, const GemmArgs& g, hipStream_t s) {
   const double inb = (p == C128_F64ACC || p == C128_IN_C64_OUT) ? 16 : 8;
   const double outb = (p == C64_F64ACC || p == C128_F64ACC) ? 16 : 8;
   const double mn = (double)g.M * g.N, nb = g.batch;
+  const int shape[5] = {g.M, g.N, g.K, g.batch, (int)p};
   prof::Scope ps(prof::GEMM, s, 8.0 * mn * g.K * nb,
                  nb * (inb * ((double)g.M * g.K + (double)g.K * g.N) +
-                       outb * mn * ((g.beta_re != 0 || g.beta_im != 0) ? 2 : 1)));
+                       outb * mn * ((g.beta_re != 0 || g.beta_im != 0) ? 2 : 1)),
+                 shape);
   switch (p) {
     case C64_F32ACC: return launch<float2, float, float2, 128, 128, 16>(g, s);
     case C64_F64ACC: return launch<float2, double, double2, 64, 64, 16>(g, s);

@@ -78,6 +78,7 @@ thread_local std::string t_oob;
 thread_local bool t_dry = false;
 thread_local size_t t_ws_bytes = 0;   // workspace size of the last planned layer
 thread_local std::vector<int> t_kept;   // per pair of the last layer: max kept count of its gate-time split
+thread_local std::vector<int> t_gemms;  // dry run: (source line, M, N, K, batch, precision) of every GEMM
 
 void add_span(const void* p, size_t bytes) {
   if (t_spans && p && bytes) t_spans->push_back(Span{(uintptr_t)p, (uintptr_t)p + bytes});
@@ -116,7 +117,10 @@ hipError_t k_gemm(Prec p, const GemmArgs& g, hipStream_t s, int line) {
       !fits(g.B, ext(g.batch, g.bsB, g.K, g.rsB, g.N, g.csB), in_bytes(p), "GEMM operand B", line) ||
       !fits(g.C, ext(g.batch, g.bsC, g.M, g.rsC, g.N, g.csC), out_bytes(p), "GEMM result C", line))
     return hipErrorInvalidValue;
-  if (t_dry) return hipSuccess;
+  if (t_dry) {
+    t_gemms.insert(t_gemms.end(), {line, g.M, g.N, g.K, g.batch, (int)p});
+    return hipSuccess;
+  }
   return gemm(p, g, s);
 }
 #define K_GEMM(p, g, s) k_gemm((p), (g), (s), __LINE__)
@@ -222,6 +226,17 @@ hipError_t k_mask(const double2* V, long long ldv, long long bsV, int rows, int 
   return mask_cols_b(V, ldv, bsV, rows, cols, k, out, ldo, bsO, batch, s);
 }
 
+hipError_t k_kraus_gram(const double2* Pg, const float2* Gt, int K, int P1, double2* out, int batch, hipStream_t s,
+                        int line) {
+  const long long n = 2LL * P1;
+  if (!fits(Pg, batch * 4 * n * n, sizeof(double2), "core Gram blocks", line) ||
+      !fits(Gt, 16LL * K, sizeof(float2), "core Gram channel", line) ||
+      !fits(out, batch * n * n, sizeof(double2), "core Gram output", line))
+    return hipErrorInvalidValue;
+  if (t_dry) return hipSuccess;
+  return kraus_gram(Pg, Gt, K, P1, out, batch, s);
+}
+
 hipError_t k_copy(void* dst, const void* src, size_t bytes, hipStream_t s, int line) {
   if (!fits(dst, (long long)bytes, 1, "copy destination", line) || !fits(src, (long long)bytes, 1, "copy source", line))
     return hipErrorInvalidValue;
@@ -280,10 +295,36 @@ hipError_t gram_rows_b_(int line, const float2* X, int n, int cols, long long bs
 #define gram_cols_b(...) gram_cols_b_(__LINE__, __VA_ARGS__)
 #define gram_rows_b(...) gram_rows_b_(__LINE__, __VA_ARGS__)
 
-// The two cores whose consumers read only the leading eigenvectors and the eigenvalue sums (the
-// gate-time cut and the chi step) and the first pass of every eigen-QR/LQ (whose second pass
-// re-orthonormalises) use the pivoted-Cholesky-preconditioned solver (eigh_gram);
-// MPDO_LAYER_GRAM_EIG=0 selects the plain Jacobi solver for them.
+// Gram of a tall, narrow batch (n <= SPLIT_N columns, contiguous X of rows x n per matrix): the
+// rows are split into S slices whose partial Grams are one batched GEMM (batch B S, enough
+// workgroups to fill the chip), then summed in a fixed order.
+constexpr int SPLIT_N = 64;
+constexpr int SPLIT_S = 64;
+hipError_t gram_cols_split_b_(int line, const float2* X, int rows, int n, double2* parts, double2* G, int batch,
+                              hipStream_t s) {
+  int S = SPLIT_S;
+  while (S > 1 && (rows % S != 0 || rows / S < 256)) S /= 2;
+  if (S == 1) return gram_cols_b_(line, X, rows, n, (long long)rows * n, G, batch, s);
+  const long long nn = (long long)n * n;
+  GemmArgs g{};
+  g.M = n; g.N = n; g.K = rows / S; g.batch = batch * S;
+  g.A = X; g.rsA = 1; g.csA = n; g.bsA = (long long)(rows / S) * n; g.conjA = 1;
+  g.B = X; g.rsB = n; g.csB = 1; g.bsB = (long long)(rows / S) * n;
+  g.C = parts; g.rsC = n; g.csC = 1; g.bsC = nn;
+  g.alpha_re = 1;
+  hipError_t e = k_gemm(C64_F64ACC, g, s, line);
+  if (e != hipSuccess) return e;
+  if (!fits(parts, (long long)batch * S * nn, sizeof(double2), "split Gram partials", line) ||
+      !fits(G, (long long)batch * nn, sizeof(double2), "split Gram output", line))
+    return hipErrorInvalidValue;
+  if (t_dry) return hipSuccess;
+  return sum_parts(parts, S, nn, G, batch, s);
+}
+#define gram_cols_split_b(...) gram_cols_split_b_(__LINE__, __VA_ARGS__)
+
+// The Gram matrices whose consumers read only the leading eigenvectors and the eigenvalue sums
+// (the gate-time cut, the chi steps and the kappa cuts) use the pivoted-Cholesky-preconditioned
+// solver (eigh_gram); MPDO_LAYER_GRAM_EIG=0 selects the plain Jacobi solver for them.
 bool use_gram_eig() {
   static const bool on = [] {
     const char* v = getenv("MPDO_LAYER_GRAM_EIG");
@@ -570,6 +611,7 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     explicit DryScope(bool d) { t_dry = d; }
     ~DryScope() { t_dry = false; }
   } dry_scope(dry);
+  if (dry) t_gemms.clear();
   const int n = P.n, B = batch, K1 = P.K1, K2 = P.K2;
   // ---------------- layout ----------------
   std::vector<PairStore> pairs;
@@ -596,6 +638,7 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     }
   // sizes (elements per chain) of the shared scratch buffers: each the maximum over its uses
   auto kapplies = [&](int i) { return P.out[i].a != P.chi_out[i].a; };
+  size_t mxPg = 1;
   size_t mxQR_rows = 1, mxQR_cols = 1, mxR = 1, mxY = 1, mxOm = 1, mxZ = 1, mxQ2P = 1, mxC = 1;
   size_t mxTmin = 1, mxWm = 1, mxTm = 1, mxFL = 1, mxPn = 1, mxRT = 1, mxX1 = 1, mxLast = 1, mxTcur = 1;
   size_t mxTchi = 1, mxMp = 1, mxKout = 1, mxX1t = 1, mxQ1t = 1, mxR1t = 1;
@@ -642,6 +685,7 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     if (!ps.q2id) { mxQR_rows = std::max(mxQR_rows, (size_t)ps.cols2); mxQR_cols = std::max(mxQR_cols, (size_t)ps.rows2); }
     mxEigQR = std::max(mxEigQR, std::max({2 * ps.m, ps.cols1, ps.rows2}));
     mxEigC = std::max(mxEigC, 2 * ps.p1t);
+    mxPg = std::max(mxPg, (size_t)16 * ps.p1t * ps.p1t);
     // QR of (Ra (x) I) Um, [2 p1 x mq] with mq <= 2 p1t, factored only when 2 p1 > mq
     mxQR_rows = std::max(mxQR_rows, (size_t)2 * ps.p1);
     mxQR_cols = std::max(mxQR_cols, (size_t)std::min(2 * ps.p1, 2 * ps.p1t));
@@ -676,7 +720,7 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
   Site1q* dtab = nullptr;
   float2 *Mlast = nullptr, *FLq, *Rcur, *Rnext, *RT, *X1, *X1t, *Q1t, *R1t, *Cs, *Cp, *Cu, *Zb, *Q2P, *FL, *FR, *Tmin, *Wm,
          *Tm, *Pn, *Pn2, *PW, *Pp, *Tchi, *Mp, *Vk, *Kout, *dG, *Tcur2, *dK;
-  double2 *G, *GR, *Om64, *Q2w, *Z64, *H64, *Cw64, *Z64c;
+  double2 *G, *GR, *Om64, *Q2w, *Z64, *H64, *Cw64, *Z64c, *Gparts, *Pg;
   double* ddisc;
   int* dk;
   EigB eE{};
@@ -745,6 +789,8 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     dG = c.get<float2>((size_t)pairs.size() * 16 * K2 + 16);
     dK = c.get<float2>((size_t)n * B * 4 * K1);
     G = c.get<double2>((size_t)mxEig * mxEig * B);
+    Gparts = c.get<double2>((size_t)SPLIT_S * SPLIT_N * SPLIT_N * B);
+    Pg = c.get<double2>(mxPg * B);
     GR = c.get<double2>((size_t)mxEig * mxEig * B);
     carve_eig(c, mxEig, B, eE);
     carve_qr(c, (int)mxQR_rows, (int)mxQR_cols, B, qw);
@@ -909,11 +955,12 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
         LCK(K_GEMM(g_tensor_prec, g, s));
       }
     {
-      const long long szCt = 2LL * p1t * 2 * K2 * ps.p2;
-      LCK(k_channel2q(ps.Y, dG + (pi - 1) * 16 * K2, K2, p1t, ps.p2, Cs, s, B, 4LL * p1t * ps.p2, szCt, __LINE__));
       const int nT = 2 * p1t;
       const int len_gate = (int)std::min(2LL * l * a1, 2LL * a2 * K2 * r);
-      LCK(gram_rows_b(Cs, nT, (int)(2LL * K2 * ps.p2), szCt, G, B, s));
+      // Gram of the core through its Kraus structure: the 4 p1t x 4 p1t Gram of Y's blocks and the
+      // 8 x 8 Gram of the channel (16x fewer flops than C C^H over the 2 K2 p2 columns)
+      LCK(gram_rows_b(ps.Y, 4 * p1t, ps.p2, 4LL * p1t * ps.p2, Pg, B, s));
+      LCK(k_kraus_gram(Pg, dG + (pi - 1) * 16 * K2, K2, p1t, G, B, s, __LINE__));
       LST(eig_b(nT, B, G, eE, s, true));
       LCK(K_KEEP(eE.w, nT, nT, len_gate, -1, GATE_EPS, 0, dk, ddisc + (size_t)B * (nsteps + n + i), B, s));
       // the projector onto the kept directions is applied only when some chain drops one; the
@@ -1003,8 +1050,9 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     const Dims& c = P.chi_out[i];
     const long long sz = (long long)c.l * 2 * c.a * c.r;
     const int rows = 2 * c.l * c.r, a = c.a, k = P.out[i].a;
-    LCK(gram_cols_b(Mq, rows, a, sz, G, B, s));
-    LST(eig_b(a, B, G, eE, s));
+    if (a <= SPLIT_N) LCK(gram_cols_split_b(Mq, rows, a, Gparts, G, B, s));
+    else LCK(gram_cols_b(Mq, rows, a, sz, G, B, s));
+    LST(eig_b(a, B, G, eE, s, true));
     LCK(K_KEEP(eE.w, a, a, std::min(rows, a), k, -1.0, 1, dk, ddisc + (size_t)B * (nsteps + i), B, s));
     LCK(K_FACTOR(eE.V, a, (long long)a * a, eE.w, a, a, a, k, 0.0, 0.0, 0, Vk, k, (long long)a * k, B, s));
     LCK(mmb(g_tensor_prec, rows, k, a, Mq, a, sz, Vk, k, (long long)a * k, Kout, k, (long long)rows * k, B, s));
@@ -1029,7 +1077,7 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
   auto chi_step = [&](const float2* Bt, int D, long long colsB, int k, int len, float2* Tright, float2* PnOut,
                       int disc_slot) -> mpdo_status {
     LCK(gram_rows_b(Bt, D, (int)colsB, D * colsB, G, B, s));
-    LST(eig_b(D, B, G, eE, s));
+    LST(eig_b(D, B, G, eE, s, true));
     LCK(K_KEEP(eE.w, D, D, len, k, -1.0, 1, dk, ddisc + (size_t)B * disc_slot, B, s));
     LCK(K_FACTOR(eE.V, D, (long long)D * D, eE.w, D, D, D, k, -0.25, SPLIT_FLOOR, 1, FR, D, (long long)k * D, B, s));
     LCK(mmb(g_tensor_prec, k, (int)colsB, D, FR, D, (long long)k * D, Bt, colsB, D * colsB, Tright, colsB,
@@ -1220,6 +1268,12 @@ mpdo_status mpdo_layer_batched(int batch, int n, const mpdo_site* in, const floa
     if (out[i].l != P.out[i].l || out[i].a != P.out[i].a || out[i].r != P.out[i].r)
       return layer_fail(MPDO_EINVAL, "output descriptors must carry mpdo_layer_dims dims");
   return run_layer(batch, P, in, kraus1, gates2, out, discarded, (hipStream_t)stream, false);
+}
+
+int mpdo_debug_layer_gemm_log(int* out, int cap) {
+  const int n = (int)t_gemms.size() / 6;
+  for (int i = 0; i < 6 * n && i < cap; ++i) out[i] = t_gemms[i];
+  return n;
 }
 
 int mpdo_debug_layer_kept(int* out, int cap) {

@@ -13,10 +13,11 @@ bool enabled();
 // one record per eigensolver call while timing is on: order, batch, sweeps
 void eig_log(int n, int batch, int sweeps);
 
-// RAII: records a start event on construction and a stop event on destruction
+// RAII: records a start event on construction and a stop event on destruction; `shape` (five
+// ints, e.g. M, N, K, batch, precision of a GEMM) keys a per-shape breakdown of the class
 class Scope {
  public:
-  Scope(int cls, hipStream_t s, double flops, double bytes);
+  Scope(int cls, hipStream_t s, double flops, double bytes, const int* shape = nullptr);
   ~Scope();
   Scope(const Scope&) = delete;
   Scope& operator=(const Scope&) = delete;
@@ -25,6 +26,8 @@ class Scope {
   int cls_;
   hipStream_t s_;
   double flops_, bytes_;
+  int shape_[5] = {0, 0, 0, 0, 0};
+  bool has_shape_ = false;
   hipEvent_t e0_ = nullptr, e1_ = nullptr;
 };
 

@@ -1,5 +1,7 @@
 #include "prof.h"
 
+#include <array>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -14,6 +16,12 @@ struct Rec {
   int cls;
   hipEvent_t e0, e1;
   double flops, bytes;
+  bool has_shape;
+  std::array<int, 5> shape;
+};
+struct ShapeAcc {
+  long long calls = 0;
+  double ms = 0, flops = 0;
 };
 struct Acc {
   long long launches = 0;
@@ -26,6 +34,7 @@ std::vector<Rec> g_recs;
 std::vector<hipEvent_t> g_free;
 Acc g_acc[NCLS];
 std::vector<int> g_eig;   // (n, batch, sweeps) triples
+std::map<std::pair<int, std::array<int, 5>>, ShapeAcc> g_shapes;
 
 // fold finished records into the totals (waits for the last one) and recycle their events
 void drain_locked() {
@@ -39,6 +48,12 @@ void drain_locked() {
       a.ms += ms;
       a.flops += r.flops;
       a.bytes += r.bytes;
+      if (r.has_shape) {
+        ShapeAcc& sa = g_shapes[{r.cls, r.shape}];
+        sa.calls += 1;
+        sa.ms += ms;
+        sa.flops += r.flops;
+      }
     }
     g_free.push_back(r.e0);
     g_free.push_back(r.e1);
@@ -69,8 +84,13 @@ void eig_log(int n, int batch, int sweeps) {
   g_eig.push_back(sweeps);
 }
 
-Scope::Scope(int cls, hipStream_t s, double flops, double bytes) : cls_(cls), s_(s), flops_(flops), bytes_(bytes) {
+Scope::Scope(int cls, hipStream_t s, double flops, double bytes, const int* shape)
+    : cls_(cls), s_(s), flops_(flops), bytes_(bytes) {
   if (!g_on) return;
+  if (shape) {
+    has_shape_ = true;
+    for (int i = 0; i < 5; ++i) shape_[i] = shape[i];
+  }
   std::lock_guard<std::mutex> lk(g_mu);
   e0_ = take_locked();
   e1_ = take_locked();
@@ -81,7 +101,8 @@ Scope::~Scope() {
   if (!e0_) return;
   std::lock_guard<std::mutex> lk(g_mu);
   (void)hipEventRecord(e1_, s_);
-  g_recs.push_back(Rec{cls_, e0_, e1_, flops_, bytes_});
+  std::array<int, 5> sh{shape_[0], shape_[1], shape_[2], shape_[3], shape_[4]};
+  g_recs.push_back(Rec{cls_, e0_, e1_, flops_, bytes_, has_shape_, sh});
   if (g_recs.size() >= 4096) drain_locked();
 }
 
@@ -97,6 +118,7 @@ mpdo_status mpdo_debug_prof_enable(int on) {
   if (on) {
     for (auto& a : g_acc) a = Acc{};
     g_eig.clear();
+    g_shapes.clear();
   }
   g_on = on != 0;
   return MPDO_OK;
@@ -113,6 +135,24 @@ mpdo_status mpdo_debug_prof_read(int cls, long long* launches, double* ms, doubl
   if (flops) *flops = a.flops;
   if (bytes) *bytes = a.bytes;
   return MPDO_OK;
+}
+
+int mpdo_debug_prof_shapes(int cls, int* shapes, double* stats, int cap) {
+  using namespace mpdo::prof;
+  std::lock_guard<std::mutex> lk(g_mu);
+  drain_locked();
+  int n = 0;
+  for (const auto& kv : g_shapes) {
+    if (kv.first.first != cls) continue;
+    if (n < cap) {
+      for (int i = 0; i < 5; ++i) shapes[5 * n + i] = kv.first.second[i];
+      stats[3 * n] = (double)kv.second.calls;
+      stats[3 * n + 1] = kv.second.ms;
+      stats[3 * n + 2] = kv.second.flops;
+    }
+    ++n;
+  }
+  return n;
 }
 
 int mpdo_debug_prof_eig_log(int* out, int cap) {

@@ -28,6 +28,11 @@ hipError_t block_diag_sum64(const double2* M, int A, int R, double2* out, long l
 hipError_t mask_cols_b(const double2* V, long long ldv, long long bsV, int rows, int cols, const int* k, float2* out,
                        long long ldo, long long bsO, int batch, hipStream_t s);
 hipError_t widen(const float2* in, double2* out, long long n, long long bsI, long long bsO, int batch, hipStream_t s);
+// C C^H of the two-site core C = G(Y) (channel2q layout) from Pg = Ycat Ycat^H (4 P1 x 4 P1 per
+// batch entry) and the channel tensor: out (2 P1 x 2 P1, complex128)
+hipError_t kraus_gram(const double2* Pg, const float2* Gt, int K, int P1, double2* out, int batch, hipStream_t s);
+// out[b] = sum over t < S of in[b S + t] (n complex128 each)
+hipError_t sum_parts(const double2* in, int S, long long n, double2* out, int batch, hipStream_t s);
 hipError_t eye_b(float2* out, int rows, int cols, long long bs, int batch, hipStream_t s);
 hipError_t factor_b(const double2* V, long long ldv, long long bsV, const double* lam, long long bsL, int rows,
                     int nev, int k, double power, double floor_rel, int trans, float2* out, long long ldo,

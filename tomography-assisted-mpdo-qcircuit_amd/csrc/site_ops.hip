@@ -188,6 +188,74 @@ hipError_t widen(const float2* in, double2* out, long long n, long long bsI, lon
   return hipGetLastError();
 }
 
+// Gram of the two-site core C = G(Y) without forming C's Kraus columns. With
+// C[(p, o1), (o2, k, q)] = sum_i G[(o1 o2), i, k] Y[i][p][q] (i = input pair (s1 s2)),
+// C C^H [(p,o1),(p',o1')] = sum_{i,i'} E[o1, i, o1', i'] Pg[(i, p), (i', p')] with
+// E = sum_{o2,k} G[(o1 o2), i, k] conj(G[(o1' o2), i', k]) and Pg = Ycat Ycat^H (Ycat the four
+// P1 x P2 blocks of Y stacked, 4 P1 x 4 P1, complex128). Gt: 16 K complex64, [(o1 o2)][i][k].
+__global__ void kraus_gram_kernel(const double2* Pg, const float2* Gt, int K, int P1, double2* out, long long bsP,
+                                  long long bsO) {
+  __shared__ double2 E[64];   // [o1][i][o1'][i']
+  Pg += blockIdx.y * bsP;
+  out += blockIdx.y * bsO;
+  if (threadIdx.x < 64) {
+    const int o1 = threadIdx.x >> 5, i = (threadIdx.x >> 3) & 3, o1p = (threadIdx.x >> 2) & 1, ip = threadIdx.x & 3;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int o2 = 0; o2 < 2; ++o2)
+      for (int k = 0; k < K; ++k) {
+        const float2 g = Gt[((o1 * 2 + o2) * 4 + i) * K + k];
+        const float2 h = Gt[((o1p * 2 + o2) * 4 + ip) * K + k];
+        acc.x += (double)g.x * h.x + (double)g.y * h.y;
+        acc.y += (double)g.y * h.x - (double)g.x * h.y;
+      }
+    E[threadIdx.x] = acc;
+  }
+  __syncthreads();
+  const long long n = 2LL * P1, ldp = 4LL * P1;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n * n; e += (long long)gridDim.x * blockDim.x) {
+    const int r = (int)(e / n), c = (int)(e % n);
+    const int p = r >> 1, o1 = r & 1, pp = c >> 1, o1p = c & 1;
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int ip = 0; ip < 4; ++ip) {
+        const double2 w = E[((o1 * 4 + i) * 2 + o1p) * 4 + ip];
+        const double2 v = Pg[((long long)i * P1 + p) * ldp + (long long)ip * P1 + pp];
+        acc.x += w.x * v.x - w.y * v.y;
+        acc.y += w.x * v.y + w.y * v.x;
+      }
+    out[e] = acc;
+  }
+}
+
+hipError_t kraus_gram(const double2* Pg, const float2* Gt, int K, int P1, double2* out, int batch, hipStream_t s) {
+  const long long n = 2LL * P1;
+  hipLaunchKernelGGL(kraus_gram_kernel, dim3(grid1(n * n), batch), dim3(256), 0, s, Pg, Gt, K, P1, out,
+                     16LL * P1 * P1, n * n);
+  return hipGetLastError();
+}
+
+// out[b][e] = sum_s in[b S + s][e], e < n (partial products of a split reduction)
+__global__ void sum_parts_kernel(const double2* in, int S, long long n, double2* out) {
+  const int b = blockIdx.y;
+  const double2* src = in + (long long)b * S * n;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int t = 0; t < S; ++t) {
+      const double2 v = src[(long long)t * n + e];
+      acc.x += v.x;
+      acc.y += v.y;
+    }
+    out[(long long)b * n + e] = acc;
+  }
+}
+
+hipError_t sum_parts(const double2* in, int S, long long n, double2* out, int batch, hipStream_t s) {
+  hipLaunchKernelGGL(sum_parts_kernel, dim3(grid1(n), batch), dim3(256), 0, s, in, S, n, out);
+  return hipGetLastError();
+}
+
 // batched identity
 __global__ void eye_b_kernel(float2* out, int rows, int cols, long long bs) {
   out += blockIdx.y * bs;

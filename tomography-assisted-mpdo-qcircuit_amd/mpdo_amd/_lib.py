@@ -69,7 +69,9 @@ def load() -> ctypes.CDLL:
         'mpdo_debug_layer_dry_run': (I, [I, I, SP, I, I, I, I, I, SP, P(ctypes.c_ulonglong)]),
         'mpdo_debug_prof_enable': (I, [I]),
         'mpdo_debug_layer_kept': (I, [P(I), I]),
+        'mpdo_debug_layer_gemm_log': (I, [P(I), I]),
         'mpdo_debug_prof_eig_log': (I, [P(I), I]),
+        'mpdo_debug_prof_shapes': (I, [I, P(I), P(D), I]),
         'mpdo_debug_prof_read': (I, [I, P(ctypes.c_longlong), P(D), P(D), P(D)]),
     }
     for name, (res, args) in sig.items():
@@ -474,12 +476,32 @@ def prof_read() -> dict:
     return out
 
 
+def layer_gemm_log() -> List[Tuple[int, int, int, int, int, int]]:
+    """(source line, M, N, K, batch, precision) of every GEMM of the last fused-layer dry run."""
+    lib = load()
+    n = lib.mpdo_debug_layer_gemm_log(None, 0)
+    buf = (ctypes.c_int * max(6 * n, 1))()
+    lib.mpdo_debug_layer_gemm_log(buf, 6 * n)
+    return [tuple(buf[6 * i:6 * i + 6]) for i in range(n)]
+
+
 def layer_kept() -> List[int]:
     """Per pair of the last fused layer on this thread: the largest kept count of its gate-time
     split over the chains."""
     buf = (ctypes.c_int * 1024)()
     n = load().mpdo_debug_layer_kept(buf, 1024)
     return [int(buf[i]) for i in range(min(n, 1024))]
+
+
+def prof_shapes(cls: str = 'gemm') -> List[Tuple[Tuple[int, ...], int, float, float]]:
+    """Per-shape totals of one timing class: ((M, N, K, batch, prec), launches, ms, flops)."""
+    lib = load()
+    c = PROF_CLASSES.index(cls)
+    n = lib.mpdo_debug_prof_shapes(c, None, None, 0)
+    sh = (ctypes.c_int * max(5 * n, 1))()
+    st = (ctypes.c_double * max(3 * n, 1))()
+    n = lib.mpdo_debug_prof_shapes(c, sh, st, n)
+    return [(tuple(sh[5 * i:5 * i + 5]), int(st[3 * i]), st[3 * i + 1], st[3 * i + 2]) for i in range(n)]
 
 
 def prof_eig_log() -> List[Tuple[int, int, int]]:

@@ -184,13 +184,17 @@ def main():
         c[1] += sw_e
     print('# blocked eigensolver calls (order: calls, mean sweeps): ' +
           ', '.join(f'{k}: {v[0]}, {v[1] / v[0]:.1f}' for k, v in sorted(eigs.items())), file=sys.stderr, flush=True)
+    shapes = sorted(_lib.prof_shapes('gemm'), key=lambda r: -r[2])
+    print('# GEMM shapes by time (M, N, K, batch, prec: launches, ms, TFLOP/s): ' +
+          '; '.join(f'{sh}: {c}, {t:.0f}, {f / (t * 1e-3) / 1e12:.1f}' for sh, c, t, f in shapes[:14]),
+          file=sys.stderr, flush=True)
 
     # dominant kernel class by summed launch time among those with algorithmic flops
     name, (n_l, ms, fl, by) = max(((k, v) for k, v in prof.items() if v[2] > 0), key=lambda kv: kv[1][1])
     prec = 'fp64'
     achieved = fl / (ms * 1e-3) / 1e12
     kernels = {'gemm': 'mpdo::gemm_kernel (complex GEMM, fp64 MFMA accumulate)',
-               'eig_rotate': 'mpdo::bj_rotate (block-Jacobi rotation update, fp64)',
+               'eig_rotate': 'mpdo::bj_sym + mpdo::bj_rotate_v (block-Jacobi rotation updates, fp64 MFMA)',
                'eig_solve': 'mpdo::bj_solve (block-Jacobi 64x64 subproblem, fp64)'}
     roofline = {'bound': 'mfma', 'achieved': round(achieved, 3), 'peak': PEAK_TFLOPS[prec], 'unit': 'TFLOP/s',
                 'frac': round(achieved / PEAK_TFLOPS[prec], 4), 'traffic': None,

@@ -39,6 +39,11 @@ mpdo_status mpdo_debug_eigh_gram(int n, int batch, void* A, double* w, void* V, 
 mpdo_status mpdo_debug_layer_dry_run(int batch, int n, const mpdo_site* in, int K1, int parity, int K2, int chi,
                                      int kappa, const mpdo_site* out, unsigned long long* workspace_bytes);
 
+/* Tile shape of the fp64-accumulating GEMMs for tuning sweeps: 0 = 64x64x16 (default),
+ * 1 = 64x64x32, 2 = 128x64x16, 3 = 64x128x16, 4 = 128x128x16; returns the previous choice
+ * (an out-of-range v only reads it). */
+int mpdo_debug_gemm_tiles(int v);
+
 /* The GEMMs the last dry run of the fused layer would launch, in order: out [6 n] = source line,
  * M, N, K, batch, precision mode; returns n (fills at most cap ints). */
 int mpdo_debug_layer_gemm_log(int* out, int cap);

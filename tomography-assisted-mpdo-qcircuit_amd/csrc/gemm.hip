@@ -184,6 +184,21 @@ static hipError_t launch(const GemmArgs& g, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Tile shape of the fp64-accumulating GEMMs (BM x BN x BK): 0 = 64x64x16, 1 = 64x64x32,
+// 2 = 128x64x16, 3 = 64x128x16, 4 = 128x128x16.
+int g_tile_variant = 0;
+
+template <typename TIn, typename TOut>
+static hipError_t launch_f64(const GemmArgs& g, hipStream_t s) {
+  switch (g_tile_variant) {
+    case 1: return launch<TIn, double, TOut, 64, 64, 32>(g, s);
+    case 2: return launch<TIn, double, TOut, 128, 64, 16>(g, s);
+    case 3: return launch<TIn, double, TOut, 64, 128, 16>(g, s);
+    case 4: return launch<TIn, double, TOut, 128, 128, 16>(g, s);
+    default: return launch<TIn, double, TOut, 64, 64, 16>(g, s);
+  }
+}
+
 hipError_t gemm(Prec p, const GemmArgs& g, hipStream_t s) {
   if (g.K <= 0) {
     // C = beta*C (degenerate reduction): handled by a K=1 zero-operand pass is not needed
@@ -201,13 +216,23 @@ hipError_t gemm(Prec p, const GemmArgs& g, hipStream_t s) {
                  shape);
   switch (p) {
     case C64_F32ACC: return launch<float2, float, float2, 128, 128, 16>(g, s);
-    case C64_F64ACC: return launch<float2, double, double2, 64, 64, 16>(g, s);
-    case C128_F64ACC: return launch<double2, double, double2, 64, 64, 16>(g, s);
-    case C128_IN_C64_OUT: return launch<double2, double, float2, 64, 64, 16>(g, s);
-    case C64_F64ACC_C64OUT: return launch<float2, double, float2, 64, 64, 16>(g, s);
+    case C64_F64ACC: return launch_f64<float2, double2>(g, s);
+    case C128_F64ACC: return launch_f64<double2, double2>(g, s);
+    case C128_IN_C64_OUT: return launch_f64<double2, float2>(g, s);
+    case C64_F64ACC_C64OUT: return launch_f64<float2, float2>(g, s);
   }
   return hipErrorInvalidValue;
 }
+
+}  // namespace mpdo
+
+extern "C" int mpdo_debug_gemm_tiles(int v) {
+  const int old = mpdo::g_tile_variant;
+  if (v >= 0 && v <= 4) mpdo::g_tile_variant = v;
+  return old;
+}
+
+namespace mpdo {
 
 GemmArgs gemm_rowmajor(int M, int N, int K,
                        const void* A, long long lda, bool transA, bool conjA,

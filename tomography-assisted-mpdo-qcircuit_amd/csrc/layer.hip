@@ -237,6 +237,27 @@ hipError_t k_kraus_gram(const double2* Pg, const float2* Gt, int K, int P1, doub
   return kraus_gram(Pg, Gt, K, P1, out, batch, s);
 }
 
+hipError_t k_kappa_gram(const double2* Hg, const float2* Gt, int K, int A, double2* out, int batch, hipStream_t s,
+                        int line) {
+  const long long n = (long long)A * K;
+  if (!fits(Hg, batch * 64LL * A * A, sizeof(double2), "kappa core Gram input", line) ||
+      !fits(Gt, 16LL * K, sizeof(float2), "kappa core channel", line) ||
+      !fits(out, batch * n * n, sizeof(double2), "kappa core Gram output", line))
+    return hipErrorInvalidValue;
+  if (t_dry) return hipSuccess;
+  return kappa_core_gram(Hg, Gt, K, A, out, batch, s);
+}
+
+hipError_t k_kappa_vec(const float2* Vk, const float2* Gt, int K, int A, int kk, float2* Vt, int batch, hipStream_t s,
+                       int line) {
+  if (!fits(Vk, (long long)batch * A * K * kk, sizeof(float2), "kappa core vectors", line) ||
+      !fits(Gt, 16LL * K, sizeof(float2), "kappa core channel", line) ||
+      !fits(Vt, (long long)batch * 16 * A * kk, sizeof(float2), "kappa core mapped vectors", line))
+    return hipErrorInvalidValue;
+  if (t_dry) return hipSuccess;
+  return kappa_core_vec(Vk, Gt, K, A, kk, Vt, batch, s);
+}
+
 hipError_t k_copy(void* dst, const void* src, size_t bytes, hipStream_t s, int line) {
   if (!fits(dst, (long long)bytes, 1, "copy destination", line) || !fits(src, (long long)bytes, 1, "copy source", line))
     return hipErrorInvalidValue;
@@ -638,7 +659,7 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     }
   // sizes (elements per chain) of the shared scratch buffers: each the maximum over its uses
   auto kapplies = [&](int i) { return P.out[i].a != P.chi_out[i].a; };
-  size_t mxPg = 1;
+  size_t mxPg = 1, mxZm = 1, mxV = 1, mxHk = 1, mxVt = 1;
   size_t mxQR_rows = 1, mxQR_cols = 1, mxR = 1, mxY = 1, mxOm = 1, mxZ = 1, mxQ2P = 1, mxC = 1;
   size_t mxTmin = 1, mxWm = 1, mxTm = 1, mxFL = 1, mxPn = 1, mxRT = 1, mxX1 = 1, mxLast = 1, mxTcur = 1;
   size_t mxTchi = 1, mxMp = 1, mxKout = 1, mxX1t = 1, mxQ1t = 1, mxR1t = 1;
@@ -703,6 +724,12 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     mxTcur = std::max(mxTcur, (size_t)ps.kprev * 2 * P.a1[ps.i].a * kp);
     mxWm = std::max(mxWm, (size_t)kp * 2 * K2 * ps.p2);
     mxTm = std::max(mxTm, (size_t)kp * 2 * K2 * c3);
+    if (kapplies(ps.i + 1) && 2 * K2 > 8) {   // Kraus-factored kappa cut of the max site
+      mxZm = std::max(mxZm, (size_t)kp * 2 * ps.p1t);
+      mxV = std::max(mxV, (size_t)kp * 8 * ps.p2);
+      mxHk = std::max(mxHk, (size_t)64 * a2 * a2);
+      mxVt = std::max(mxVt, (size_t)16 * a2 * P.out[ps.i + 1].a);
+    }
     mxPn = std::max(mxPn, (size_t)P.in[ps.i + 1].r * std::max(kr, P.in[ps.i + 1].r));
   }
   const int mxEig = std::max({mxEigQR, mxEigC, mxEigChi, mxEigK, mxEigR});
@@ -720,7 +747,8 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
   Site1q* dtab = nullptr;
   float2 *Mlast = nullptr, *FLq, *Rcur, *Rnext, *RT, *X1, *X1t, *Q1t, *R1t, *Cs, *Cp, *Cu, *Zb, *Q2P, *FL, *FR, *Tmin, *Wm,
          *Tm, *Pn, *Pn2, *PW, *Pp, *Tchi, *Mp, *Vk, *Kout, *dG, *Tcur2, *dK;
-  double2 *G, *GR, *Om64, *Q2w, *Z64, *H64, *Cw64, *Z64c, *Gparts, *Pg;
+  double2 *G, *GR, *Om64, *Q2w, *Z64, *H64, *Cw64, *Z64c, *Gparts, *Pg, *Hk;
+  float2 *Zm, *Vf, *Vt;
   double* ddisc;
   int* dk;
   EigB eE{};
@@ -791,6 +819,10 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     G = c.get<double2>((size_t)mxEig * mxEig * B);
     Gparts = c.get<double2>((size_t)SPLIT_S * SPLIT_N * SPLIT_N * B);
     Pg = c.get<double2>(mxPg * B);
+    Hk = c.get<double2>(mxHk * B);
+    Zm = c.get<float2>(mxZm * B);
+    Vf = c.get<float2>(mxV * B);
+    Vt = c.get<float2>(mxVt * B);
     GR = c.get<double2>((size_t)mxEig * mxEig * B);
     carve_eig(c, mxEig, B, eE);
     carve_qr(c, (int)mxQR_rows, (int)mxQR_cols, B, qw);
@@ -1060,6 +1092,84 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     LCK(K_PERMUTE(Kout, (float2*)out[i].data, 4, d2, p2, s, B, (long long)rows * k));
     return MPDO_OK;
   };
+  // Kappa cut of a pair's new max site without forming it. The site is
+  //   T = S^{-1/2} U^H Rc (I (x) Q2P),  Rc = Rs C,  C[(p, o1), (s2, K2, q)] = sum_i G[(o1 s2), i, K2] Y[i][p][q]
+  // with Rs the map from C's rows to the core's (Rz Um^H, or Ra (x) I without the gate-time cut).
+  // With Zm = S^{-1/2} U^H Rs (kp x 2 p1t), V[k, (o1 i), q] = sum_p Zm[k, (p o1)] Y[i][p][q] and
+  // W = V Q2P [(k, o1 i) x (a2 kr)], T[k, s2, K2, a2, kr] = sum_{o1 i} G[(o1 s2), i, K2] W[k, (o1 i), a2, kr]:
+  // the inner index (a2, K2) of T is the image of (o1 i, a2) (8 a2 directions instead of
+  // 2 K2 a2), so T's Gram over the inner index comes from the 8 a2 x 8 a2 Gram of W and the
+  // channel (kappa_core_gram), and the kept directions act on W directly (kappa_core_vec).
+  // FR (kp x nC) is S^{-1/2} U^H of the chi step.
+  auto kappa_factored = [&](PairStore& ps, int j, int gidx, int kp, int kr, int a2, const float2* Qright) -> mpdo_status {
+    const int p1t = ps.p1t, nT = 2 * p1t, nC = ps.nc, k = P.out[j].a, a = a2 * K2;
+    const long long bsFR = (long long)kp * nC, bsZm = (long long)kp * nT;
+    const float2* Gt = dG + (size_t)gidx * 16 * K2;
+    // Zm = FR Rs
+    if (ps.has_P) {
+      // FR Rz (kp x mq), then (FR Rz) Um^H
+      LCK(mmb(g_tensor_prec, kp, ps.mq, ps.pz, FR, nC, bsFR, ps.Rz, ps.mq, (long long)ps.pz * ps.mq, Wm, ps.mq,
+              (long long)kp * ps.mq, B, s));
+      GemmArgs g{};
+      g.M = kp; g.N = nT; g.K = ps.mq; g.batch = B;
+      g.A = Wm; g.rsA = ps.mq; g.csA = 1; g.bsA = (long long)kp * ps.mq;
+      g.B = ps.Um; g.rsB = 1; g.csB = ps.mq; g.bsB = (long long)nT * ps.mq; g.conjB = 1;
+      g.C = Zm; g.rsC = nT; g.csC = 1; g.bsC = bsZm;
+      g.alpha_re = 1;
+      LCK(K_GEMM(g_tensor_prec, g, s));
+    } else {
+      // Zm[k, (p', o1)] = sum_p FR[k, (p, o1)] Ra[p, p']
+      for (int o1 = 0; o1 < 2; ++o1) {
+        GemmArgs g{};
+        g.M = kp; g.N = p1t; g.K = ps.p1; g.batch = B;
+        g.A = FR + o1; g.rsA = nC; g.csA = 2; g.bsA = bsFR;
+        g.B = ps.Ra; g.rsB = p1t; g.csB = 1; g.bsB = (long long)ps.p1 * p1t;
+        g.C = Zm + o1; g.rsC = nT; g.csC = 2; g.bsC = bsZm;
+        g.alpha_re = 1;
+        LCK(K_GEMM(g_tensor_prec, g, s));
+      }
+    }
+    // V[k, (o1 i), q] = sum_p Zm[k, (p o1)] Y[i][p][q]
+    const long long bsV = (long long)kp * 8 * ps.p2;
+    for (int o1 = 0; o1 < 2; ++o1)
+      for (int i = 0; i < 4; ++i) {
+        GemmArgs g{};
+        g.M = kp; g.N = ps.p2; g.K = p1t; g.batch = B;
+        g.A = Zm + o1; g.rsA = nT; g.csA = 2; g.bsA = bsZm;
+        g.B = ps.Y + (long long)i * p1t * ps.p2; g.rsB = ps.p2; g.csB = 1; g.bsB = 4LL * p1t * ps.p2;
+        g.C = Vf + (long long)(o1 * 4 + i) * ps.p2; g.rsC = 8LL * ps.p2; g.csC = 1; g.bsC = bsV;
+        g.alpha_re = 1;
+        LCK(K_GEMM(g_tensor_prec, g, s));
+      }
+    // W = V Q2P: [(k, o1 i) x (a2 kr)] in Tm, then rows (o1 i, a2) over columns (k, kr) in Mp
+    const long long c3 = (long long)a2 * kr, bsW = 8LL * kp * c3;
+    LCK(mmb(g_tensor_prec, 8 * kp, (int)c3, ps.p2, Vf, ps.p2, bsV, Qright, c3, (long long)ps.p2 * c3, Tm, c3, bsW, B,
+            s));
+    {
+      int dims[4] = {kp, 8, a2, kr}, perm[4] = {1, 2, 0, 3};
+      LCK(K_PERMUTE(Tm, Mp, 4, dims, perm, s, B, bsW));
+    }
+    LCK(gram_rows_b(Mp, 8 * a2, kp * kr, bsW, Hk, B, s));
+    LCK(k_kappa_gram(Hk, Gt, K2, a2, G, B, s, __LINE__));
+    LST(eig_b(a, B, G, eE, s, true));
+    LCK(K_KEEP(eE.w, a, a, (int)std::min(2LL * kp * kr, (long long)a), k, -1.0, 1, dk, ddisc + (size_t)B * (nsteps + j), B,
+               s));
+    LCK(K_FACTOR(eE.V, a, (long long)a * a, eE.w, a, a, a, k, 0.0, 0.0, 0, Vk, k, (long long)a * k, B, s));
+    LCK(k_kappa_vec(Vk, Gt, K2, a2, k, Vt, B, s, __LINE__));
+    // out2[(s2 j), (k' kr)] = sum_{(o1 i a2)} Vt[(o1 i a2), (s2 j)] W[(o1 i a2), (k' kr)]
+    {
+      GemmArgs g{};
+      g.M = 2 * k; g.N = kp * kr; g.K = 8 * a2; g.batch = B;
+      g.A = Vt; g.rsA = 1; g.csA = 2 * k; g.bsA = 16LL * a2 * k;
+      g.B = Mp; g.rsB = (long long)kp * kr; g.csB = 1; g.bsB = bsW;
+      g.C = Kout; g.rsC = (long long)kp * kr; g.csC = 1; g.bsC = 2LL * k * kp * kr;
+      g.alpha_re = 1;
+      LCK(K_GEMM(g_tensor_prec, g, s));
+    }
+    int dims[4] = {2, k, kp, kr}, perm[4] = {2, 0, 1, 3};
+    LCK(K_PERMUTE(Kout, (float2*)out[j].data, 4, dims, perm, s, B, 2LL * k * kp * kr));
+    return MPDO_OK;
+  };
   // where the chi step writes site i: straight into the output when no kappa cut follows
   auto chi_target = [&](int i) -> float2* { return kapplies(i) ? Tchi : (float2*)out[i].data; };
   // kappa cut of site i held as T = [l][2][a][r] in chi_target(i)
@@ -1192,19 +1302,23 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     }
     // new max site: S^{-1/2} U^H Rc (I (x) Q2P) -> (kp, s2, K2, a2, kr) -> (kp, s2, a2, K2, kr)
     LCK(K_FACTOR(eE.V, nC, (long long)nC * nC, eE.w, nC, nC, nC, kp, -0.25, SPLIT_FLOOR, 1, FR, nC, (long long)kp * nC, B, s));
-    LCK(mmb(g_tensor_prec, kp, (int)colsR, nC, FR, nC, (long long)kp * nC, Cu, colsR, szC, Wm, colsR,
-            kp * colsR, B, s));
-    LCK(mmb(g_tensor_prec, kp * 2 * K2, c3, ps.p2, Wm, ps.p2, kp * colsR, Qright, c3, (long long)ps.p2 * c3, Tm, c3,
-            (long long)kp * 2 * K2 * c3, B, s));
-    const float2* Ts = Tm;
-    const long long bsT = (long long)kp * 2 * K2 * c3;
-    if (kapplies(j)) {   // (kp, s2, K2, a2, kr) -> (kp, s2, kr, a2, K2): inner index last for the kappa cut
-      int dims[5] = {kp, 2, K2, a2, kr}, perm[5] = {0, 1, 4, 3, 2};
-      LCK(K_PERMUTE(Ts, Mp, 5, dims, perm, s, B, bsT));
-      LST(kappa_on(j, Mp));
-    } else {             // (kp, s2, K2, a2, kr) -> (kp, s2, a2, K2, kr)
-      int dims[5] = {kp, 2, K2, a2, kr}, perm[5] = {0, 1, 3, 2, 4};
-      LCK(K_PERMUTE(Ts, (float2*)out[j].data, 5, dims, perm, s, B, bsT));
+    if (kapplies(j) && 2 * K2 > 8) {
+      LST(kappa_factored(ps, j, pidx + 1, kp, kr, a2, Qright));
+    } else {
+      LCK(mmb(g_tensor_prec, kp, (int)colsR, nC, FR, nC, (long long)kp * nC, Cu, colsR, szC, Wm, colsR,
+              kp * colsR, B, s));
+      LCK(mmb(g_tensor_prec, kp * 2 * K2, c3, ps.p2, Wm, ps.p2, kp * colsR, Qright, c3, (long long)ps.p2 * c3, Tm,
+              c3, (long long)kp * 2 * K2 * c3, B, s));
+      const float2* Ts = Tm;
+      const long long bsT = (long long)kp * 2 * K2 * c3;
+      if (kapplies(j)) {   // (kp, s2, K2, a2, kr) -> (kp, s2, kr, a2, K2): inner index last for the kappa cut
+        int dims[5] = {kp, 2, K2, a2, kr}, perm[5] = {0, 1, 4, 3, 2};
+        LCK(K_PERMUTE(Ts, Mp, 5, dims, perm, s, B, bsT));
+        LST(kappa_on(j, Mp));
+      } else {             // (kp, s2, K2, a2, kr) -> (kp, s2, a2, K2, kr)
+        int dims[5] = {kp, 2, K2, a2, kr}, perm[5] = {0, 1, 3, 2, 4};
+        LCK(K_PERMUTE(Ts, (float2*)out[j].data, 5, dims, perm, s, B, bsT));
+      }
     }
     // step at bond (i-1 | i) on the new min-site tensor
     if (i == 0) {

@@ -31,6 +31,12 @@ hipError_t widen(const float2* in, double2* out, long long n, long long bsI, lon
 // C C^H of the two-site core C = G(Y) (channel2q layout) from Pg = Ycat Ycat^H (4 P1 x 4 P1 per
 // batch entry) and the channel tensor: out (2 P1 x 2 P1, complex128)
 hipError_t kraus_gram(const double2* Pg, const float2* Gt, int K, int P1, double2* out, int batch, hipStream_t s);
+// Kappa cut of a Kraus-factored max site T[k,s2,K2,a2,r] = sum_oi g[s2][oi][K2] W[k,oi,a2,r]:
+// the inner Gram (A K x A K) from Hg = Wp Wp^H (8 A x 8 A, rows (oi, a2)), and the kept
+// directions Vk (A K x kk) mapped to Vt[(oi a2), (s2 j)] (8 A x 2 kk)
+hipError_t kappa_core_gram(const double2* Hg, const float2* Gt, int K, int A, double2* out, int batch, hipStream_t s);
+hipError_t kappa_core_vec(const float2* Vk, const float2* Gt, int K, int A, int kk, float2* Vt, int batch,
+                          hipStream_t s);
 // out[b] = sum over t < S of in[b S + t] (n complex128 each)
 hipError_t sum_parts(const double2* in, int S, long long n, double2* out, int batch, hipStream_t s);
 hipError_t eye_b(float2* out, int rows, int cols, long long bs, int batch, hipStream_t s);

@@ -236,6 +236,83 @@ hipError_t kraus_gram(const double2* Pg, const float2* Gt, int K, int P1, double
   return hipGetLastError();
 }
 
+// Kappa Gram of a new max site held in Kraus-factored form. The site is
+// T[k, s2, K2, a2, r] = sum_{oi} g[s2][oi][K2] W[k, oi, a2, r] with oi = (o1, i), i the pair's
+// input physical index, g[s2][(o1,i)][K2] = Gt[((o1 2 + s2) 4 + i) K + K2]; given
+// Hg = Wp Wp^H (Wp = W as rows (oi, a2) over columns (k, r); 8 A x 8 A, complex128), its Gram over
+// the inner index (a2, K2) is
+// out[(a2 K2), (a2' K2')] = sum_{s2, oi, oi'} conj(g[s2][oi][K2]) g[s2][oi'][K2'] conj(Hg[(oi a2), (oi' a2')]).
+__global__ void kappa_core_gram_kernel(const double2* Hg, const float2* Gt, int K, int A, double2* out, long long bsH,
+                                       long long bsO) {
+  extern __shared__ double2 gs[];   // [s2][oi][K2]
+  Hg += blockIdx.y * bsH;
+  out += blockIdx.y * bsO;
+  for (int t = threadIdx.x; t < 16 * K; t += blockDim.x) {
+    const int s2 = t / (8 * K), oi = (t / K) % 8, k2 = t % K;
+    const int o1 = oi >> 2, i = oi & 3;
+    const float2 v = Gt[((o1 * 2 + s2) * 4 + i) * K + k2];
+    gs[t] = make_double2(v.x, v.y);
+  }
+  __syncthreads();
+  const long long n = (long long)A * K, ldh = 8LL * A;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n * n; e += (long long)gridDim.x * blockDim.x) {
+    const int r = (int)(e / n), c = (int)(e % n);
+    const int a2 = r / K, k2 = r % K, a2p = c / K, k2p = c % K;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int s2 = 0; s2 < 2; ++s2)
+      for (int oi = 0; oi < 8; ++oi) {
+        const double2 g1 = gs[(s2 * 8 + oi) * K + k2];
+        double2 inner = make_double2(0.0, 0.0);   // sum_oi' g[s2][oi'][K2'] conj(Hg[(oi a2), (oi' a2')])
+        for (int oj = 0; oj < 8; ++oj) {
+          const double2 g2 = gs[(s2 * 8 + oj) * K + k2p];
+          const double2 h = Hg[((long long)oi * A + a2) * ldh + (long long)oj * A + a2p];
+          inner.x += g2.x * h.x + g2.y * h.y;
+          inner.y += g2.y * h.x - g2.x * h.y;
+        }
+        acc.x += g1.x * inner.x + g1.y * inner.y;   // conj(g1) * inner
+        acc.y += g1.x * inner.y - g1.y * inner.x;
+      }
+    out[e] = acc;
+  }
+}
+
+hipError_t kappa_core_gram(const double2* Hg, const float2* Gt, int K, int A, double2* out, int batch, hipStream_t s) {
+  const long long n = (long long)A * K;
+  hipLaunchKernelGGL(kappa_core_gram_kernel, dim3(grid1(n * n), batch), dim3(256), sizeof(double2) * 16 * K, s, Hg,
+                     Gt, K, A, out, 64LL * A * A, n * n);
+  return hipGetLastError();
+}
+
+// The kept inner directions mapped through the Kraus factorisation:
+// Vt[(oi a2), (s2 j)] = sum_K2 g[s2][oi][K2] Vk[(a2 K2), j]  (Vk: A K x kk, Vt: 8 A x 2 kk, complex64)
+__global__ void kappa_core_vec_kernel(const float2* Vk, const float2* Gt, int K, int A, int kk, float2* Vt, long long bsV,
+                                      long long bsT) {
+  Vk += blockIdx.y * bsV;
+  Vt += blockIdx.y * bsT;
+  const long long tot = 8LL * A * 2 * kk;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot; e += (long long)gridDim.x * blockDim.x) {
+    const int row = (int)(e / (2 * kk)), col = (int)(e % (2 * kk));
+    const int oi = row / A, a2 = row % A, s2 = col / kk, j = col % kk;
+    const int o1 = oi >> 2, i = oi & 3;
+    double ax = 0.0, ay = 0.0;
+    for (int k2 = 0; k2 < K; ++k2) {
+      const float2 g = Gt[((o1 * 2 + s2) * 4 + i) * K + k2];
+      const float2 v = Vk[((long long)a2 * K + k2) * kk + j];
+      ax += (double)g.x * v.x - (double)g.y * v.y;
+      ay += (double)g.x * v.y + (double)g.y * v.x;
+    }
+    Vt[e] = make_float2((float)ax, (float)ay);
+  }
+}
+
+hipError_t kappa_core_vec(const float2* Vk, const float2* Gt, int K, int A, int kk, float2* Vt, int batch,
+                          hipStream_t s) {
+  const long long tot = 8LL * A * 2 * kk;
+  hipLaunchKernelGGL(kappa_core_vec_kernel, dim3(grid1(tot), batch), dim3(256), 0, s, Vk, Gt, K, A, kk, Vt,
+                     (long long)A * K * kk, tot);
+  return hipGetLastError();
+}
+
 // out[b][e] = sum_s in[b S + s][e], e < n (partial products of a split reduction)
 __global__ void sum_parts_kernel(const double2* in, int S, long long n, double2* out) {
   const int b = blockIdx.y;

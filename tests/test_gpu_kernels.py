@@ -117,4 +117,6 @@ def test_eigh_gram(cuda, n, rank):
         zero = w[b] == 0
         assert np.all(V[b][:, zero] == 0)
         assert np.all(w[b][rank:] <= 1e-12 * top)
+        # the eigenvalues below the pivot threshold stay as the Schur complement's diagonal
+        assert abs(w[b].sum() - np.trace(H[b]).real) <= 1e-12 * np.trace(H[b]).real
 

@@ -209,6 +209,31 @@ __global__ void scatter_x_kernel(const double2* Linv, const int* piv, const int*
   }
 }
 
+// w[b][rank_b + j] for j < n - rank_b: the diagonal left in the Schur complement of matrix b (the
+// rows never pivoted), clamped at 0 and sorted descending: the Gram mass below the pivot threshold
+// stays in the eigenvalue list. One 1024-thread block per matrix (n <= 1024), bitonic sort in LDS.
+__global__ __launch_bounds__(1024) void pchol_tail_kernel(const double* d, const unsigned char* used, const int* rank,
+                                                          int n, double* w, long long ldw) {
+  __shared__ double t[1024];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const double* db = d + (long long)b * n;
+  const unsigned char* ub = used + (long long)b * n;
+  t[tid] = (tid < n && !ub[tid]) ? fmax(db[tid], 0.0) : -1.0;
+  __syncthreads();
+  for (int k = 2; k <= 1024; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const int p = tid ^ j;
+      if (p > tid) {
+        const bool desc = (tid & k) == 0;
+        const double x = t[tid], y = t[p];
+        if (desc ? (x < y) : (x > y)) { t[tid] = y; t[p] = x; }
+      }
+      __syncthreads();
+    }
+  const int rb = rank[b];
+  if (tid < n - rb) w[(long long)b * ldw + rb + tid] = t[tid];
+}
+
 int grid_for(long long n) { return (int)std::max<long long>(1, std::min<long long>((n + 255) / 256, 2048)); }
 
 }  // namespace
@@ -258,6 +283,16 @@ hipError_t pchol_b(int n, int batch, void* A, long long lda, long long bsA, doub
     }
   }
   return hipSuccess;
+}
+
+hipError_t pchol_tail_eigs(int n, int batch, const void* ws, const int* rank, double* w, long long ldw, hipStream_t st) {
+  if (n <= 0 || batch <= 0) return hipSuccess;
+  if (n > 1024) return hipErrorInvalidValue;
+  const char* p = (const char*)ws;
+  const double* d = (const double*)p;
+  const unsigned char* used = (const unsigned char*)(p + a256((size_t)n * sizeof(double) * batch));
+  hipLaunchKernelGGL(pchol_tail_kernel, dim3(batch), dim3(1024), 0, st, d, used, rank, n, w, ldw);
+  return hipGetLastError();
 }
 
 int pinv_order(int n) {

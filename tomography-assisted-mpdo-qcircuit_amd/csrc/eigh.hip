@@ -590,7 +590,7 @@ size_t eigh_gram_workspace_bytes(int n, int batch) {
 }
 
 hipError_t eigh_gram(int n, int batch, void* A, long long lda, long long bsA, double* w, long long ldw, void* V,
-                     long long ldv, long long bsV, void* ws, hipStream_t st, int* sweeps_out) {
+                     long long ldv, long long bsV, void* ws, hipStream_t st, int* sweeps_out, double tol_rel) {
   if (n <= 0 || batch <= 0) return hipSuccess;
   if (n <= NS || n > PC_MAX) return eigh(n, batch, A, lda, bsA, w, ldw, V, ldv, bsV, ws, st, sweeps_out);
   hipError_t err;
@@ -605,10 +605,9 @@ hipError_t eigh_gram(int n, int batch, void* A, long long lda, long long bsA, do
   int* rank = (int*)p; p += align256(sizeof(int) * batch);
   void* pws = p; p += pchol_workspace_bytes(n, batch);
   void* ws_inner = p;
-  // pivots stop at 1e-14 of the largest diagonal: the Gram mass left out is at most
-  // (n - rank) * 1e-14 * max G_ii, the level of the fp64 rounding of the Schur complement
+  // eigenvectors are zeroed for eigenvalues of L^H L below 1e-14 of the largest (the Jacobi floor)
   const double tol = 1e-14;
-  if ((err = pchol_b(n, batch, A, lda, bsA, tol, L, piv, rank, pws, st)) != hipSuccess) return err;
+  if ((err = pchol_b(n, batch, A, lda, bsA, tol_rel, L, piv, rank, pws, st)) != hipSuccess) return err;
   std::vector<int> hr(batch);
   if ((err = hipMemcpyAsync(hr.data(), rank, sizeof(int) * batch, hipMemcpyDeviceToHost, st)) != hipSuccess)
     return err;
@@ -644,7 +643,8 @@ hipError_t eigh_gram(int n, int batch, void* A, long long lda, long long bsA, do
       hipSuccess)
     return err;
   hipLaunchKernelGGL(pad_eigs_kernel, dim3(batch), dim3(256), 0, st, wm, r, w, ldw, n);
-  return hipGetLastError();
+  if ((err = hipGetLastError()) != hipSuccess) return err;
+  return pchol_tail_eigs(n, batch, pws, rank, w, ldw, st);
 }
 
 }  // namespace mpdo

@@ -374,8 +374,8 @@ mpdo_status eig_b_(int line, int n, int batch, double2* G, EigB& e, hipStream_t 
     return layer_fail(MPDO_EINVAL, t_oob.c_str());
   }
   if (t_dry) return MPDO_OK;
-  if (gram && use_gram_eig())
-    LCK(eigh_gram(n, batch, G, n, (long long)n * n, e.w, n, e.V, n, (long long)n * n, e.ws, s, &sweeps));
+  if (gram && use_gram_eig())   // pivots down to 1e-12 of the largest diagonal; the rest stays as tail mass
+    LCK(eigh_gram(n, batch, G, n, (long long)n * n, e.w, n, e.V, n, (long long)n * n, e.ws, s, &sweeps, 1e-12));
   else
     LCK(eigh(n, batch, G, n, (long long)n * n, e.w, n, e.V, n, (long long)n * n, e.ws, s, &sweeps));
   if (sweeps > 60) {

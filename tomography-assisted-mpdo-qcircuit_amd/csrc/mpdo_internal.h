@@ -41,14 +41,15 @@ hipError_t eigh(int n, int batch, void* A, long long lda, long long bsA,
                 double* w, long long ldw, void* V, long long ldv, long long bsV,
                 void* ws, hipStream_t s, int* sweeps_out);
 
-// The same for positive semidefinite Gram matrices, preconditioned by pivoted Cholesky and
-// solved at their numerical rank: eigenpairs below the Jacobi floor (and past the rank) come
-// back as zero eigenvalues with zero vectors. Synchronises `s` once (the rank). `ws` must hold
-// eigh_gram_workspace_bytes(n, batch) bytes.
+// The same for positive semidefinite Gram matrices, preconditioned by pivoted Cholesky (pivots
+// down to tol_rel of the largest diagonal) and solved at that rank r: the first r eigenpairs come
+// from the Jacobi method on L^H L; the remaining n - r eigenvalues are the diagonal left in the
+// Schur complement (sorted, so the eigenvalue sum stays the trace) with zero vectors.
+// Synchronises `s` once (the rank). `ws` must hold eigh_gram_workspace_bytes(n, batch) bytes.
 size_t eigh_gram_workspace_bytes(int n, int batch);
 hipError_t eigh_gram(int n, int batch, void* A, long long lda, long long bsA,
                      double* w, long long ldw, void* V, long long ldv, long long bsV,
-                     void* ws, hipStream_t s, int* sweeps_out);
+                     void* ws, hipStream_t s, int* sweeps_out, double tol_rel = 1e-14);
 
 // Pivoted Cholesky of positive semidefinite Gram matrices A[b] (n x n, leading dim lda, batch
 // stride bsA; destroyed): L[b] (n x n, stored by columns: L[b][k n + i] = L_ik) with A = L L^H,
@@ -57,6 +58,9 @@ hipError_t eigh_gram(int n, int batch, void* A, long long lda, long long bsA,
 size_t pchol_workspace_bytes(int n, int batch);
 hipError_t pchol_b(int n, int batch, void* A, long long lda, long long bsA, double tol_rel, double2* L, int* piv,
                    int* rank, void* ws, hipStream_t s);
+// After pchol_b (same ws, rank): w[b][rank[b] + j], j < n - rank[b], receives the diagonal left in
+// matrix b's Schur complement, sorted descending (the Gram mass below the pivot threshold); n <= 1024.
+hipError_t pchol_tail_eigs(int n, int batch, const void* ws, const int* rank, double* w, long long ldw, hipStream_t s);
 // The Cholesky-QR factor of a pivoted factor, as complex128 X[b] (n x n): with Lt[t][u] =
 // L[piv[t]][u] (t, u < rank), mode 0 gives X[piv[t]][u] = conj(Lt^-1[u][t]) (Q = M X has
 // orthonormal columns when M^H M = L L^H and M = Q L^H), mode 1 X[u][piv[t]] = Lt^-1[u][t]

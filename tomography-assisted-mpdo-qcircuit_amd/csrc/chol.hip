@@ -234,6 +234,21 @@ __global__ __launch_bounds__(1024) void pchol_tail_kernel(const double* d, const
   if (tid < n - rb) w[(long long)b * ldw + rb + tid] = t[tid];
 }
 
+// cond2[b] = (L_00 / L_{r-1,r-1})^2 in pivot order (first over last pivot of matrix b), 1 when r <= 1
+__global__ void pchol_cond_kernel(const double2* L, const int* piv, const int* rank, int n, int batch, double* cond2) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  const int r = rank[b];
+  double c = 1.0;
+  if (r > 1) {
+    const double2* Lb = L + (long long)b * n * n;
+    const int* pb = piv + (long long)b * n;
+    const double d0 = Lb[pb[0]].x, dl = Lb[(long long)(r - 1) * n + pb[r - 1]].x;
+    c = dl > 0 ? (d0 / dl) * (d0 / dl) : 1e300;
+  }
+  cond2[b] = c;
+}
+
 int grid_for(long long n) { return (int)std::max<long long>(1, std::min<long long>((n + 255) / 256, 2048)); }
 
 }  // namespace
@@ -292,6 +307,13 @@ hipError_t pchol_tail_eigs(int n, int batch, const void* ws, const int* rank, do
   const double* d = (const double*)p;
   const unsigned char* used = (const unsigned char*)(p + a256((size_t)n * sizeof(double) * batch));
   hipLaunchKernelGGL(pchol_tail_kernel, dim3(batch), dim3(1024), 0, st, d, used, rank, n, w, ldw);
+  return hipGetLastError();
+}
+
+hipError_t pchol_cond2(int n, int batch, const double2* L, const int* piv, const int* rank, double* cond2,
+                       hipStream_t st) {
+  if (n <= 0 || batch <= 0) return hipSuccess;
+  hipLaunchKernelGGL(pchol_cond_kernel, dim3((batch + 255) / 256), dim3(256), 0, st, L, piv, rank, n, batch, cond2);
   return hipGetLastError();
 }
 

@@ -258,6 +258,14 @@ hipError_t k_kappa_vec(const float2* Vk, const float2* Gt, int K, int A, int kk,
   return kappa_core_vec(Vk, Gt, K, A, kk, Vt, batch, s);
 }
 
+hipError_t k_narrow_factor(const double2* L, int n, int mode, float2* out, int batch, hipStream_t s, int line) {
+  if (!fits(L, (long long)batch * n * n, sizeof(double2), "factor input", line) ||
+      !fits(out, (long long)batch * n * n, sizeof(float2), "factor output", line))
+    return hipErrorInvalidValue;
+  if (t_dry) return hipSuccess;
+  return narrow_factor(L, n, mode, out, batch, s);
+}
+
 hipError_t k_copy(void* dst, const void* src, size_t bytes, hipStream_t s, int line) {
   if (!fits(dst, (long long)bytes, 1, "copy destination", line) || !fits(src, (long long)bytes, 1, "copy source", line))
     return hipErrorInvalidValue;
@@ -395,6 +403,7 @@ mpdo_status eig_b_(int line, int n, int batch, double2* G, EigB& e, hipStream_t 
 struct QRWS {
   double2 *G, *L1, *L2, *F, *W;
   int *piv1, *piv2, *rank1, *rank2;
+  double* cond2;
   void *pws, *iws;
   float2* A1;
 };
@@ -407,6 +416,7 @@ void carve_qr(Carver& c, int rows, int cols, int batch, QRWS& w) {
   w.piv2 = c.get<int>(n * batch);
   w.rank1 = c.get<int>((size_t)batch);
   w.rank2 = c.get<int>((size_t)batch);
+  w.cond2 = c.get<double>((size_t)batch);
   w.pws = c.get<char>(pchol_workspace_bytes(cols, batch));
   w.iws = c.get<char>(pchol_inverse_workspace_bytes(cols, batch));
   w.F = c.get<double2>(n * n * batch);
@@ -438,6 +448,23 @@ hipError_t k_pinv(int n, int batch, const double2* L, const int* piv, const int*
   return pchol_inverse_b(n, batch, L, piv, rank, mode, X, w.iws, s);
 }
 #define K_PCHOL(n, batch, G, tol, L, piv, rank, w, s) k_pchol(n, batch, G, tol, L, piv, rank, w, s, __LINE__)
+
+// One pass of Cholesky QR leaves the columns orthonormal to about 1e-16 times the squared
+// condition number of the input; below QR1_COND2 that is under the complex64 rounding of the
+// stored factor and the second pass is skipped. Reads the estimates back (one sync).
+constexpr double QR1_COND2 = 1e8;
+mpdo_status one_pass_enough(int n, int batch, QRWS& w, bool& ok, hipStream_t s, int line) {
+  ok = false;
+  if (!fits(w.cond2, batch, sizeof(double), "QR condition estimates", line)) return layer_fail(MPDO_EINVAL, t_oob.c_str());
+  if (t_dry) return MPDO_OK;
+  LCK(pchol_cond2(n, batch, w.L1, w.piv1, w.rank1, w.cond2, s));
+  std::vector<double> c(batch);
+  LCK(hipMemcpyAsync(c.data(), w.cond2, sizeof(double) * batch, hipMemcpyDeviceToHost, s));
+  LCK(hipStreamSynchronize(s));
+  ok = true;
+  for (double v : c) ok = ok && v < QR1_COND2;
+  return MPDO_OK;
+}
 #define K_PINV(n, batch, L, piv, rank, mode, X, w, s) k_pinv(n, batch, L, piv, rank, mode, X, w, s, __LINE__)
 
 // Q (rows x cols) with orthonormal columns, Rf (cols x cols) complex64 with M = Q Rf; M is tall.
@@ -449,6 +476,13 @@ mpdo_status qr_b(const float2* M, int rows, int cols, float2* Q, float2* Rf, int
   LCK(K_PCHOL(cols, batch, w.G, FLOOR, w.L1, w.piv1, w.rank1, w, s));
   LCK(K_PINV(cols, batch, w.L1, w.piv1, w.rank1, 0, w.F, w, s));
   LCK(k_widen(M, w.W, rc, rc, rc, batch, s, __LINE__));
+  bool one = false;
+  LST(one_pass_enough(cols, batch, w, one, s, __LINE__));
+  if (one) {   // Q = M X1, Rf = L1^H
+    LCK(mmb(C128_IN_C64_OUT, rows, cols, cols, w.W, cols, rc, w.F, cols, cc, Q, cols, rc, batch, s));
+    LCK(k_narrow_factor(w.L1, cols, 0, Rf, batch, s, __LINE__));
+    return MPDO_OK;
+  }
   LCK(mmb(C128_IN_C64_OUT, rows, cols, cols, w.W, cols, rc, w.F, cols, cc, w.A1, cols, rc, batch, s));
   LCK(gram_cols_b(w.A1, rows, cols, rc, w.G, batch, s));
   LCK(K_PCHOL(cols, batch, w.G, FLOOR, w.L2, w.piv2, w.rank2, w, s));
@@ -473,6 +507,13 @@ mpdo_status lq_b(const float2* X, int rows, int cols, float2* Qt, float2* Lf, in
   LCK(K_PCHOL(rows, batch, w.G, FLOOR, w.L1, w.piv1, w.rank1, w, s));
   LCK(K_PINV(rows, batch, w.L1, w.piv1, w.rank1, 1, w.F, w, s));
   LCK(k_widen(X, w.W, rc, rc, rc, batch, s, __LINE__));
+  bool one = false;
+  LST(one_pass_enough(rows, batch, w, one, s, __LINE__));
+  if (one) {   // Qt = X1 X, Lf = L1
+    LCK(mmb(C128_IN_C64_OUT, rows, cols, rows, w.F, rows, rr, w.W, cols, rc, Qt, cols, rc, batch, s));
+    LCK(k_narrow_factor(w.L1, rows, 1, Lf, batch, s, __LINE__));
+    return MPDO_OK;
+  }
   LCK(mmb(C128_IN_C64_OUT, rows, cols, rows, w.F, rows, rr, w.W, cols, rc, w.A1, cols, rc, batch, s));
   LCK(gram_rows_b(w.A1, rows, cols, rc, w.G, batch, s));
   LCK(K_PCHOL(rows, batch, w.G, FLOOR, w.L2, w.piv2, w.rank2, w, s));

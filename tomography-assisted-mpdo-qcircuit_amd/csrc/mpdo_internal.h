@@ -61,6 +61,11 @@ hipError_t pchol_b(int n, int batch, void* A, long long lda, long long bsA, doub
 // After pchol_b (same ws, rank): w[b][rank[b] + j], j < n - rank[b], receives the diagonal left in
 // matrix b's Schur complement, sorted descending (the Gram mass below the pivot threshold); n <= 1024.
 hipError_t pchol_tail_eigs(int n, int batch, const void* ws, const int* rank, double* w, long long ldw, hipStream_t s);
+// cond2[b] (device) = (first pivot / last pivot) of matrix b's factor: the squared ratio of the
+// extreme diagonal entries of the pivot-ordered triangle, an estimate of the condition number of
+// the Gram matrix restricted to its numerical rank.
+hipError_t pchol_cond2(int n, int batch, const double2* L, const int* piv, const int* rank, double* cond2,
+                       hipStream_t s);
 // The Cholesky-QR factor of a pivoted factor, as complex128 X[b] (n x n): with Lt[t][u] =
 // L[piv[t]][u] (t, u < rank), mode 0 gives X[piv[t]][u] = conj(Lt^-1[u][t]) (Q = M X has
 // orthonormal columns when M^H M = L L^H and M = Q L^H), mode 1 X[u][piv[t]] = Lt^-1[u][t]

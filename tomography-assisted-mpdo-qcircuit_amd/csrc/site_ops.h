@@ -37,6 +37,8 @@ hipError_t kraus_gram(const double2* Pg, const float2* Gt, int K, int P1, double
 hipError_t kappa_core_gram(const double2* Hg, const float2* Gt, int K, int A, double2* out, int batch, hipStream_t s);
 hipError_t kappa_core_vec(const float2* Vk, const float2* Gt, int K, int A, int kk, float2* Vt, int batch,
                           hipStream_t s);
+// complex64 copy of a column-stored n x n factor: mode 0 its conjugate transpose, mode 1 itself (row-major)
+hipError_t narrow_factor(const double2* L, int n, int mode, float2* out, int batch, hipStream_t s);
 // out[b] = sum over t < S of in[b S + t] (n complex128 each)
 hipError_t sum_parts(const double2* in, int S, long long n, double2* out, int batch, hipStream_t s);
 hipError_t eye_b(float2* out, int rows, int cols, long long bs, int batch, hipStream_t s);

@@ -313,6 +313,29 @@ hipError_t kappa_core_vec(const float2* Vk, const float2* Gt, int K, int A, int 
   return hipGetLastError();
 }
 
+// Complex64 copies of a factor stored by columns (L[v n + i] = L_iv): mode 0 out[u][j] = conj(L_ju)
+// (the factor's conjugate transpose, row-major), mode 1 out[i][v] = L_iv (row-major)
+__global__ void narrow_factor_kernel(const double2* L, int n, int mode, float2* out) {
+  const long long nn = (long long)n * n;
+  L += blockIdx.y * nn;
+  out += blockIdx.y * nn;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < nn; e += (long long)gridDim.x * blockDim.x) {
+    if (mode == 0) {
+      const double2 v = L[e];
+      out[e] = make_float2((float)v.x, -(float)v.y);
+    } else {
+      const long long i = e / n, v = e % n;
+      const double2 x = L[v * n + i];
+      out[e] = make_float2((float)x.x, (float)x.y);
+    }
+  }
+}
+
+hipError_t narrow_factor(const double2* L, int n, int mode, float2* out, int batch, hipStream_t s) {
+  hipLaunchKernelGGL(narrow_factor_kernel, dim3(grid1((long long)n * n), batch), dim3(256), 0, s, L, n, mode, out);
+  return hipGetLastError();
+}
+
 // out[b][e] = sum_s in[b S + s][e], e < n (partial products of a split reduction)
 __global__ void sum_parts_kernel(const double2* in, int S, long long n, double2* out) {
   const int b = blockIdx.y;

@@ -34,6 +34,22 @@ for _p in (ROOT, PKG):
 METRIC = 'circuit-layers/sec at χ=256,κ=8 (batched vs plain) @ ≤1e-5 DM error'
 PEAK_TFLOPS = {'fp64': 78.6, 'fp32': 157.3}     # MI355X dense peaks
 N_QUBITS, CHI, KAPPA, P_NOISE, FIRST_LAYER = 32, 256, 8, 0.002, 10
+# HBM bytes per GEMM launch of the timed step, from the two rocprofv3 --pmc passes (FETCH_SIZE,
+# WRITE_SIZE) of this benchmark, scaled by the copy calibration; the file records the commands
+TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'gemm_kernel_traffic_fused_layer_n32_chi256_kappa8_b32.md')
+CALIB_BYTES = 1 << 30
+
+
+def _traffic_per_launch():
+    """The measured HBM bytes per GEMM launch recorded in TRAFFIC_FILE (line
+    `traffic_bytes_per_launch = N`), or None when that measurement is absent."""
+    import re
+    try:
+        with open(TRAFFIC_FILE) as f:
+            m = re.search(r'traffic_bytes_per_launch = ([0-9.e+]+)', f.read())
+    except OSError:
+        return None
+    return float(m.group(1)) if m else None
 
 
 def _steady_chains(dims, copies, dev, seed=2024):
@@ -125,6 +141,10 @@ def main():
     ap.add_argument('--copies', type=int, default=32)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-threads', type=int, default=16)
+    ap.add_argument('--copy-calibration', action='store_true',
+                    help='before the warmup, run torch.neg over 1 GiB of float32 three times (1 GiB read and '
+                         '1 GiB written per launch): the reference launch that scales the HBM counters of a '
+                         'rocprofv3 --pmc run of this benchmark')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). '
@@ -139,6 +159,15 @@ def main():
     if not torch.cuda.is_available():
         raise SystemExit('bench.py needs the MI355X (cuda:0)')
     dev = torch.device('cuda:0')
+    if args.copy_calibration:
+        src = torch.ones(CALIB_BYTES // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
+        print(f'# copy calibration: 3 launches of torch.neg, {CALIB_BYTES} bytes read and {CALIB_BYTES} '
+              'bytes written each', file=sys.stderr, flush=True)
     lib = _lib.load()
     prev_mode = lib.mpdo_set_math_mode(0)
     B = args.copies
@@ -196,10 +225,11 @@ def main():
     kernels = {'gemm': 'mpdo::gemm_kernel (complex GEMM, fp64 MFMA accumulate)',
                'eig_rotate': 'mpdo::bj_sym + mpdo::bj_rotate_v (block-Jacobi rotation updates, fp64 MFMA)',
                'eig_solve': 'mpdo::bj_solve (block-Jacobi 64x64 subproblem, fp64)'}
+    traffic = _traffic_per_launch() if name == 'gemm' else None
     roofline = {'bound': 'mfma', 'achieved': round(achieved, 3), 'peak': PEAK_TFLOPS[prec], 'unit': 'TFLOP/s',
-                'frac': round(achieved / PEAK_TFLOPS[prec], 4), 'traffic': None,
+                'frac': round(achieved / PEAK_TFLOPS[prec], 4), 'traffic': traffic,
                 'kernel': kernels.get(name, name), 'launches': n_l, 'avg_launch_ms': round(ms / max(n_l, 1), 4),
-                'flops_per_launch': fl / max(n_l, 1),
+                'flops_per_launch': fl / max(n_l, 1), 'operand_bytes_per_launch': by / max(n_l, 1),
                 'share_of_kernel_time': round(ms / max(sum(v[1] for v in prof.values()), 1e-9), 3),
                 'classes_ms': {k: round(v[1], 1) for k, v in prof.items()}}
     value = B * args.steps / T

@@ -18,7 +18,10 @@
 //   Cu (I (x) H) Cu^H of the core (H the Gram of Q2 P_right, so that factor is never factored):
 //   U sqrt(S) gives the new min site, S^{-1/2} U^H Cu (I (x) Q2 P_right) the new max site. The
 //   kappa cut of every site runs as soon as the site is final, so the large max-site tensor
-//   exists for one pair at a time.
+//   exists for one pair at a time. Cu's 2 K2 p2 columns are the channel's image of the 4 blocks
+//   of Y = R1t L2, so both Grams of Cu come from 4 p1t x 4 p1t Grams of Y's blocks and the 8 x 8
+//   Gram of the channel; Cu is formed only when the new max site is not kappa-cut in factored
+//   form.
 // Every stage is batched over the chains (same shapes; rank rules deterministic because no
 // error threshold is used).
 #include "mpdo.h"
@@ -235,6 +238,25 @@ hipError_t k_kraus_gram(const double2* Pg, const float2* Gt, int K, int P1, doub
     return hipErrorInvalidValue;
   if (t_dry) return hipSuccess;
   return kraus_gram(Pg, Gt, K, P1, out, batch, s);
+}
+
+hipError_t k_kraus_dual(const double2* N, const float2* Gt, int K, int P1, double2* out, int batch, hipStream_t s,
+                        int line) {
+  const long long n = 2LL * P1;
+  if (!fits(N, batch * n * n, sizeof(double2), "dual core input", line) ||
+      !fits(Gt, 16LL * K, sizeof(float2), "dual core channel", line) ||
+      !fits(out, batch * 4 * n * n, sizeof(double2), "dual core output", line))
+    return hipErrorInvalidValue;
+  if (t_dry) return hipSuccess;
+  return kraus_dual(N, Gt, K, P1, out, batch, s);
+}
+
+hipError_t k_kron_i2(const float2* R, int P, int Q, double2* out, int batch, hipStream_t s, int line) {
+  if (!fits(R, (long long)batch * P * Q, sizeof(float2), "kron input", line) ||
+      !fits(out, (long long)batch * 4 * P * Q, sizeof(double2), "kron output", line))
+    return hipErrorInvalidValue;
+  if (t_dry) return hipSuccess;
+  return kron_i2(R, P, Q, out, batch, s);
 }
 
 hipError_t k_kappa_gram(const double2* Hg, const float2* Gt, int K, int A, double2* out, int batch, hipStream_t s,
@@ -700,7 +722,7 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     }
   // sizes (elements per chain) of the shared scratch buffers: each the maximum over its uses
   auto kapplies = [&](int i) { return P.out[i].a != P.chi_out[i].a; };
-  size_t mxPg = 1, mxZm = 1, mxV = 1, mxHk = 1, mxVt = 1;
+  size_t mxPg = 1, mxZm = 1, mxV = 1, mxHk = 1, mxVt = 1, mxRs = 1, mxNn = 1, mxCp = 1;
   size_t mxQR_rows = 1, mxQR_cols = 1, mxR = 1, mxY = 1, mxOm = 1, mxZ = 1, mxQ2P = 1, mxC = 1;
   size_t mxTmin = 1, mxWm = 1, mxTm = 1, mxFL = 1, mxPn = 1, mxRT = 1, mxX1 = 1, mxLast = 1, mxTcur = 1;
   size_t mxTchi = 1, mxMp = 1, mxKout = 1, mxX1t = 1, mxQ1t = 1, mxR1t = 1;
@@ -752,7 +774,9 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     mxQR_rows = std::max(mxQR_rows, (size_t)2 * ps.p1);
     mxQR_cols = std::max(mxQR_cols, (size_t)std::min(2 * ps.p1, 2 * ps.p1t));
     mxY = std::max(mxY, (size_t)4 * ps.p1t * ps.p2);
-    mxC = std::max(mxC, (size_t)2 * ps.p1t * 2 * K2 * ps.p2);
+    mxRs = std::max(mxRs, (size_t)2 * ps.p1 * 2 * ps.p1t);
+    mxNn = std::max(mxNn, (size_t)4 * ps.p1t * ps.p1t);
+    mxCp = std::max(mxCp, (size_t)2 * ps.p1 * 2 * ps.p1t);
     mxOm = std::max(mxOm, (size_t)ps.p2 * ps.p2);
     mxZ = std::max(mxZ, (size_t)ps.p2 * ps.cols2);
     mxEigR = std::max(mxEigR, P.in[ps.i + 1].r);
@@ -763,9 +787,14 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     mxFL = std::max(mxFL, (size_t)2 * ps.p1 * kp);
     mxTmin = std::max(mxTmin, (size_t)ps.rows1 * 2 * kp);
     mxTcur = std::max(mxTcur, (size_t)ps.kprev * 2 * P.a1[ps.i].a * kp);
-    mxWm = std::max(mxWm, (size_t)kp * 2 * K2 * ps.p2);
-    mxTm = std::max(mxTm, (size_t)kp * 2 * K2 * c3);
-    if (kapplies(ps.i + 1) && 2 * K2 > 8) {   // Kraus-factored kappa cut of the max site
+    if (!(kapplies(ps.i + 1) && 2 * K2 > 8)) {   // the max site formed from the explicit core
+      mxC = std::max(mxC, (size_t)2 * ps.p1t * 2 * K2 * ps.p2);
+      mxCp = std::max(mxCp, mxC);
+      mxWm = std::max(mxWm, (size_t)kp * 2 * K2 * ps.p2);
+      mxTm = std::max(mxTm, (size_t)kp * 2 * K2 * c3);
+    } else {                                     // Kraus-factored kappa cut of the max site
+      mxWm = std::max(mxWm, (size_t)kp * 2 * ps.p1t);
+      mxTm = std::max(mxTm, (size_t)8 * kp * c3);
       mxZm = std::max(mxZm, (size_t)kp * 2 * ps.p1t);
       mxV = std::max(mxV, (size_t)kp * 8 * ps.p2);
       mxHk = std::max(mxHk, (size_t)64 * a2 * a2);
@@ -788,7 +817,7 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
   Site1q* dtab = nullptr;
   float2 *Mlast = nullptr, *FLq, *Rcur, *Rnext, *RT, *X1, *X1t, *Q1t, *R1t, *Cs, *Cp, *Cu, *Zb, *Q2P, *FL, *FR, *Tmin, *Wm,
          *Tm, *Pn, *Pn2, *PW, *Pp, *Tchi, *Mp, *Vk, *Kout, *dG, *Tcur2, *dK;
-  double2 *G, *GR, *Om64, *Q2w, *Z64, *H64, *Cw64, *Z64c, *Gparts, *Pg, *Hk;
+  double2 *G, *GR, *Om64, *Q2w, *Z64, *H64, *Gparts, *Pg, *Hk, *Yw, *Tw, *Rs64, *Zr, *Nn;
   float2 *Zm, *Vf, *Vt;
   double* ddisc;
   int* dk;
@@ -829,8 +858,13 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     Q1t = c.get<float2>(mxQ1t * B);
     R1t = c.get<float2>(mxR1t * B);
     Cs = c.get<float2>(mxC * B);
-    Cp = c.get<float2>(mxC * B);
+    Cp = c.get<float2>(mxCp * B);
     Cu = c.get<float2>(mxC * B);
+    Yw = c.get<double2>(mxY * B);
+    Tw = c.get<double2>(mxY * B);
+    Rs64 = c.get<double2>(mxRs * B);
+    Zr = c.get<double2>(mxRs * B);
+    Nn = c.get<double2>(mxNn * B);
     Om64 = c.get<double2>(mxOm * B);
     Q2w = c.get<double2>(mxZ * B);
     Z64 = c.get<double2>(mxZ * B);
@@ -838,8 +872,6 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     dtab = c.get<Site1q>((size_t)n * B);
     Q2P = c.get<float2>(mxQ2P * B);
     H64 = c.get<double2>(mxOm * B);
-    Cw64 = c.get<double2>(mxC * B);
-    Z64c = c.get<double2>(mxC * B);
     FL = c.get<float2>(mxFL * B);
     FLq = c.get<float2>(mxFL * B);
     FR = c.get<float2>(mxFL * B);
@@ -952,6 +984,26 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     LCK(mmb(g_tensor_prec, ps.pz, (int)cols, mq, ps.Rz, mq, (long long)ps.pz * mq, Cp, cols, (long long)mq * cols,
             Cu, cols, (long long)ps.pz * cols, B, s));
     ps.nc = ps.pz;
+    return MPDO_OK;
+  };
+  // Rs: the core's rows as a map from the channel output's rows (p, o1) (nT = 2 p1t), so that
+  // Rc = Rs G(Y): Rz Um^H with the gate-time cut, Ra (x) I_2 without it; complex128 in Rs64
+  // (nc x nT). Sets nc.
+  auto core_rows = [&](PairStore& ps) -> mpdo_status {
+    const int nT = 2 * ps.p1t;
+    if (ps.has_P) {
+      GemmArgs g{};
+      g.M = ps.pz; g.N = nT; g.K = ps.mq; g.batch = B;
+      g.A = ps.Rz; g.rsA = ps.mq; g.csA = 1; g.bsA = (long long)ps.pz * ps.mq;
+      g.B = ps.Um; g.rsB = 1; g.csB = ps.mq; g.bsB = (long long)nT * ps.mq; g.conjB = 1;
+      g.C = Rs64; g.rsC = nT; g.csC = 1; g.bsC = (long long)ps.pz * nT;
+      g.alpha_re = 1;
+      LCK(K_GEMM(C64_F64ACC, g, s));
+      ps.nc = ps.pz;
+    } else {
+      LCK(k_kron_i2(ps.Ra, ps.p1, ps.p1t, Rs64, B, s, __LINE__));
+      ps.nc = 2 * ps.p1;
+    }
     return MPDO_OK;
   };
 
@@ -1077,20 +1129,27 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
       }
     }
     if (!ps.has_R) continue;
-    // (3) the core of R theta: (Q1 (x) I) Qc Rc with Rc = [nc x (s2 K2 p2)] in Cu
-    LST(make_core(ps, (int)pi - 1));
-    const long long szC = (long long)ps.nc * 2 * K2 * ps.p2;
-    // right-bond Gram of R theta, all in complex128 so that it is Hermitian to fp64 rounding:
-    // Om = C_v^H C_v with C_v = [(2 p1 2 K2) x p2]; GR = sum_{q a2} conj(Q2[q a2, r]) (Om Q2)[q a2, r']
+    // (3) the core of R theta is (Q1 (x) I) Qc Rc with Rc = Rs G(Y) = [nc x (s2 K2 p2)]; its
+    //     right-bond Gram, all in complex128 so that it is Hermitian to fp64 rounding, is
+    //     GR = sum_{q a2} conj(Q2[q a2, r]) (Om Q2)[q a2, r'] with Om = Rc_v^H Rc_v
+    //     (Rc_v = [(nc s2 K2) x p2]) = sum_{s2 K2} C_{s2 K2}^H (Rs^H Rs) C_{s2 K2} = Ycat^H Nt Ycat,
+    //     Nt the Kraus dual of Rs^H Rs: the core itself is never formed
+    LST(core_rows(ps));
     {
+      const int nT = 2 * p1t, n4 = 4 * p1t;
+      const long long bsY = (long long)n4 * ps.p2;
       GemmArgs g{};
-      const int rowsv = ps.nc * 2 * K2;
-      g.M = ps.p2; g.N = ps.p2; g.K = rowsv; g.batch = B;
-      g.A = Cu; g.rsA = 1; g.csA = ps.p2; g.bsA = szC; g.conjA = 1;
-      g.B = Cu; g.rsB = ps.p2; g.csB = 1; g.bsB = szC;
-      g.C = Om64; g.rsC = ps.p2; g.csC = 1; g.bsC = (long long)ps.p2 * ps.p2;
+      g.M = nT; g.N = nT; g.K = ps.nc; g.batch = B;
+      g.A = Rs64; g.rsA = 1; g.csA = nT; g.bsA = (long long)ps.nc * nT; g.conjA = 1;
+      g.B = Rs64; g.rsB = nT; g.csB = 1; g.bsB = (long long)ps.nc * nT;
+      g.C = Nn; g.rsC = nT; g.csC = 1; g.bsC = (long long)nT * nT;
       g.alpha_re = 1;
-      LCK(K_GEMM(C64_F64ACC, g, s));
+      LCK(K_GEMM(C128_F64ACC, g, s));
+      LCK(k_kraus_dual(Nn, dG + (pi - 1) * 16 * K2, K2, p1t, Pg, B, s, __LINE__));
+      LCK(k_widen(ps.Y, Yw, bsY, bsY, bsY, B, s, __LINE__));
+      LCK(mmb(C128_F64ACC, n4, ps.p2, n4, Pg, n4, (long long)n4 * n4, Yw, ps.p2, bsY, Tw, ps.p2, bsY, B, s));
+      LCK(mmb(C128_F64ACC, ps.p2, ps.p2, n4, Yw, ps.p2, bsY, Tw, ps.p2, bsY, Om64, ps.p2, (long long)ps.p2 * ps.p2, B, s,
+              true));
     }
     if (ps.q2id) {
       LCK(k_bds64(Om64, a2, r, GR, (long long)ps.p2 * ps.p2, (long long)r * r, B, s, __LINE__));
@@ -1302,22 +1361,33 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
       Qright = Q2P;
     }
     // Theta = (Q1 (x) I) Qc Rc (I (x) Q2P): its core's Gram is Rc (I (x) H) Rc^H with
-    // H = Q2P Q2P^H [p2 x p2], all in complex128 (Q2P itself is never factored)
+    // H = Q2P Q2P^H [p2 x p2], = Rs kraus_gram(Ycat H Ycat^H) Rs^H, all in complex128 (neither
+    // Q2P nor the core is factored or formed)
     LCK(gram_rows_b(Qright, ps.p2, c3, (long long)ps.p2 * c3, H64, B, s));
-    LST(make_core(ps, pidx + 1));
+    LST(core_rows(ps));
     const int nC = ps.nc;
-    const long long colsR = 2LL * K2 * ps.p2, szC = (long long)nC * colsR;
-    LCK(k_widen(Cu, Cw64, szC, szC, szC, B, s, __LINE__));
-    LCK(mmb(C128_F64ACC, nC * 2 * K2, ps.p2, ps.p2, Cw64, ps.p2, szC, H64, ps.p2, (long long)ps.p2 * ps.p2, Z64c,
-            ps.p2, szC, B, s));
     {
+      const int p1t = ps.p1t, nT = 2 * p1t, n4 = 4 * p1t;
+      const long long bsY = (long long)n4 * ps.p2, bsRs = (long long)nC * nT;
+      LCK(k_widen(ps.Y, Yw, bsY, bsY, bsY, B, s, __LINE__));
+      LCK(mmb(C128_F64ACC, n4, ps.p2, ps.p2, Yw, ps.p2, bsY, H64, ps.p2, (long long)ps.p2 * ps.p2, Tw, ps.p2, bsY, B,
+              s));
       GemmArgs g{};
-      g.M = nC; g.N = nC; g.K = (int)colsR; g.batch = B;
-      g.A = Z64c; g.rsA = colsR; g.csA = 1; g.bsA = szC;
-      g.B = Cw64; g.rsB = 1; g.csB = colsR; g.bsB = szC; g.conjB = 1;
-      g.C = G; g.rsC = nC; g.csC = 1; g.bsC = (long long)nC * nC;
+      g.M = n4; g.N = n4; g.K = ps.p2; g.batch = B;
+      g.A = Tw; g.rsA = ps.p2; g.csA = 1; g.bsA = bsY;
+      g.B = Yw; g.rsB = 1; g.csB = ps.p2; g.bsB = bsY; g.conjB = 1;
+      g.C = Pg; g.rsC = n4; g.csC = 1; g.bsC = (long long)n4 * n4;
       g.alpha_re = 1;
       LCK(K_GEMM(C128_F64ACC, g, s));
+      LCK(k_kraus_gram(Pg, dG + (size_t)(pidx + 1) * 16 * K2, K2, p1t, Nn, B, s, __LINE__));
+      LCK(mmb(C128_F64ACC, nC, nT, nT, Rs64, nT, bsRs, Nn, nT, (long long)nT * nT, Zr, nT, bsRs, B, s));
+      GemmArgs h{};
+      h.M = nC; h.N = nC; h.K = nT; h.batch = B;
+      h.A = Zr; h.rsA = nT; h.csA = 1; h.bsA = bsRs;
+      h.B = Rs64; h.rsB = 1; h.csB = nT; h.bsB = bsRs; h.conjB = 1;
+      h.C = G; h.rsC = nC; h.csC = 1; h.bsC = (long long)nC * nC;
+      h.alpha_re = 1;
+      LCK(K_GEMM(C128_F64ACC, h, s));
     }
     LST(eig_b(nC, B, G, eE, s, true));
     LCK(K_KEEP(eE.w, nC, nC, P.step_len[i], kp, -1.0, 1, dk, ddisc + (size_t)B * i, B, s));
@@ -1346,6 +1416,8 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     if (kapplies(j) && 2 * K2 > 8) {
       LST(kappa_factored(ps, j, pidx + 1, kp, kr, a2, Qright));
     } else {
+      LST(make_core(ps, pidx + 1));
+      const long long colsR = 2LL * K2 * ps.p2, szC = (long long)nC * colsR;
       LCK(mmb(g_tensor_prec, kp, (int)colsR, nC, FR, nC, (long long)kp * nC, Cu, colsR, szC, Wm, colsR,
               kp * colsR, B, s));
       LCK(mmb(g_tensor_prec, kp * 2 * K2, c3, ps.p2, Wm, ps.p2, kp * colsR, Qright, c3, (long long)ps.p2 * c3, Tm,

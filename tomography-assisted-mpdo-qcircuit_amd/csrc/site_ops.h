@@ -31,6 +31,11 @@ hipError_t widen(const float2* in, double2* out, long long n, long long bsI, lon
 // C C^H of the two-site core C = G(Y) (channel2q layout) from Pg = Ycat Ycat^H (4 P1 x 4 P1 per
 // batch entry) and the channel tensor: out (2 P1 x 2 P1, complex128)
 hipError_t kraus_gram(const double2* Pg, const float2* Gt, int K, int P1, double2* out, int batch, hipStream_t s);
+// The dual: sum over the channel's output/Kraus columns of C^H N C = Ycat^H Nt Ycat, Nt (4 P1 x 4 P1,
+// complex128) from N (2 P1 x 2 P1, rows (p, o1)) and the channel tensor
+hipError_t kraus_dual(const double2* N, const float2* Gt, int K, int P1, double2* out, int batch, hipStream_t s);
+// out = R (x) I_2 with rows (p, s), columns (p', s') (complex64 in, complex128 out)
+hipError_t kron_i2(const float2* R, int P, int Q, double2* out, int batch, hipStream_t s);
 // Kappa cut of a Kraus-factored max site T[k,s2,K2,a2,r] = sum_oi g[s2][oi][K2] W[k,oi,a2,r]:
 // the inner Gram (A K x A K) from Hg = Wp Wp^H (8 A x 8 A, rows (oi, a2)), and the kept
 // directions Vk (A K x kk) mapped to Vt[(oi a2), (s2 j)] (8 A x 2 kk)

@@ -236,6 +236,79 @@ hipError_t kraus_gram(const double2* Pg, const float2* Gt, int K, int P1, double
   return hipGetLastError();
 }
 
+// The dual contraction: for C as in kraus_gram and N (2 P1 x 2 P1, rows (p, o1)),
+// sum_{o2,k} C_{o2 k}^H N C_{o2 k} = Ycat^H Nt Ycat with
+// Nt[(i, p), (i', p')] = sum_{o1,o1'} conj(E[o1, i, o1', i']) N[(p, o1), (p', o1')]  (4 P1 x 4 P1).
+// One thread per (p, p'): its four N entries serve all sixteen (i, i') blocks.
+__global__ void kraus_dual_kernel(const double2* N, const float2* Gt, int K, int P1, double2* out, long long bsN,
+                                  long long bsO) {
+  __shared__ double2 E[64];   // [o1][i][o1'][i'], as in kraus_gram_kernel
+  N += blockIdx.y * bsN;
+  out += blockIdx.y * bsO;
+  if (threadIdx.x < 64) {
+    const int o1 = threadIdx.x >> 5, i = (threadIdx.x >> 3) & 3, o1p = (threadIdx.x >> 2) & 1, ip = threadIdx.x & 3;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int o2 = 0; o2 < 2; ++o2)
+      for (int k = 0; k < K; ++k) {
+        const float2 g = Gt[((o1 * 2 + o2) * 4 + i) * K + k];
+        const float2 h = Gt[((o1p * 2 + o2) * 4 + ip) * K + k];
+        acc.x += (double)g.x * h.x + (double)g.y * h.y;
+        acc.y += (double)g.y * h.x - (double)g.x * h.y;
+      }
+    E[threadIdx.x] = acc;
+  }
+  __syncthreads();
+  const long long ldn = 2LL * P1, ldo = 4LL * P1, pp = (long long)P1 * P1;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < pp; e += (long long)gridDim.x * blockDim.x) {
+    const int p = (int)(e / P1), q = (int)(e % P1);
+    double2 nv[2][2];
+#pragma unroll
+    for (int o1 = 0; o1 < 2; ++o1)
+#pragma unroll
+      for (int o1p = 0; o1p < 2; ++o1p) nv[o1][o1p] = N[(2LL * p + o1) * ldn + 2LL * q + o1p];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int ip = 0; ip < 4; ++ip) {
+        double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int o1 = 0; o1 < 2; ++o1)
+#pragma unroll
+          for (int o1p = 0; o1p < 2; ++o1p) {
+            const double2 w = E[((o1 * 4 + i) * 2 + o1p) * 4 + ip];   // conj(w) * n
+            const double2 v = nv[o1][o1p];
+            acc.x += w.x * v.x + w.y * v.y;
+            acc.y += w.x * v.y - w.y * v.x;
+          }
+        out[((long long)i * P1 + p) * ldo + (long long)ip * P1 + q] = acc;
+      }
+  }
+}
+
+hipError_t kraus_dual(const double2* N, const float2* Gt, int K, int P1, double2* out, int batch, hipStream_t s) {
+  const long long pp = (long long)P1 * P1;
+  hipLaunchKernelGGL(kraus_dual_kernel, dim3(grid1(pp), batch), dim3(256), 0, s, N, Gt, K, P1, out, 4 * pp, 16 * pp);
+  return hipGetLastError();
+}
+
+// out[(p, s), (p', s')] = delta_{s s'} R[p, p'] (2 P x 2 Q, complex128) from R (P x Q, complex64)
+__global__ void kron_i2_kernel(const float2* R, int P, int Q, double2* out, long long bsR, long long bsO) {
+  R += blockIdx.y * bsR;
+  out += blockIdx.y * bsO;
+  const long long tot = 4LL * P * Q;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot; e += (long long)gridDim.x * blockDim.x) {
+    const long long row = e / (2LL * Q), col = e % (2LL * Q);
+    const float2 v = R[(row >> 1) * Q + (col >> 1)];
+    out[e] = ((row & 1) == (col & 1)) ? make_double2(v.x, v.y) : make_double2(0.0, 0.0);
+  }
+}
+
+hipError_t kron_i2(const float2* R, int P, int Q, double2* out, int batch, hipStream_t s) {
+  const long long tot = 4LL * P * Q;
+  hipLaunchKernelGGL(kron_i2_kernel, dim3(grid1(tot), batch), dim3(256), 0, s, R, P, Q, out, (long long)P * Q, tot);
+  return hipGetLastError();
+}
+
 // Kappa Gram of a new max site held in Kraus-factored form. The site is
 // T[k, s2, K2, a2, r] = sum_{oi} g[s2][oi][K2] W[k, oi, a2, r] with oi = (o1, i), i the pair's
 // input physical index, g[s2][(o1,i)][K2] = Gt[((o1 2 + s2) 4 + i) K + K2]; given

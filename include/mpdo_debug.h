@@ -44,6 +44,10 @@ mpdo_status mpdo_debug_layer_dry_run(int batch, int n, const mpdo_site* in, int 
  * (an out-of-range v only reads it). */
 int mpdo_debug_gemm_tiles(int v);
 
+/* Threads per workgroup of the block-Jacobi subproblem kernel (256, 512 or 1024); returns the
+ * previous choice (any other v only reads it). */
+int mpdo_debug_bj_threads(int v);
+
 /* The GEMMs the last dry run of the fused layer would launch, in order: out [6 n] = source line,
  * M, N, K, batch, precision mode; returns n (fills at most cap ints). */
 int mpdo_debug_layer_gemm_log(int* out, int cap);

@@ -191,12 +191,16 @@ __global__ __launch_bounds__(256) void eigh_small_kernel(int n, double2* Ag, lon
 // ---------------------------------------------------------------------------------------
 // n > 64: block Jacobi on an N x N padded copy (N % 64 == 0), block width 32
 // ---------------------------------------------------------------------------------------
+// threads per bj_solve workgroup (mpdo_debug_bj_threads)
+int g_bj_threads = 1024;
+
 struct BJ {
   int n, N, batch;
   double2* A;   // [batch][N][N]
   double2* V;   // [batch][N][N]
   double2* W;   // [batch][N/64][64][64]
   int* flag;    // [batch]
+  int* ident;   // [batch][N/64]: the round's rotation of block pair i is the identity
   double* floor_abs;   // [batch]: ABS_TOL * max diagonal of the whole matrix
 };
 
@@ -232,8 +236,10 @@ __device__ inline void block_pair(int P, int r, int i, int& I, int& J) {
   if (I > J) { int t = I; I = J; J = t; }
 }
 
-// one workgroup per (block pair, matrix): diagonalise the 64x64 principal submatrix
-__global__ __launch_bounds__(256) void bj_solve(BJ s, int r, int inner_sweeps) {
+// one workgroup per (block pair, matrix): diagonalise the 64x64 principal submatrix (any
+// multiple of 64 threads up to 1024: the round's rotations, 2x2-block updates and V columns are
+// strided over the workgroup)
+__global__ __launch_bounds__(1024) void bj_solve(BJ s, int r, int inner_sweeps) {
   __shared__ double2 A[NS][LD];
   __shared__ double2 V[NS][LD];
   __shared__ Rot rot[NS / 2];
@@ -259,7 +265,10 @@ __global__ __launch_bounds__(256) void bj_solve(BJ s, int r, int inner_sweeps) {
   }
   double2* Wb = s.W + ((long long)b * (P / 2) + pi) * NS * NS;
   for (int e = tid; e < NS * NS; e += blockDim.x) Wb[e] = V[e / NS][e % NS];
-  if (tid == 0 && cnt > 0) atomicOr(&s.flag[b], 1);
+  if (tid == 0) {
+    if (cnt > 0) atomicOr(&s.flag[b], 1);
+    s.ident[b * (P / 2) + pi] = cnt == 0;
+  }
 }
 
 // X[rows, K u L] <- X[rows, K u L] * W_KL   (cols=1)   or
@@ -274,6 +283,7 @@ __device__ __forceinline__ void rotate_body(BJ s, int r, int cols, int z) {
   int b = isV ? z - s.batch : z;
   int I, J;
   block_pair(P, r, pi, I, J);
+  if (s.ident[b * (P / 2) + pi]) return;   // X J = X (uniform over the workgroup)
   double2* M = (isV ? s.V : s.A) + (long long)b * s.N * s.N;
   const double2* Wb = s.W + ((long long)b * (P / 2) + pi) * NS * NS;
   for (int e = tid; e < NS * NS; e += blockDim.x) {
@@ -408,6 +418,8 @@ __global__ __launch_bounds__(256) void bj_sym(BJ s, int r) {
   int Ir, Jr, Ic, Jc;
   block_pair(P, r, pr, Ir, Jr);
   block_pair(P, r, pc, Ic, Jc);
+  const bool idR = s.ident[b * (P / 2) + pr] != 0, idC = s.ident[b * (P / 2) + pc] != 0;
+  if (idR && idC) return;   // W_R^H A W_C = A (uniform over the workgroup)
   double2* A = s.A + (long long)b * s.N * s.N;
   const double2* Wr = s.W + ((long long)b * (P / 2) + pr) * NS * NS;
   const double2* Wc = s.W + ((long long)b * (P / 2) + pc) * NS * NS;
@@ -421,17 +433,32 @@ __global__ __launch_bounds__(256) void bj_sym(BJ s, int r) {
   const int wave = tid / 64, lane = tid % 64, li = lane % 16, lk = lane / 16;
   const int r0 = (wave / 2) * 32, c0 = (wave % 2) * 32;
   doublex4 re[2][2], im[2][2];
-  mfma_tile64(X, Wr, true, re, im, r0, c0, li, lk);     // T = W_R^H X
-  __syncthreads();
+  if (!idR) {
+    mfma_tile64(X, Wr, true, re, im, r0, c0, li, lk);     // T = W_R^H X
+    __syncthreads();
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int c = 0; c < 2; ++c)
+      for (int c = 0; c < 2; ++c)
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-        X[r0 + 16 * a + 4 * q + lk][c0 + 16 * c + li] = make_double2(re[a][c][q], im[a][c][q]);
-  __syncthreads();
-  mfma_tile64(X, Wc, false, re, im, r0, c0, li, lk);    // T W_C
+        for (int q = 0; q < 4; ++q)
+          X[r0 + 16 * a + 4 * q + lk][c0 + 16 * c + li] = make_double2(re[a][c][q], im[a][c][q]);
+    __syncthreads();
+  }
+  if (!idC) {
+    mfma_tile64(X, Wc, false, re, im, r0, c0, li, lk);    // T W_C
+  } else {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const double2 v = X[r0 + 16 * a + 4 * q + lk][c0 + 16 * c + li];
+          re[a][c][q] = v.x;
+          im[a][c][q] = v.y;
+        }
+  }
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -482,8 +509,8 @@ size_t eigh_workspace_bytes(int n, int batch) {
   size_t N = (size_t)((n + 63) / 64) * 64;
   size_t mats = align256(N * N * sizeof(double2) * batch);
   size_t w = align256((N / 64) * NS * NS * sizeof(double2) * batch);
-  return 2 * mats + w + align256(sizeof(int) * batch) + align256(sizeof(double) * batch) +
-         align256(sizeof(int) * N * batch) + 256;
+  return 2 * mats + w + align256(sizeof(int) * batch) + align256(sizeof(int) * (N / 64) * batch) +
+         align256(sizeof(double) * batch) + align256(sizeof(int) * N * batch) + 256;
 }
 
 hipError_t eigh(int n, int batch, void* A, long long lda, long long bsA, double* w, long long ldw, void* V,
@@ -516,6 +543,7 @@ hipError_t eigh(int n, int batch, void* A, long long lda, long long bsA, double*
   s.V = (double2*)p; p += mats;
   s.W = (double2*)p; p += align256((size_t)(s.N / 64) * NS * NS * sizeof(double2) * batch);
   s.flag = (int*)p; p += align256(sizeof(int) * batch);
+  s.ident = (int*)p; p += align256(sizeof(int) * (s.N / 64) * batch);
   s.floor_abs = (double*)p; p += align256(sizeof(double) * batch);
   int* order = (int*)p;
   const int P = s.N / 32;
@@ -535,7 +563,7 @@ hipError_t eigh(int n, int batch, void* A, long long lda, long long bsA, double*
     for (int r = 0; r < P - 1; ++r) {
       {
         prof::Scope ps(prof::EIG_SOLVE, st, solve_flops, solve_bytes);
-        hipLaunchKernelGGL(bj_solve, dim3(P / 2, batch), dim3(256), 0, st, s, r, 1);
+        hipLaunchKernelGGL(bj_solve, dim3(P / 2, batch), dim3(g_bj_threads), 0, st, s, r, 1);
       }
       {   // A <- J^H A J block pair by block pair (Hermitian: lower triangle + mirror)
         prof::Scope ps(prof::EIG_ROTATE, st, tiles_sym * 2 * tile_flops, tiles_sym * tile_bytes);
@@ -561,6 +589,12 @@ hipError_t eigh(int n, int batch, void* A, long long lda, long long bsA, double*
 }
 
 }  // namespace mpdo
+
+extern "C" int mpdo_debug_bj_threads(int v) {
+  const int old = mpdo::g_bj_threads;
+  if (v == 256 || v == 512 || v == 1024) mpdo::g_bj_threads = v;
+  return old;
+}
 
 // =======================================================================================
 // Eigen-decomposition of positive semidefinite Gram matrices, preconditioned by pivoted

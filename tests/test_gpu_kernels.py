@@ -41,6 +41,30 @@ def test_gemm(cuda, prec, M, N, K, tA, tB, cA, cB):
     assert err < tol, (prec, M, N, K, err)
 
 
+@pytest.mark.parametrize('prec', [1, 2, 3, 4])
+@pytest.mark.parametrize('n,K', [(1, 5), (17, 33), (64, 16), (65, 130), (200, 70), (257, 512)])
+def test_gemm_hermitian(cuda, prec, n, K):
+    """Hermitian-output mode (prec + 10): X^H X computed from the tiles on and below the
+    diagonal, the upper triangle mirrored; vs numpy, and exactly Hermitian."""
+    from mpdo_amd import _lib
+    rng = np.random.default_rng(n * 7 + K)
+    batch = 3
+    din = np.complex64 if prec in (1, 4) else np.complex128
+    X = _crand(rng, batch, K, n).astype(din)
+    out_t = torch.complex64 if prec in (3, 4) else torch.complex128
+    Cd = torch.full((batch, n, n), 7.0 + 3.0j, dtype=out_t, device=cuda)
+    Xd = torch.from_numpy(X).to(cuda)
+    _lib.debug_gemm(prec + 10, Xd, Xd, Cd, n, n, K, (1, n, K * n), (n, 1, K * n), (n, 1, n * n), True, False,
+                    1.0, 0.0, batch)
+    Xw = X.astype(np.complex128)
+    ref = np.conj(np.swapaxes(Xw, 1, 2)) @ Xw
+    got = Cd.cpu().numpy()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err < (1e-13 if prec == 2 else 1e-6), (prec, n, K, err)
+    assert np.array_equal(got, np.conj(np.swapaxes(got, 1, 2)) * np.where(np.eye(n, dtype=bool), 0, 1) +
+                          got * np.eye(n)), 'not exactly Hermitian off the diagonal'
+
+
 @pytest.mark.parametrize('n', [1, 2, 3, 17, 64, 65, 100, 192])
 def test_eigh(cuda, n):
     from mpdo_amd import _lib

@@ -934,8 +934,12 @@ mpdo_status mpdo_debug_gemm(int prec, int M, int N, int K, int batch, const void
                             long long bsA, int conjA, const void* B, long long rsB, long long csB, long long bsB,
                             int conjB, void* C, long long rsC, long long csC, long long bsC, double alpha_re,
                             double alpha_im, double beta_re, double beta_im, void* stream) {
+  const int herm = prec >= 10;
+  if (herm) prec -= 10;
   if (prec < 0 || prec > 4) return fail(MPDO_EINVAL, "prec %d", prec);
+  if (herm && M != N) return fail(MPDO_EINVAL, "Hermitian output needs M == N (%d, %d)", M, N);
   GemmArgs g{};
+  g.herm = herm;
   g.M = M; g.N = N; g.K = K; g.batch = batch;
   g.A = A; g.rsA = rsA; g.csA = csA; g.bsA = bsA; g.conjA = conjA;
   g.B = B; g.rsB = rsB; g.csB = csB; g.bsB = bsB; g.conjB = conjB;

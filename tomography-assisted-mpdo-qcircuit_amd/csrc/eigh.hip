@@ -657,6 +657,7 @@ hipError_t eigh_gram(int n, int batch, void* A, long long lda, long long bsA, do
     g.B = L; g.rsB = 1; g.csB = n; g.bsB = (long long)n * n;
     g.C = M; g.rsC = r; g.csC = 1; g.bsC = (long long)r * r;
     g.alpha_re = 1;
+    g.herm = 1;
     if ((err = gemm(C128_F64ACC, g, st)) != hipSuccess) return err;
   }
   int sw = 0;

@@ -13,6 +13,8 @@
 #include "mpdo_internal.h"
 #include "prof.h"
 
+#include <algorithm>
+
 namespace mpdo {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -58,9 +60,21 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * WM, wn = (wave & 1) * WN;
   const int b = blockIdx.z;
-  // tile order: blockIdx.x enumerates (m-tile, n-tile) with n fastest
-  const int ntn = (g.N + BN - 1) / BN;
-  const int m0 = (blockIdx.x / ntn) * BM, n0 = (blockIdx.x % ntn) * BN;
+  // tile order: blockIdx.x enumerates (m-tile, n-tile) with n fastest; a Hermitian output
+  // (BM == BN) enumerates the tiles with n-tile <= m-tile only
+  int m0, n0;
+  if (g.herm) {
+    const int e = blockIdx.x;
+    int mt = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+    while (mt * (mt + 1) / 2 > e) --mt;
+    while ((mt + 1) * (mt + 2) / 2 <= e) ++mt;
+    m0 = mt * BM;
+    n0 = (e - mt * (mt + 1) / 2) * BN;
+  } else {
+    const int ntn = (g.N + BN - 1) / BN;
+    m0 = (blockIdx.x / ntn) * BM;
+    n0 = (blockIdx.x % ntn) * BN;
+  }
   const TIn* A = (const TIn*)g.A + (long long)b * g.bsA;
   const TIn* B = (const TIn*)g.B + (long long)b * g.bsB;
   const bool aK = (g.csA == 1);   // A contiguous along k
@@ -175,13 +189,54 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
       }
 }
 
+// C[i][j] = conj(C[j][i]) for j > i (n x n per batch entry), 32 x 32 tiles through LDS: the
+// workgroup of lower tile (ti, tj) reads it row by row and writes its conjugate transpose into
+// tile (tj, ti) row by row (a diagonal tile only its own upper part).
+template <typename T>
+__global__ __launch_bounds__(256) void herm_mirror_kernel(T* C, int n, long long rs, long long cs, long long bs) {
+  __shared__ T t[32][33];
+  const int e = blockIdx.x;
+  int ti = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+  while (ti * (ti + 1) / 2 > e) --ti;
+  while ((ti + 1) * (ti + 2) / 2 <= e) ++ti;
+  const int tj = e - ti * (ti + 1) / 2;
+  C += (long long)blockIdx.y * bs;
+  const int r0 = ti * 32, c0 = tj * 32;
+  for (int q = threadIdx.x; q < 32 * 32; q += 256) {
+    const int i = q / 32, j = q % 32;
+    if (r0 + i < n && c0 + j < n) t[i][j] = C[(long long)(r0 + i) * rs + (long long)(c0 + j) * cs];
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < 32 * 32; q += 256) {
+    const int i = q / 32, j = q % 32;   // element (c0 + i, r0 + j) of the upper tile
+    if (c0 + i < n && r0 + j < n && r0 + j > c0 + i) {
+      T v = t[j][i];
+      v.y = -v.y;
+      C[(long long)(c0 + i) * rs + (long long)(r0 + j) * cs] = v;
+    }
+  }
+}
+
+template <typename T>
+static hipError_t herm_mirror(const GemmArgs& g, hipStream_t s) {
+  const int T32 = (g.N + 31) / 32;
+  dim3 grid(T32 * (T32 + 1) / 2, g.batch);
+  hipLaunchKernelGGL(herm_mirror_kernel<T>, grid, dim3(256), 0, s, (T*)g.C, g.N, g.rsC, g.csC, g.bsC);
+  return hipGetLastError();
+}
+
 template <typename TIn, typename TA, typename TOut, int BM, int BN, int BK>
-static hipError_t launch(const GemmArgs& g, hipStream_t s) {
-  if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return hipSuccess;
-  int tiles = ((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
+static hipError_t launch(const GemmArgs& g0, hipStream_t s) {
+  if (g0.M <= 0 || g0.N <= 0 || g0.batch <= 0) return hipSuccess;
+  GemmArgs g = g0;
+  g.herm = (g0.herm && BM == BN) ? 1 : 0;
+  const int tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
+  const int tiles = g.herm ? tm * (tm + 1) / 2 : tm * tn;
   dim3 grid(tiles, 1, g.batch);
   hipLaunchKernelGGL((gemm_kernel<TIn, TA, TOut, BM, BN, BK>), grid, dim3(256), 0, s, g);
-  return hipGetLastError();
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !g0.herm) return e;
+  return herm_mirror<TOut>(g0, s);
 }
 
 // Tile shape of the fp64-accumulating GEMMs (BM x BN x BK): 0 = 64x64x16, 1 = 64x64x32,
@@ -206,11 +261,15 @@ hipError_t gemm(Prec p, const GemmArgs& g, hipStream_t s) {
     return hipErrorInvalidValue;
   }
   if (g.batch > 65535) return hipErrorInvalidValue;
+  if (g.herm && g.M != g.N) return hipErrorInvalidValue;
   const double inb = (p == C128_F64ACC || p == C128_IN_C64_OUT) ? 16 : 8;
   const double outb = (p == C64_F64ACC || p == C128_F64ACC) ? 16 : 8;
   const double mn = (double)g.M * g.N, nb = g.batch;
-  const int shape[5] = {g.M, g.N, g.K, g.batch, (int)p};
-  prof::Scope ps(prof::GEMM, s, 8.0 * mn * g.K * nb,
+  // a Hermitian output computes the 64 x 64 tiles on and below the diagonal only
+  const double tt = (g.M + 63) / 64;
+  const double mn_work = g.herm ? std::min(mn, 0.5 * tt * (tt + 1) * 4096.0) : mn;
+  const int shape[5] = {g.M, g.N, g.K, g.batch, (int)p + (g.herm ? 10 : 0)};
+  prof::Scope ps(prof::GEMM, s, 8.0 * mn_work * g.K * nb,
                  nb * (inb * ((double)g.M * g.K + (double)g.K * g.N) +
                        outb * mn * ((g.beta_re != 0 || g.beta_im != 0) ? 2 : 1)),
                  shape);

@@ -120,6 +120,12 @@ hipError_t k_gemm(Prec p, const GemmArgs& g, hipStream_t s, int line) {
       !fits(g.B, ext(g.batch, g.bsB, g.K, g.rsB, g.N, g.csB), in_bytes(p), "GEMM operand B", line) ||
       !fits(g.C, ext(g.batch, g.bsC, g.M, g.rsC, g.N, g.csC), out_bytes(p), "GEMM result C", line))
     return hipErrorInvalidValue;
+  if (g.herm && g.M != g.N) {
+    char b[128];
+    snprintf(b, sizeof b, "layer.hip:%d Hermitian GEMM output is not square", line);
+    t_oob = b;
+    return hipErrorInvalidValue;
+  }
   if (t_dry) {
     t_gemms.insert(t_gemms.end(), {line, g.M, g.N, g.K, g.batch, (int)p});
     return hipSuccess;
@@ -312,8 +318,9 @@ struct Carver {
 // ------------------------------------------------------------------ batched helpers
 hipError_t mmb_(int line, Prec p, int M, int N, int K, const void* A, long long lda, long long bsA, const void* B,
                long long ldb, long long bsB, void* C, long long ldc, long long bsC, int batch, hipStream_t s,
-               bool conjTransA = false) {
+               bool conjTransA = false, bool herm = false) {
   GemmArgs g{};
+  g.herm = herm;
   g.M = M; g.N = N; g.K = K; g.batch = batch;
   g.A = A; g.rsA = conjTransA ? 1 : lda; g.csA = conjTransA ? lda : 1; g.bsA = bsA; g.conjA = conjTransA;
   g.B = B; g.rsB = ldb; g.csB = 1; g.bsB = bsB;
@@ -330,6 +337,7 @@ hipError_t gram_cols_b_(int line, const float2* X, int rows, int n, long long bs
   g.B = X; g.rsB = n; g.csB = 1; g.bsB = bsX;
   g.C = G; g.rsC = n; g.csC = 1; g.bsC = (long long)n * n;
   g.alpha_re = 1;
+  g.herm = 1;
   return k_gemm(C64_F64ACC, g, s, line);
 }
 // G[b] (n x n, complex128) = X[b] X[b]^H, X (n x cols) row-major
@@ -340,6 +348,7 @@ hipError_t gram_rows_b_(int line, const float2* X, int n, int cols, long long bs
   g.B = X; g.rsB = 1; g.csB = cols; g.bsB = bsX; g.conjB = 1;
   g.C = G; g.rsC = n; g.csC = 1; g.bsC = (long long)n * n;
   g.alpha_re = 1;
+  g.herm = 1;
   return k_gemm(C64_F64ACC, g, s, line);
 }
 
@@ -363,6 +372,7 @@ hipError_t gram_cols_split_b_(int line, const float2* X, int rows, int n, double
   g.B = X; g.rsB = n; g.csB = 1; g.bsB = (long long)(rows / S) * n;
   g.C = parts; g.rsC = n; g.csC = 1; g.bsC = nn;
   g.alpha_re = 1;
+  g.herm = 1;
   hipError_t e = k_gemm(C64_F64ACC, g, s, line);
   if (e != hipSuccess) return e;
   if (!fits(parts, (long long)batch * S * nn, sizeof(double2), "split Gram partials", line) ||
@@ -1144,12 +1154,13 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
       g.B = Rs64; g.rsB = nT; g.csB = 1; g.bsB = (long long)ps.nc * nT;
       g.C = Nn; g.rsC = nT; g.csC = 1; g.bsC = (long long)nT * nT;
       g.alpha_re = 1;
+      g.herm = 1;
       LCK(K_GEMM(C128_F64ACC, g, s));
       LCK(k_kraus_dual(Nn, dG + (pi - 1) * 16 * K2, K2, p1t, Pg, B, s, __LINE__));
       LCK(k_widen(ps.Y, Yw, bsY, bsY, bsY, B, s, __LINE__));
       LCK(mmb(C128_F64ACC, n4, ps.p2, n4, Pg, n4, (long long)n4 * n4, Yw, ps.p2, bsY, Tw, ps.p2, bsY, B, s));
       LCK(mmb(C128_F64ACC, ps.p2, ps.p2, n4, Yw, ps.p2, bsY, Tw, ps.p2, bsY, Om64, ps.p2, (long long)ps.p2 * ps.p2, B, s,
-              true));
+              true, true));
     }
     if (ps.q2id) {
       LCK(k_bds64(Om64, a2, r, GR, (long long)ps.p2 * ps.p2, (long long)r * r, B, s, __LINE__));
@@ -1165,6 +1176,7 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
       g.B = Z64; g.rsB = r; g.csB = 1; g.bsB = qz;
       g.C = GR; g.rsC = r; g.csC = 1; g.bsC = (long long)r * r;
       g.alpha_re = 1;
+      g.herm = 1;
       LCK(K_GEMM(C128_F64ACC, g, s));
     }
     {
@@ -1378,6 +1390,7 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
       g.B = Yw; g.rsB = 1; g.csB = ps.p2; g.bsB = bsY; g.conjB = 1;
       g.C = Pg; g.rsC = n4; g.csC = 1; g.bsC = (long long)n4 * n4;
       g.alpha_re = 1;
+      g.herm = 1;
       LCK(K_GEMM(C128_F64ACC, g, s));
       LCK(k_kraus_gram(Pg, dG + (size_t)(pidx + 1) * 16 * K2, K2, p1t, Nn, B, s, __LINE__));
       LCK(mmb(C128_F64ACC, nC, nT, nT, Rs64, nT, bsRs, Nn, nT, (long long)nT * nT, Zr, nT, bsRs, B, s));
@@ -1387,6 +1400,7 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
       h.B = Rs64; h.rsB = 1; h.csB = nT; h.bsB = bsRs; h.conjB = 1;
       h.C = G; h.rsC = nC; h.csC = 1; h.bsC = (long long)nC * nC;
       h.alpha_re = 1;
+      h.herm = 1;
       LCK(K_GEMM(C128_F64ACC, h, s));
     }
     LST(eig_b(nC, B, G, eE, s, true));

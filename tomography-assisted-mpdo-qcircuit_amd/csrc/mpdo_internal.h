@@ -16,6 +16,10 @@ struct GemmArgs {
   const void* B; long long rsB, csB, bsB; int conjB;
   void* C; long long rsC, csC, bsC;
   double alpha_re, alpha_im, beta_re, beta_im;
+  // herm: C is Hermitian (M == N, the caller's product is Hermitian in exact arithmetic): only
+  // the tiles on and below the diagonal are computed, then the strict upper triangle is set to
+  // the conjugate transpose of the lower one (C exactly Hermitian)
+  int herm;
 };
 
 enum Prec { C64_F32ACC = 0,   // complex64 in, fp32 MFMA accumulate, complex64 out

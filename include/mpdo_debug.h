@@ -59,6 +59,11 @@ int mpdo_debug_layer_gemm_log(int* out, int cap);
  * values, returns the number of pairs. */
 int mpdo_debug_layer_kept(int* out, int cap);
 
+/* Recording of the gate-time cut spectra of chain 0 in the fused layers that follow on this
+ * thread (record = 1 on, 0 off: each pair then costs one extra readback); copies the last layer's
+ * recorded eigenvalues (per pair its 2 p1 values, descending) into out and returns their count. */
+int mpdo_debug_layer_gate_eigs(int record, double* out, int cap);
+
 /* Launch timing by kernel class (0 complex GEMM, 1 block-Jacobi subproblem solve, 2 block-Jacobi
  * rotation update, 3 single-workgroup Jacobi, 4 permutation): HIP events recorded on each
  * launch's own stream. enable(1) resets the totals and starts recording, enable(0) stops;

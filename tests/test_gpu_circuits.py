@@ -114,6 +114,35 @@ def test_realnoise_ghz(cuda):
     assert _rho_err(rho, O.density_matrix(st)) < 1e-5
 
 
+def test_realnoise_builder_rotations(cuda):
+    """The rest of the builder under realNoise (AbstractCircuit.py:231-342): rxx/ryy/rzz/cx/cnot
+    expanded into CX/CZEXP lists with the tomographed czDefault channel, measured-basis
+    rotations, chi and kappa truncating; density matrix vs the oracle evolving the oracle
+    builder's list."""
+    import mpdo_amd as M
+    n = 4
+    cz = os.path.join(HERE, 'golden', 'czDefault.mat')
+    files = {'CZ': {f'{i}{i + 1}': cz for i in range(n - 1)}, 'CP': {}}
+    c = M.TensorCircuit(n, ideal=False, noiseType='realNoise', chiFileDict=files, chi=6, kappa=3, chip='best',
+                        device=str(cuda))
+    exp = {f'{i}{i + 1}': O.chi_to_kraus(O.load_chi(cz)) for i in range(n - 1)}
+    b = O.CircuitBuilder(n, 'realNoise', exp)
+    for q in range(n):
+        c.h(q); b.add('H', [q])
+    c.rxx(torch.tensor(0.4), 0, 1); b.add('RXX', [0, 1], [0.4])
+    c.ryy(torch.tensor(-0.7), 2, 3); b.add('RYY', [2, 3], [-0.7])
+    c.truncate(); b.truncate()
+    c.rzz(torch.tensor(1.3), 1, 2); b.add('RZZ', [1, 2], [1.3])
+    c.cx(3, 2); b.add('CX', [3, 2])
+    c.truncate(); b.truncate()
+    state = M.Tools.create_ket0Series(n, device=str(cuda))
+    c.evolve(state)
+    rho = c.cal_dm().cpu().numpy()
+    st = O.product_state(n)
+    O.evolve(b.ops, st, 'realNoise', False, O.NoiseSpec(chip='best'), chi=6, kappa=3)
+    assert _rho_err(rho, O.density_matrix(st)) < 1e-5
+
+
 def test_config0_golden(cuda):
     """8 qubits, 4 brickwork layers, dep1 p=1e-3, dep2 p=1e-2, chi=32, kappa=4; the
     full 256x256 density matrix against the oracle fixture (tests/golden/make_golden.py)."""

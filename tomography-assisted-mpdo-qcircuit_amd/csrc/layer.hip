@@ -81,6 +81,9 @@ thread_local std::string t_oob;
 thread_local bool t_dry = false;
 thread_local size_t t_ws_bytes = 0;   // workspace size of the last planned layer
 thread_local std::vector<int> t_kept;   // per pair of the last layer: max kept count of its gate-time split
+// gate-time cut spectra of chain 0 (per pair: nT eigenvalues, descending), recorded on request
+thread_local bool t_rec_gate = false;
+thread_local std::vector<double> t_gate_eigs;
 thread_local std::vector<int> t_gemms;  // dry run: (source line, M, N, K, batch, precision) of every GEMM
 
 void add_span(const void* p, size_t bytes) {
@@ -1018,6 +1021,7 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
   };
 
   t_kept.clear();
+  t_gate_eigs.clear();
   // ---------------- phase 2: QR sweep left to right ----------------
   // Rcur: [B][kp][l] with kp the QR bond so far (1 at the left end)
   int kcur = 1;
@@ -1097,6 +1101,12 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
       LCK(gram_rows_b(ps.Y, 4 * p1t, ps.p2, 4LL * p1t * ps.p2, Pg, B, s));
       LCK(k_kraus_gram(Pg, dG + (pi - 1) * 16 * K2, K2, p1t, G, B, s, __LINE__));
       LST(eig_b(nT, B, G, eE, s, true));
+      if (t_rec_gate && !dry) {
+        const size_t o = t_gate_eigs.size();
+        t_gate_eigs.resize(o + nT);
+        LCK(hipMemcpyAsync(t_gate_eigs.data() + o, eE.w, sizeof(double) * nT, hipMemcpyDeviceToHost, s));
+        LCK(hipStreamSynchronize(s));
+      }
       LCK(K_KEEP(eE.w, nT, nT, len_gate, -1, GATE_EPS, 0, dk, ddisc + (size_t)B * (nsteps + n + i), B, s));
       // the projector onto the kept directions is applied only when some chain drops one; the
       // core then lives in the mq-dimensional span of the kept directions (mq = max over chains)
@@ -1514,6 +1524,13 @@ mpdo_status mpdo_layer_batched(int batch, int n, const mpdo_site* in, const floa
 int mpdo_debug_layer_gemm_log(int* out, int cap) {
   const int n = (int)t_gemms.size() / 6;
   for (int i = 0; i < 6 * n && i < cap; ++i) out[i] = t_gemms[i];
+  return n;
+}
+
+int mpdo_debug_layer_gate_eigs(int record, double* out, int cap) {
+  t_rec_gate = record != 0;
+  const int n = (int)t_gate_eigs.size();
+  for (int i = 0; i < n && i < cap; ++i) out[i] = t_gate_eigs[i];
   return n;
 }
 

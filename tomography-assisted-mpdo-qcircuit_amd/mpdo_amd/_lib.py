@@ -69,6 +69,7 @@ def load() -> ctypes.CDLL:
         'mpdo_debug_layer_dry_run': (I, [I, I, SP, I, I, I, I, I, SP, P(ctypes.c_ulonglong)]),
         'mpdo_debug_prof_enable': (I, [I]),
         'mpdo_debug_layer_kept': (I, [P(I), I]),
+        'mpdo_debug_layer_gate_eigs': (I, [I, P(D), I]),
         'mpdo_debug_layer_gemm_log': (I, [P(I), I]),
         'mpdo_debug_gemm_tiles': (I, [I]),
         'mpdo_debug_bj_threads': (I, [I]),

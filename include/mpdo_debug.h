@@ -24,6 +24,15 @@ mpdo_status mpdo_debug_gemm(int prec, int M, int N, int K, int batch,
                             void* C, long long rsC, long long csC, long long bsC,
                             double alpha_re, double alpha_im, double beta_re, double beta_im, void* stream);
 
+/* mpdo_debug_gemm with each setting explicit: herm = 1 for a Hermitian output (as prec + 10
+ * above); inA / inB the storage of each operand (0 = as prec says, 1 = complex64, 2 = complex128;
+ * a complex128 operand needs an fp64-accumulating prec). */
+mpdo_status mpdo_debug_gemm_ex(int prec, int herm, int inA, int inB, int M, int N, int K, int batch,
+                               const void* A, long long rsA, long long csA, long long bsA, int conjA,
+                               const void* B, long long rsB, long long csB, long long bsB, int conjB,
+                               void* C, long long rsC, long long csC, long long bsC,
+                               double alpha_re, double alpha_im, double beta_re, double beta_im, void* stream);
+
 /* Eigen-decomposition of `batch` Hermitian complex128 n x n matrices (row-major, contiguous,
  * destroyed): w[b][n] descending, V[b][n][n] with eigenvectors in columns. *sweeps = sweeps used. */
 mpdo_status mpdo_debug_eigh(int n, int batch, void* A, double* w, void* V, int* sweeps, void* stream);

@@ -65,6 +65,39 @@ def test_gemm_hermitian(cuda, prec, n, K):
                           got * np.eye(n)), 'not exactly Hermitian off the diagonal'
 
 
+@pytest.mark.parametrize('prec', [1, 2, 3, 4])
+@pytest.mark.parametrize('inA,inB', [(1, 2), (2, 1)])
+@pytest.mark.parametrize('M,N,K,tA,tB', [(70, 33, 130, False, False), (64, 128, 16, True, False),
+                                         (129, 65, 64, False, True), (5, 300, 17, True, True)])
+def test_gemm_mixed_operands(cuda, prec, inA, inB, M, N, K, tA, tB):
+    """complex64 x complex128 operands (each in its own storage, widened in the staging) with
+    fp64 accumulation, any orientation, conjugation and alpha/beta; vs numpy in complex128."""
+    from mpdo_amd import _lib
+    rng = np.random.default_rng(M + 7 * N + 13 * K + inA)
+    batch = 2
+    tya = np.complex64 if inA == 1 else np.complex128
+    tyb = np.complex64 if inB == 1 else np.complex128
+    A = _crand(rng, batch, *((K, M) if tA else (M, K))).astype(tya)
+    B = _crand(rng, batch, *((N, K) if tB else (K, N))).astype(tyb)
+    C0 = _crand(rng, batch, M, N)
+    out_t = torch.complex64 if prec in (3, 4) else torch.complex128
+    Cd = torch.from_numpy(C0).to(out_t).to(cuda)
+    sa = (1, M, M * K) if tA else (K, 1, M * K)
+    sb = (1, K, K * N) if tB else (N, 1, K * N)
+    alpha, beta = 0.7 - 0.2j, 0.3 + 0.5j
+    _lib.debug_gemm_ex(prec, False, inA, inB, torch.from_numpy(A).to(cuda), torch.from_numpy(B).to(cuda), Cd, M, N, K,
+                       sa, sb, (N, 1, M * N), tA, not tA, alpha, beta, batch)
+    opA = np.swapaxes(A, 1, 2) if tA else A
+    opB = np.swapaxes(B, 1, 2) if tB else B
+    opA = opA.conj() if tA else opA
+    opB = opB.conj() if not tA else opB
+    Cref = C0.astype(np.complex64 if prec in (3, 4) else np.complex128).astype(np.complex128)
+    ref = alpha * (opA.astype(np.complex128) @ opB.astype(np.complex128)) + beta * Cref
+    got = Cd.cpu().numpy()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err < (1e-13 if prec in (1, 2) else 1e-6), (prec, inA, inB, M, N, K, err)
+
+
 @pytest.mark.parametrize('n', [1, 2, 3, 17, 64, 65, 100, 192])
 def test_eigh(cuda, n):
     from mpdo_amd import _lib

@@ -930,16 +930,17 @@ mpdo_status mpdo_contract_state(int n, const mpdo_site* sites, void* out, void* 
 }
 
 // ------------------------------------------------------------------------------ test hooks
-mpdo_status mpdo_debug_gemm(int prec, int M, int N, int K, int batch, const void* A, long long rsA, long long csA,
-                            long long bsA, int conjA, const void* B, long long rsB, long long csB, long long bsB,
-                            int conjB, void* C, long long rsC, long long csC, long long bsC, double alpha_re,
-                            double alpha_im, double beta_re, double beta_im, void* stream) {
-  const int herm = prec >= 10;
-  if (herm) prec -= 10;
+mpdo_status mpdo_debug_gemm_ex(int prec, int herm, int inA, int inB, int M, int N, int K, int batch, const void* A,
+                               long long rsA, long long csA, long long bsA, int conjA, const void* B, long long rsB,
+                               long long csB, long long bsB, int conjB, void* C, long long rsC, long long csC,
+                               long long bsC, double alpha_re, double alpha_im, double beta_re, double beta_im,
+                               void* stream) {
   if (prec < 0 || prec > 4) return fail(MPDO_EINVAL, "prec %d", prec);
   if (herm && M != N) return fail(MPDO_EINVAL, "Hermitian output needs M == N (%d, %d)", M, N);
+  if (inA < 0 || inA > 2 || inB < 0 || inB > 2) return fail(MPDO_EINVAL, "operand storage %d %d", inA, inB);
   GemmArgs g{};
-  g.herm = herm;
+  g.herm = herm != 0;
+  g.inA = inA; g.inB = inB;
   g.M = M; g.N = N; g.K = K; g.batch = batch;
   g.A = A; g.rsA = rsA; g.csA = csA; g.bsA = bsA; g.conjA = conjA;
   g.B = B; g.rsB = rsB; g.csB = csB; g.bsB = bsB; g.conjB = conjB;
@@ -948,6 +949,15 @@ mpdo_status mpdo_debug_gemm(int prec, int M, int N, int K, int batch, const void
   CK(gemm((Prec)prec, g, S(stream)));
   CK(hipStreamSynchronize(S(stream)));
   return MPDO_OK;
+}
+
+mpdo_status mpdo_debug_gemm(int prec, int M, int N, int K, int batch, const void* A, long long rsA, long long csA,
+                            long long bsA, int conjA, const void* B, long long rsB, long long csB, long long bsB,
+                            int conjB, void* C, long long rsC, long long csC, long long bsC, double alpha_re,
+                            double alpha_im, double beta_re, double beta_im, void* stream) {
+  const int herm = prec >= 10;
+  return mpdo_debug_gemm_ex(herm ? prec - 10 : prec, herm, 0, 0, M, N, K, batch, A, rsA, csA, bsA, conjA, B, rsB, csB,
+                            bsB, conjB, C, rsC, csC, bsC, alpha_re, alpha_im, beta_re, beta_im, stream);
 }
 
 mpdo_status mpdo_debug_eigh(int n, int batch, void* A, double* w, void* V, int* sweeps, void* stream) {

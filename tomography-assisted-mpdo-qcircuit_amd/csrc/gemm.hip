@@ -14,6 +14,7 @@
 #include "prof.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace mpdo {
 
@@ -46,8 +47,9 @@ template <> __device__ inline double2 cvt<float2, double2>(float2 v) { return ma
 template <> __device__ inline double2 cvt<double2, double2>(double2 v) { return v; }
 template <> __device__ inline float2 cvt<double2, float2>(double2 v) { return make_float2((float)v.x, (float)v.y); }
 
-// TIn: operand storage type; TA: accumulation scalar; TOut: output storage type.
-template <typename TIn, typename TA, typename TOut, int BM, int BN, int BK>
+// TInA, TInB: operand storage types (float2 or double2, independently); TA: accumulation
+// scalar; TOut: output storage type.
+template <typename TInA, typename TInB, typename TA, typename TOut, int BM, int BN, int BK>
 __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
   typedef typename Acc<TA>::V V;
   typedef typename Acc<TA>::C CA;   // complex in accumulation precision (LDS element)
@@ -75,8 +77,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
     m0 = (blockIdx.x / ntn) * BM;
     n0 = (blockIdx.x % ntn) * BN;
   }
-  const TIn* A = (const TIn*)g.A + (long long)b * g.bsA;
-  const TIn* B = (const TIn*)g.B + (long long)b * g.bsB;
+  const TInA* A = (const TInA*)g.A + (long long)b * g.bsA;
+  const TInB* B = (const TInB*)g.B + (long long)b * g.bsB;
   const bool aK = (g.csA == 1);   // A contiguous along k
   const bool bK = (g.rsB == 1);   // B contiguous along k
   const float sA = g.conjA ? -1.f : 1.f, sB = g.conjB ? -1.f : 1.f;
@@ -89,7 +91,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) { accr[i][j][q] = 0; acci[i][j][q] = 0; }
 
-  TIn ra[PA], rb[PB];
+  TInA ra[PA];
+  TInB rb[PB];
   auto load = [&](int k0) {
 #pragma unroll
     for (int i = 0; i < PA; ++i) {
@@ -97,7 +100,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
       int row = aK ? e / BK : e % BM;
       int kk = aK ? e % BK : e / BM;
       int gm = m0 + row, gk = k0 + kk;
-      TIn v;
+      TInA v;
       v.x = 0; v.y = 0;
       if (gm < g.M && gk < g.K) v = A[(long long)gm * g.rsA + (long long)gk * g.csA];
       ra[i] = v;
@@ -108,7 +111,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
       int col = bK ? e / BK : e % BN;
       int kk = bK ? e % BK : e / BN;
       int gn = n0 + col, gk = k0 + kk;
-      TIn v;
+      TInB v;
       v.x = 0; v.y = 0;
       if (gn < g.N && gk < g.K) v = B[(long long)gk * g.rsB + (long long)gn * g.csB];
       rb[i] = v;
@@ -120,7 +123,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
       int e = aK ? tid * PA + i : i * 256 + tid;
       int row = aK ? e / BK : e % BM;
       int kk = aK ? e % BK : e / BM;
-      CA v = cvt<TIn, CA>(ra[i]);
+      CA v = cvt<TInA, CA>(ra[i]);
       v.y *= sA;
       As[kk][row] = v;
     }
@@ -129,7 +132,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
       int e = bK ? tid * PB + i : i * 256 + tid;
       int col = bK ? e / BK : e % BN;
       int kk = bK ? e % BK : e / BN;
-      CA v = cvt<TIn, CA>(rb[i]);
+      CA v = cvt<TInB, CA>(rb[i]);
       v.y *= sB;
       Bs[kk][col] = v;
     }
@@ -225,7 +228,7 @@ static hipError_t herm_mirror(const GemmArgs& g, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <typename TIn, typename TA, typename TOut, int BM, int BN, int BK>
+template <typename TInA, typename TInB, typename TA, typename TOut, int BM, int BN, int BK>
 static hipError_t launch(const GemmArgs& g0, hipStream_t s) {
   if (g0.M <= 0 || g0.N <= 0 || g0.batch <= 0) return hipSuccess;
   GemmArgs g = g0;
@@ -233,7 +236,7 @@ static hipError_t launch(const GemmArgs& g0, hipStream_t s) {
   const int tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
   const int tiles = g.herm ? tm * (tm + 1) / 2 : tm * tn;
   dim3 grid(tiles, 1, g.batch);
-  hipLaunchKernelGGL((gemm_kernel<TIn, TA, TOut, BM, BN, BK>), grid, dim3(256), 0, s, g);
+  hipLaunchKernelGGL((gemm_kernel<TInA, TInB, TA, TOut, BM, BN, BK>), grid, dim3(256), 0, s, g);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !g0.herm) return e;
   return herm_mirror<TOut>(g0, s);
@@ -243,15 +246,28 @@ static hipError_t launch(const GemmArgs& g0, hipStream_t s) {
 // 2 = 128x64x16, 3 = 64x128x16, 4 = 128x128x16.
 int g_tile_variant = 0;
 
-template <typename TIn, typename TOut>
+template <typename TInA, typename TInB, typename TOut>
 static hipError_t launch_f64(const GemmArgs& g, hipStream_t s) {
-  switch (g_tile_variant) {
-    case 1: return launch<TIn, double, TOut, 64, 64, 32>(g, s);
-    case 2: return launch<TIn, double, TOut, 128, 64, 16>(g, s);
-    case 3: return launch<TIn, double, TOut, 64, 128, 16>(g, s);
-    case 4: return launch<TIn, double, TOut, 128, 128, 16>(g, s);
-    default: return launch<TIn, double, TOut, 64, 64, 16>(g, s);
+  if constexpr (std::is_same<TInA, TInB>::value) {
+    switch (g_tile_variant) {
+      case 1: return launch<TInA, TInB, double, TOut, 64, 64, 32>(g, s);
+      case 2: return launch<TInA, TInB, double, TOut, 128, 64, 16>(g, s);
+      case 3: return launch<TInA, TInB, double, TOut, 64, 128, 16>(g, s);
+      case 4: return launch<TInA, TInB, double, TOut, 128, 128, 16>(g, s);
+      default: break;
+    }
   }
+  return launch<TInA, TInB, double, TOut, 64, 64, 16>(g, s);
+}
+
+// operand types of the fp64-accumulating paths: complex128 when the precision mode says so,
+// unless the operand's own flag (GemmArgs::inA / inB) overrides it
+template <typename TOut>
+static hipError_t launch_f64_mixed(bool a128, bool b128, const GemmArgs& g, hipStream_t s) {
+  if (a128 && b128) return launch_f64<double2, double2, TOut>(g, s);
+  if (a128) return launch_f64<double2, float2, TOut>(g, s);
+  if (b128) return launch_f64<float2, double2, TOut>(g, s);
+  return launch_f64<float2, float2, TOut>(g, s);
 }
 
 hipError_t gemm(Prec p, const GemmArgs& g, hipStream_t s) {
@@ -262,7 +278,11 @@ hipError_t gemm(Prec p, const GemmArgs& g, hipStream_t s) {
   }
   if (g.batch > 65535) return hipErrorInvalidValue;
   if (g.herm && g.M != g.N) return hipErrorInvalidValue;
-  const double inb = (p == C128_F64ACC || p == C128_IN_C64_OUT) ? 16 : 8;
+  if (g.inA < 0 || g.inA > 2 || g.inB < 0 || g.inB > 2) return hipErrorInvalidValue;
+  const bool in128 = (p == C128_F64ACC || p == C128_IN_C64_OUT);
+  const bool a128 = g.inA ? g.inA == 2 : in128, b128 = g.inB ? g.inB == 2 : in128;
+  if (p == C64_F32ACC && (a128 || b128)) return hipErrorInvalidValue;
+  const double inbA = a128 ? 16 : 8, inbB = b128 ? 16 : 8;
   const double outb = (p == C64_F64ACC || p == C128_F64ACC) ? 16 : 8;
   const double mn = (double)g.M * g.N, nb = g.batch;
   // a Hermitian output computes the 64 x 64 tiles on and below the diagonal only
@@ -270,15 +290,15 @@ hipError_t gemm(Prec p, const GemmArgs& g, hipStream_t s) {
   const double mn_work = g.herm ? std::min(mn, 0.5 * tt * (tt + 1) * 4096.0) : mn;
   const int shape[5] = {g.M, g.N, g.K, g.batch, (int)p + (g.herm ? 10 : 0)};
   prof::Scope ps(prof::GEMM, s, 8.0 * mn_work * g.K * nb,
-                 nb * (inb * ((double)g.M * g.K + (double)g.K * g.N) +
+                 nb * (inbA * (double)g.M * g.K + inbB * (double)g.K * g.N +
                        outb * mn * ((g.beta_re != 0 || g.beta_im != 0) ? 2 : 1)),
                  shape);
   switch (p) {
-    case C64_F32ACC: return launch<float2, float, float2, 128, 128, 16>(g, s);
-    case C64_F64ACC: return launch_f64<float2, double2>(g, s);
-    case C128_F64ACC: return launch_f64<double2, double2>(g, s);
-    case C128_IN_C64_OUT: return launch_f64<double2, float2>(g, s);
-    case C64_F64ACC_C64OUT: return launch_f64<float2, float2>(g, s);
+    case C64_F32ACC: return launch<float2, float2, float, float2, 128, 128, 16>(g, s);
+    case C64_F64ACC:
+    case C128_F64ACC: return launch_f64_mixed<double2>(a128, b128, g, s);
+    case C128_IN_C64_OUT:
+    case C64_F64ACC_C64OUT: return launch_f64_mixed<float2>(a128, b128, g, s);
   }
   return hipErrorInvalidValue;
 }

@@ -118,9 +118,11 @@ long long ext(int batch, long long bs, long long rows, long long rs, long long c
 size_t in_bytes(Prec p) { return (p == C128_F64ACC || p == C128_IN_C64_OUT) ? 16 : 8; }
 size_t out_bytes(Prec p) { return (p == C64_F64ACC || p == C128_F64ACC) ? 16 : 8; }
 
+size_t op_bytes(Prec p, int flag) { return flag ? (flag == 2 ? 16 : 8) : in_bytes(p); }
+
 hipError_t k_gemm(Prec p, const GemmArgs& g, hipStream_t s, int line) {
-  if (!fits(g.A, ext(g.batch, g.bsA, g.M, g.rsA, g.K, g.csA), in_bytes(p), "GEMM operand A", line) ||
-      !fits(g.B, ext(g.batch, g.bsB, g.K, g.rsB, g.N, g.csB), in_bytes(p), "GEMM operand B", line) ||
+  if (!fits(g.A, ext(g.batch, g.bsA, g.M, g.rsA, g.K, g.csA), op_bytes(p, g.inA), "GEMM operand A", line) ||
+      !fits(g.B, ext(g.batch, g.bsB, g.K, g.rsB, g.N, g.csB), op_bytes(p, g.inB), "GEMM operand B", line) ||
       !fits(g.C, ext(g.batch, g.bsC, g.M, g.rsC, g.N, g.csC), out_bytes(p), "GEMM result C", line))
     return hipErrorInvalidValue;
   if (g.herm && g.M != g.N) {
@@ -332,6 +334,20 @@ hipError_t mmb_(int line, Prec p, int M, int N, int K, const void* A, long long 
   return k_gemm(p, g, s, line);
 }
 #define mmb(...) mmb_(__LINE__, __VA_ARGS__)
+// the same with the storage of each operand given (1 = complex64, 2 = complex128), row-major
+hipError_t mmx_(int line, Prec p, int M, int N, int K, const void* A, long long lda, long long bsA, int inA,
+                const void* B, long long ldb, long long bsB, int inB, void* C, long long ldc, long long bsC, int batch,
+                hipStream_t s, bool conjTransA = false, bool herm = false) {
+  GemmArgs g{};
+  g.M = M; g.N = N; g.K = K; g.batch = batch;
+  g.A = A; g.rsA = conjTransA ? 1 : lda; g.csA = conjTransA ? lda : 1; g.bsA = bsA; g.conjA = conjTransA;
+  g.B = B; g.rsB = ldb; g.csB = 1; g.bsB = bsB;
+  g.C = C; g.rsC = ldc; g.csC = 1; g.bsC = bsC;
+  g.alpha_re = 1;
+  g.inA = inA; g.inB = inB; g.herm = herm;
+  return k_gemm(p, g, s, line);
+}
+#define mmx(...) mmx_(__LINE__, __VA_ARGS__)
 // G[b] (n x n, complex128) = X[b]^H X[b], X (rows x n) row-major
 hipError_t gram_cols_b_(int line, const float2* X, int rows, int n, long long bsX, double2* G, int batch, hipStream_t s) {
   GemmArgs g{};
@@ -436,7 +452,7 @@ mpdo_status eig_b_(int line, int n, int batch, double2* G, EigB& e, hipStream_t 
 // Directions whose pivot falls below FLOOR of the largest diagonal get zero columns of Q and zero
 // rows of R, as the zero-filled null directions of the reference's QR.
 struct QRWS {
-  double2 *G, *L1, *L2, *F, *W;
+  double2 *G, *L1, *L2, *F;
   int *piv1, *piv2, *rank1, *rank2;
   double* cond2;
   void *pws, *iws;
@@ -455,7 +471,6 @@ void carve_qr(Carver& c, int rows, int cols, int batch, QRWS& w) {
   w.pws = c.get<char>(pchol_workspace_bytes(cols, batch));
   w.iws = c.get<char>(pchol_inverse_workspace_bytes(cols, batch));
   w.F = c.get<double2>(n * n * batch);
-  w.W = c.get<double2>((size_t)rows * cols * batch);
   w.A1 = c.get<float2>((size_t)rows * cols * batch);
 }
 hipError_t k_pchol(int n, int batch, double2* G, double tol, double2* L, int* piv, int* rank, QRWS& w, hipStream_t s,
@@ -510,20 +525,18 @@ mpdo_status qr_b(const float2* M, int rows, int cols, float2* Q, float2* Rf, int
   LCK(gram_cols_b(M, rows, cols, rc, w.G, batch, s));
   LCK(K_PCHOL(cols, batch, w.G, FLOOR, w.L1, w.piv1, w.rank1, w, s));
   LCK(K_PINV(cols, batch, w.L1, w.piv1, w.rank1, 0, w.F, w, s));
-  LCK(k_widen(M, w.W, rc, rc, rc, batch, s, __LINE__));
   bool one = false;
   LST(one_pass_enough(cols, batch, w, one, s, __LINE__));
   if (one) {   // Q = M X1, Rf = L1^H
-    LCK(mmb(C128_IN_C64_OUT, rows, cols, cols, w.W, cols, rc, w.F, cols, cc, Q, cols, rc, batch, s));
+    LCK(mmx(C128_IN_C64_OUT, rows, cols, cols, M, cols, rc, 1, w.F, cols, cc, 2, Q, cols, rc, batch, s));
     LCK(k_narrow_factor(w.L1, cols, 0, Rf, batch, s, __LINE__));
     return MPDO_OK;
   }
-  LCK(mmb(C128_IN_C64_OUT, rows, cols, cols, w.W, cols, rc, w.F, cols, cc, w.A1, cols, rc, batch, s));
+  LCK(mmx(C128_IN_C64_OUT, rows, cols, cols, M, cols, rc, 1, w.F, cols, cc, 2, w.A1, cols, rc, batch, s));
   LCK(gram_cols_b(w.A1, rows, cols, rc, w.G, batch, s));
   LCK(K_PCHOL(cols, batch, w.G, FLOOR, w.L2, w.piv2, w.rank2, w, s));
   LCK(K_PINV(cols, batch, w.L2, w.piv2, w.rank2, 0, w.F, w, s));
-  LCK(k_widen(w.A1, w.W, rc, rc, rc, batch, s, __LINE__));
-  LCK(mmb(C128_IN_C64_OUT, rows, cols, cols, w.W, cols, rc, w.F, cols, cc, Q, cols, rc, batch, s));
+  LCK(mmx(C128_IN_C64_OUT, rows, cols, cols, w.A1, cols, rc, 1, w.F, cols, cc, 2, Q, cols, rc, batch, s));
   // Rf[u][j] = sum_t conj(L2[t][u]) conj(L1[j][t]); L stored by columns: L[x][y] at L[y cols + x]
   GemmArgs g{};
   g.M = cols; g.N = cols; g.K = cols; g.batch = batch;
@@ -541,20 +554,18 @@ mpdo_status lq_b(const float2* X, int rows, int cols, float2* Qt, float2* Lf, in
   LCK(gram_rows_b(X, rows, cols, rc, w.G, batch, s));
   LCK(K_PCHOL(rows, batch, w.G, FLOOR, w.L1, w.piv1, w.rank1, w, s));
   LCK(K_PINV(rows, batch, w.L1, w.piv1, w.rank1, 1, w.F, w, s));
-  LCK(k_widen(X, w.W, rc, rc, rc, batch, s, __LINE__));
   bool one = false;
   LST(one_pass_enough(rows, batch, w, one, s, __LINE__));
   if (one) {   // Qt = X1 X, Lf = L1
-    LCK(mmb(C128_IN_C64_OUT, rows, cols, rows, w.F, rows, rr, w.W, cols, rc, Qt, cols, rc, batch, s));
+    LCK(mmx(C128_IN_C64_OUT, rows, cols, rows, w.F, rows, rr, 2, X, cols, rc, 1, Qt, cols, rc, batch, s));
     LCK(k_narrow_factor(w.L1, rows, 1, Lf, batch, s, __LINE__));
     return MPDO_OK;
   }
-  LCK(mmb(C128_IN_C64_OUT, rows, cols, rows, w.F, rows, rr, w.W, cols, rc, w.A1, cols, rc, batch, s));
+  LCK(mmx(C128_IN_C64_OUT, rows, cols, rows, w.F, rows, rr, 2, X, cols, rc, 1, w.A1, cols, rc, batch, s));
   LCK(gram_rows_b(w.A1, rows, cols, rc, w.G, batch, s));
   LCK(K_PCHOL(rows, batch, w.G, FLOOR, w.L2, w.piv2, w.rank2, w, s));
   LCK(K_PINV(rows, batch, w.L2, w.piv2, w.rank2, 1, w.F, w, s));
-  LCK(k_widen(w.A1, w.W, rc, rc, rc, batch, s, __LINE__));
-  LCK(mmb(C128_IN_C64_OUT, rows, cols, rows, w.F, rows, rr, w.W, cols, rc, Qt, cols, rc, batch, s));
+  LCK(mmx(C128_IN_C64_OUT, rows, cols, rows, w.F, rows, rr, 2, w.A1, cols, rc, 1, Qt, cols, rc, batch, s));
   // Lf[i][v] = sum_u L1[i][u] L2[u][v]; L stored by columns
   GemmArgs g{};
   g.M = rows; g.N = rows; g.K = rows; g.batch = batch;
@@ -830,7 +841,7 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
   Site1q* dtab = nullptr;
   float2 *Mlast = nullptr, *FLq, *Rcur, *Rnext, *RT, *X1, *X1t, *Q1t, *R1t, *Cs, *Cp, *Cu, *Zb, *Q2P, *FL, *FR, *Tmin, *Wm,
          *Tm, *Pn, *Pn2, *PW, *Pp, *Tchi, *Mp, *Vk, *Kout, *dG, *Tcur2, *dK;
-  double2 *G, *GR, *Om64, *Q2w, *Z64, *H64, *Gparts, *Pg, *Hk, *Yw, *Tw, *Rs64, *Zr, *Nn;
+  double2 *G, *GR, *Om64, *Z64, *H64, *Gparts, *Pg, *Hk, *Tw, *Rs64, *Zr, *Nn;
   float2 *Zm, *Vf, *Vt;
   double* ddisc;
   int* dk;
@@ -873,13 +884,11 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     Cs = c.get<float2>(mxC * B);
     Cp = c.get<float2>(mxCp * B);
     Cu = c.get<float2>(mxC * B);
-    Yw = c.get<double2>(mxY * B);
     Tw = c.get<double2>(mxY * B);
     Rs64 = c.get<double2>(mxRs * B);
     Zr = c.get<double2>(mxRs * B);
     Nn = c.get<double2>(mxNn * B);
     Om64 = c.get<double2>(mxOm * B);
-    Q2w = c.get<double2>(mxZ * B);
     Z64 = c.get<double2>(mxZ * B);
     Zb = c.get<float2>(std::max(mxZ, mxBond2) * B);
     dtab = c.get<Site1q>((size_t)n * B);
@@ -1167,22 +1176,20 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
       g.herm = 1;
       LCK(K_GEMM(C128_F64ACC, g, s));
       LCK(k_kraus_dual(Nn, dG + (pi - 1) * 16 * K2, K2, p1t, Pg, B, s, __LINE__));
-      LCK(k_widen(ps.Y, Yw, bsY, bsY, bsY, B, s, __LINE__));
-      LCK(mmb(C128_F64ACC, n4, ps.p2, n4, Pg, n4, (long long)n4 * n4, Yw, ps.p2, bsY, Tw, ps.p2, bsY, B, s));
-      LCK(mmb(C128_F64ACC, ps.p2, ps.p2, n4, Yw, ps.p2, bsY, Tw, ps.p2, bsY, Om64, ps.p2, (long long)ps.p2 * ps.p2, B, s,
-              true, true));
+      LCK(mmx(C128_F64ACC, n4, ps.p2, n4, Pg, n4, (long long)n4 * n4, 2, ps.Y, ps.p2, bsY, 1, Tw, ps.p2, bsY, B, s));
+      LCK(mmx(C128_F64ACC, ps.p2, ps.p2, n4, ps.Y, ps.p2, bsY, 1, Tw, ps.p2, bsY, 2, Om64, ps.p2,
+              (long long)ps.p2 * ps.p2, B, s, true, true));
     }
     if (ps.q2id) {
       LCK(k_bds64(Om64, a2, r, GR, (long long)ps.p2 * ps.p2, (long long)r * r, B, s, __LINE__));
     } else {
       const long long qz = (long long)ps.p2 * ps.cols2;
-      LCK(k_widen(ps.Q2, Q2w, qz, qz, qz, B, s, __LINE__));
-      LCK(mmb(C128_F64ACC, ps.p2, ps.cols2, ps.p2, Om64, ps.p2, (long long)ps.p2 * ps.p2, Q2w, ps.cols2, qz, Z64,
-              ps.cols2, qz, B, s));
+      LCK(mmx(C128_F64ACC, ps.p2, ps.cols2, ps.p2, Om64, ps.p2, (long long)ps.p2 * ps.p2, 2, ps.Q2, ps.cols2, qz, 1,
+              Z64, ps.cols2, qz, B, s));
       GemmArgs g{};
       const int rowsv = ps.p2 * a2;
       g.M = r; g.N = r; g.K = rowsv; g.batch = B;
-      g.A = Q2w; g.rsA = 1; g.csA = r; g.bsA = qz; g.conjA = 1;
+      g.A = ps.Q2; g.rsA = 1; g.csA = r; g.bsA = qz; g.conjA = 1; g.inA = 1;
       g.B = Z64; g.rsB = r; g.csB = 1; g.bsB = qz;
       g.C = GR; g.rsC = r; g.csC = 1; g.bsC = (long long)r * r;
       g.alpha_re = 1;
@@ -1391,13 +1398,12 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     {
       const int p1t = ps.p1t, nT = 2 * p1t, n4 = 4 * p1t;
       const long long bsY = (long long)n4 * ps.p2, bsRs = (long long)nC * nT;
-      LCK(k_widen(ps.Y, Yw, bsY, bsY, bsY, B, s, __LINE__));
-      LCK(mmb(C128_F64ACC, n4, ps.p2, ps.p2, Yw, ps.p2, bsY, H64, ps.p2, (long long)ps.p2 * ps.p2, Tw, ps.p2, bsY, B,
-              s));
+      LCK(mmx(C128_F64ACC, n4, ps.p2, ps.p2, ps.Y, ps.p2, bsY, 1, H64, ps.p2, (long long)ps.p2 * ps.p2, 2, Tw, ps.p2,
+              bsY, B, s));
       GemmArgs g{};
       g.M = n4; g.N = n4; g.K = ps.p2; g.batch = B;
       g.A = Tw; g.rsA = ps.p2; g.csA = 1; g.bsA = bsY;
-      g.B = Yw; g.rsB = 1; g.csB = ps.p2; g.bsB = bsY; g.conjB = 1;
+      g.B = ps.Y; g.rsB = 1; g.csB = ps.p2; g.bsB = bsY; g.conjB = 1; g.inB = 1;
       g.C = Pg; g.rsC = n4; g.csC = 1; g.bsC = (long long)n4 * n4;
       g.alpha_re = 1;
       g.herm = 1;

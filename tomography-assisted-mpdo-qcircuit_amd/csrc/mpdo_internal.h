@@ -20,6 +20,9 @@ struct GemmArgs {
   // the tiles on and below the diagonal are computed, then the strict upper triangle is set to
   // the conjugate transpose of the lower one (C exactly Hermitian)
   int herm;
+  // operand storage overriding the precision mode's input type (fp64-accumulating modes only):
+  // 0 = as the mode says, 1 = complex64, 2 = complex128
+  int inA, inB;
 };
 
 enum Prec { C64_F32ACC = 0,   // complex64 in, fp32 MFMA accumulate, complex64 out

@@ -64,6 +64,9 @@ def load() -> ctypes.CDLL:
                                 VP, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, I,
                                 VP, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong,
                                 D, D, D, D, VP]),
+        'mpdo_debug_gemm_ex': (I, [I, I, I, I, I, I, I, I, VP, ctypes.c_longlong, ctypes.c_longlong,
+                                   ctypes.c_longlong, I, VP, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong,
+                                   I, VP, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, D, D, D, D, VP]),
         'mpdo_debug_eigh': (I, [I, I, VP, P(D), VP, P(I), VP]),
         'mpdo_debug_eigh_gram': (I, [I, I, VP, P(D), VP, P(I), VP]),
         'mpdo_debug_layer_dry_run': (I, [I, I, SP, I, I, I, I, I, SP, P(ctypes.c_ulonglong)]),
@@ -418,6 +421,20 @@ def debug_gemm(prec: int, A: torch.Tensor, B: torch.Tensor, C: torch.Tensor, M: 
                              complex(alpha).real, complex(alpha).imag, complex(beta).real, complex(beta).imag,
                              _stream(A.device))
     check(st, 'mpdo_debug_gemm')
+
+
+def debug_gemm_ex(prec: int, herm: bool, inA: int, inB: int, A: torch.Tensor, B: torch.Tensor, C: torch.Tensor,
+                  M: int, N: int, K: int, sa: Tuple[int, int, int], sb: Tuple[int, int, int],
+                  sc: Tuple[int, int, int], conjA=False, conjB=False, alpha=1.0, beta=0.0, batch=1):
+    """debug_gemm with a Hermitian-output flag and per-operand storage (0 = as prec, 1 = complex64,
+    2 = complex128)."""
+    lib = load()
+    st = lib.mpdo_debug_gemm_ex(prec, int(herm), inA, inB, M, N, K, batch, ctypes.c_void_p(A.data_ptr()), sa[0], sa[1],
+                                sa[2], int(conjA), ctypes.c_void_p(B.data_ptr()), sb[0], sb[1], sb[2], int(conjB),
+                                ctypes.c_void_p(C.data_ptr()), sc[0], sc[1], sc[2],
+                                complex(alpha).real, complex(alpha).imag, complex(beta).real, complex(beta).imag,
+                                _stream(A.device))
+    check(st, 'mpdo_debug_gemm_ex')
 
 
 def debug_eigh(A: torch.Tensor, gram: bool = False):

@@ -24,6 +24,13 @@ mpdo_status mpdo_debug_gemm(int prec, int M, int N, int K, int batch,
                             void* C, long long rsC, long long csC, long long bsC,
                             double alpha_re, double alpha_im, double beta_re, double beta_im, void* stream);
 
+/* G[b] = X[b] X[b]^H (X: n x K complex64 row-major, G: n x n complex128) through the
+ * split-reduction path: `slices` partial products of K / slices each in one launch (only the
+ * tiles on and below the diagonal), written to parts (slices n^2 complex128 per batch entry),
+ * summed in order and mirrored. */
+mpdo_status mpdo_debug_gram_split(int n, int K, int batch, int slices, const void* X, void* parts, void* G,
+                                  void* stream);
+
 /* mpdo_debug_gemm with each setting explicit: herm = 1 for a Hermitian output (as prec + 10
  * above); inA / inB the storage of each operand (0 = as prec says, 1 = complex64, 2 = complex128;
  * a complex128 operand needs an fp64-accumulating prec). */

@@ -98,6 +98,22 @@ def test_gemm_mixed_operands(cuda, prec, inA, inB, M, N, K, tA, tB):
     assert err < (1e-13 if prec in (1, 2) else 1e-6), (prec, inA, inB, M, N, K, err)
 
 
+@pytest.mark.parametrize('slices', [1, 2, 4, 8])
+@pytest.mark.parametrize('n,K', [(64, 4096), (200, 8192), (256, 2048), (33, 64)])
+def test_gram_split_reduction(cuda, n, K, slices):
+    """Split-reduction Hermitian Gram: the slices' partial products (one launch, lower tiles
+    only), summed in order and mirrored, equal X X^H in complex128 and are exactly Hermitian."""
+    from mpdo_amd import _lib
+    rng = np.random.default_rng(n + K + slices)
+    X = _crand(rng, 3, n, K).astype(np.complex64)
+    G = _lib.debug_gram_split(torch.from_numpy(X).to(cuda), slices).cpu().numpy()
+    Xw = X.astype(np.complex128)
+    ref = Xw @ np.conj(np.swapaxes(Xw, 1, 2))
+    assert np.abs(G - ref).max() / np.abs(ref).max() < 1e-13
+    off = ~np.eye(n, dtype=bool)
+    assert np.array_equal(G[:, off], np.conj(np.swapaxes(G, 1, 2))[:, off])
+
+
 @pytest.mark.parametrize('n', [1, 2, 3, 17, 64, 65, 100, 192])
 def test_eigh(cuda, n):
     from mpdo_amd import _lib

@@ -951,6 +951,27 @@ mpdo_status mpdo_debug_gemm_ex(int prec, int herm, int inA, int inB, int M, int 
   return MPDO_OK;
 }
 
+mpdo_status mpdo_debug_gram_split(int n, int K, int batch, int slices, const void* X, void* parts, void* G,
+                                  void* stream) {
+  if (n <= 0 || K <= 0 || batch <= 0 || slices < 1 || slices > 64 || K % slices)
+    return fail(MPDO_EINVAL, "n=%d K=%d batch=%d slices=%d", n, K, batch, slices);
+  hipStream_t st = S(stream);
+  const long long nn = (long long)n * n;
+  GemmArgs g{};
+  g.M = n; g.N = n; g.K = K / slices; g.batch = batch;
+  g.A = X; g.rsA = K; g.csA = 1; g.bsA = (long long)n * K;
+  g.B = X; g.rsB = 1; g.csB = K; g.bsB = (long long)n * K; g.conjB = 1;
+  g.C = parts; g.rsC = n; g.csC = 1; g.bsC = slices * nn;
+  g.alpha_re = 1;
+  g.herm = 1;
+  g.ksplit = slices; g.ksA = K / slices; g.ksB = K / slices; g.ksC = nn;
+  CK(gemm(C64_F64ACC, g, st));
+  CK(sum_parts((const double2*)parts, slices, nn, (double2*)G, batch, st));
+  CK(herm_mirror_b(G, true, n, nn, batch, st));
+  CK(hipStreamSynchronize(st));
+  return MPDO_OK;
+}
+
 mpdo_status mpdo_debug_gemm(int prec, int M, int N, int K, int batch, const void* A, long long rsA, long long csA,
                             long long bsA, int conjA, const void* B, long long rsB, long long csB, long long bsB,
                             int conjB, void* C, long long rsC, long long csC, long long bsC, double alpha_re,

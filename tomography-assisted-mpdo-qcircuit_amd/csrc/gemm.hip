@@ -77,8 +77,9 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
     m0 = (blockIdx.x / ntn) * BM;
     n0 = (blockIdx.x % ntn) * BN;
   }
-  const TInA* A = (const TInA*)g.A + (long long)b * g.bsA;
-  const TInB* B = (const TInB*)g.B + (long long)b * g.bsB;
+  const int ks = blockIdx.y;   // reduction slice (0 without a split)
+  const TInA* A = (const TInA*)g.A + (long long)b * g.bsA + ks * g.ksA;
+  const TInB* B = (const TInB*)g.B + (long long)b * g.bsB + ks * g.ksB;
   const bool aK = (g.csA == 1);   // A contiguous along k
   const bool bK = (g.rsB == 1);   // B contiguous along k
   const float sA = g.conjA ? -1.f : 1.f, sB = g.conjB ? -1.f : 1.f;
@@ -164,7 +165,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
     }
   }
 
-  TOut* C = (TOut*)g.C + (long long)b * g.bsC;
+  TOut* C = (TOut*)g.C + (long long)b * g.bsC + ks * g.ksC;
   const bool beta0 = (g.beta_re == 0.0 && g.beta_im == 0.0);
 #pragma unroll
   for (int i = 0; i < TM; ++i)
@@ -233,12 +234,14 @@ static hipError_t launch(const GemmArgs& g0, hipStream_t s) {
   if (g0.M <= 0 || g0.N <= 0 || g0.batch <= 0) return hipSuccess;
   GemmArgs g = g0;
   g.herm = (g0.herm && BM == BN) ? 1 : 0;
+  const int split = g0.ksplit > 1 ? g0.ksplit : 1;
+  if (split == 1) g.ksA = g.ksB = g.ksC = 0;
   const int tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
   const int tiles = g.herm ? tm * (tm + 1) / 2 : tm * tn;
-  dim3 grid(tiles, 1, g.batch);
+  dim3 grid(tiles, split, g.batch);
   hipLaunchKernelGGL((gemm_kernel<TInA, TInB, TA, TOut, BM, BN, BK>), grid, dim3(256), 0, s, g);
   hipError_t e = hipGetLastError();
-  if (e != hipSuccess || !g0.herm) return e;
+  if (e != hipSuccess || !g0.herm || split > 1) return e;
   return herm_mirror<TOut>(g0, s);
 }
 
@@ -276,7 +279,7 @@ hipError_t gemm(Prec p, const GemmArgs& g, hipStream_t s) {
     // by any caller; reject explicitly.
     return hipErrorInvalidValue;
   }
-  if (g.batch > 65535) return hipErrorInvalidValue;
+  if (g.batch > 65535 || g.ksplit > 64) return hipErrorInvalidValue;
   if (g.herm && g.M != g.N) return hipErrorInvalidValue;
   if (g.inA < 0 || g.inA > 2 || g.inB < 0 || g.inB > 2) return hipErrorInvalidValue;
   const bool in128 = (p == C128_F64ACC || p == C128_IN_C64_OUT);
@@ -288,9 +291,10 @@ hipError_t gemm(Prec p, const GemmArgs& g, hipStream_t s) {
   // a Hermitian output computes the 64 x 64 tiles on and below the diagonal only
   const double tt = (g.M + 63) / 64;
   const double mn_work = g.herm ? std::min(mn, 0.5 * tt * (tt + 1) * 4096.0) : mn;
-  const int shape[5] = {g.M, g.N, g.K, g.batch, (int)p + (g.herm ? 10 : 0)};
-  prof::Scope ps(prof::GEMM, s, 8.0 * mn_work * g.K * nb,
-                 nb * (inbA * (double)g.M * g.K + inbB * (double)g.K * g.N +
+  const double kt = (double)g.K * (g.ksplit > 1 ? g.ksplit : 1);   // the whole reduction
+  const int shape[5] = {g.M, g.N, (int)kt, g.batch, (int)p + (g.herm ? 10 : 0)};
+  prof::Scope ps(prof::GEMM, s, 8.0 * mn_work * kt * nb,
+                 nb * (inbA * (double)g.M * kt + inbB * kt * g.N +
                        outb * mn * ((g.beta_re != 0 || g.beta_im != 0) ? 2 : 1)),
                  shape);
   switch (p) {
@@ -301,6 +305,13 @@ hipError_t gemm(Prec p, const GemmArgs& g, hipStream_t s) {
     case C64_F64ACC_C64OUT: return launch_f64_mixed<float2>(a128, b128, g, s);
   }
   return hipErrorInvalidValue;
+}
+
+hipError_t herm_mirror_b(void* C, bool c128, int n, long long bs, int batch, hipStream_t s) {
+  GemmArgs g{};
+  g.M = g.N = n; g.batch = batch;
+  g.C = C; g.rsC = n; g.csC = 1; g.bsC = bs;
+  return c128 ? herm_mirror<double2>(g, s) : herm_mirror<float2>(g, s);
 }
 
 }  // namespace mpdo

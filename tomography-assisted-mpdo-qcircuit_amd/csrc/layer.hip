@@ -121,10 +121,14 @@ size_t out_bytes(Prec p) { return (p == C64_F64ACC || p == C128_F64ACC) ? 16 : 8
 size_t op_bytes(Prec p, int flag) { return flag ? (flag == 2 ? 16 : 8) : in_bytes(p); }
 
 hipError_t k_gemm(Prec p, const GemmArgs& g, hipStream_t s, int line) {
-  if (!fits(g.A, ext(g.batch, g.bsA, g.M, g.rsA, g.K, g.csA), op_bytes(p, g.inA), "GEMM operand A", line) ||
-      !fits(g.B, ext(g.batch, g.bsB, g.K, g.rsB, g.N, g.csB), op_bytes(p, g.inB), "GEMM operand B", line) ||
-      !fits(g.C, ext(g.batch, g.bsC, g.M, g.rsC, g.N, g.csC), out_bytes(p), "GEMM result C", line))
+  const long long sp = g.ksplit > 1 ? g.ksplit - 1 : 0;   // extra reduction slices
+  if (!fits(g.A, ext(g.batch, g.bsA, g.M, g.rsA, g.K, g.csA) + sp * g.ksA, op_bytes(p, g.inA), "GEMM operand A",
+            line) ||
+      !fits(g.B, ext(g.batch, g.bsB, g.K, g.rsB, g.N, g.csB) + sp * g.ksB, op_bytes(p, g.inB), "GEMM operand B",
+            line) ||
+      !fits(g.C, ext(g.batch, g.bsC, g.M, g.rsC, g.N, g.csC) + sp * g.ksC, out_bytes(p), "GEMM result C", line))
     return hipErrorInvalidValue;
+  if (sp && (g.ksA < 0 || g.ksB < 0 || g.ksC < 0)) return hipErrorInvalidValue;
   if (g.herm && g.M != g.N) {
     char b[128];
     snprintf(b, sizeof b, "layer.hip:%d Hermitian GEMM output is not square", line);
@@ -132,10 +136,48 @@ hipError_t k_gemm(Prec p, const GemmArgs& g, hipStream_t s, int line) {
     return hipErrorInvalidValue;
   }
   if (t_dry) {
-    t_gemms.insert(t_gemms.end(), {line, g.M, g.N, g.K, g.batch, (int)p});
+    t_gemms.insert(t_gemms.end(), {line, g.M, g.N, g.K * (g.ksplit > 1 ? g.ksplit : 1), g.batch, (int)p});
     return hipSuccess;
   }
   return gemm(p, g, s);
+}
+
+// Partial-product buffer of the split-reduction Hermitian GEMMs of the running layer
+// (SPLITK_PARTS complex128 per chain: 8 slices of 256 x 256, or 2 of 512 x 512)
+constexpr long long SPLITK_PARTS = 8LL * 256 * 256;
+thread_local double2* t_split_parts = nullptr;
+thread_local long long t_split_cap = 0;   // complex128 elements
+
+// A Hermitian product G (n x n complex128, contiguous per batch entry): with few output tiles
+// and a long reduction, the reduction is split into S slices whose partial products run as one
+// launch (enough workgroups to fill the chip), then summed in a fixed order and mirrored.
+hipError_t herm_gemm_(int line, Prec p, GemmArgs g, hipStream_t s) {
+  const int n = g.M;
+  const long long nn = (long long)n * n;
+  const long long T = (n + 63) / 64, tiles = T * (T + 1) / 2;
+  int S = 1;
+  if (g.herm && t_split_parts && out_bytes(p) == 16 && g.rsC == n && g.csC == 1 && g.bsC == nn)
+    while (S < 8 && g.K % (2 * S) == 0 && g.K / (2 * S) >= 2048 && tiles * g.batch * 2 * S <= 4096 &&
+           2LL * S * nn * g.batch <= t_split_cap)
+      S *= 2;
+  if (S == 1) return k_gemm(p, g, s, line);
+  double2* G = (double2*)g.C;
+  GemmArgs h = g;
+  h.K = g.K / S;
+  h.ksplit = S;
+  h.ksA = (long long)h.K * g.csA;
+  h.ksB = (long long)h.K * g.rsB;
+  h.C = t_split_parts;
+  h.bsC = S * nn;
+  h.ksC = nn;
+  hipError_t e = k_gemm(p, h, s, line);
+  if (e != hipSuccess) return e;
+  if (!fits(t_split_parts, g.batch * S * nn, sizeof(double2), "split-reduction partials", line) ||
+      !fits(G, g.batch * nn, sizeof(double2), "split-reduction output", line))
+    return hipErrorInvalidValue;
+  if (t_dry) return hipSuccess;
+  if ((e = sum_parts(t_split_parts, S, nn, G, g.batch, s)) != hipSuccess) return e;
+  return herm_mirror_b(G, true, n, nn, g.batch, s);
 }
 #define K_GEMM(p, g, s) k_gemm((p), (g), (s), __LINE__)
 
@@ -357,7 +399,7 @@ hipError_t gram_cols_b_(int line, const float2* X, int rows, int n, long long bs
   g.C = G; g.rsC = n; g.csC = 1; g.bsC = (long long)n * n;
   g.alpha_re = 1;
   g.herm = 1;
-  return k_gemm(C64_F64ACC, g, s, line);
+  return herm_gemm_(line, C64_F64ACC, g, s);
 }
 // G[b] (n x n, complex128) = X[b] X[b]^H, X (n x cols) row-major
 hipError_t gram_rows_b_(int line, const float2* X, int n, int cols, long long bsX, double2* G, int batch, hipStream_t s) {
@@ -368,7 +410,7 @@ hipError_t gram_rows_b_(int line, const float2* X, int n, int cols, long long bs
   g.C = G; g.rsC = n; g.csC = 1; g.bsC = (long long)n * n;
   g.alpha_re = 1;
   g.herm = 1;
-  return k_gemm(C64_F64ACC, g, s, line);
+  return herm_gemm_(line, C64_F64ACC, g, s);
 }
 
 #define gram_cols_b(...) gram_cols_b_(__LINE__, __VA_ARGS__)
@@ -841,7 +883,7 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
   Site1q* dtab = nullptr;
   float2 *Mlast = nullptr, *FLq, *Rcur, *Rnext, *RT, *X1, *X1t, *Q1t, *R1t, *Cs, *Cp, *Cu, *Zb, *Q2P, *FL, *FR, *Tmin, *Wm,
          *Tm, *Pn, *Pn2, *PW, *Pp, *Tchi, *Mp, *Vk, *Kout, *dG, *Tcur2, *dK;
-  double2 *G, *GR, *Om64, *Z64, *H64, *Gparts, *Pg, *Hk, *Tw, *Rs64, *Zr, *Nn;
+  double2 *G, *GR, *Om64, *Z64, *H64, *Gparts, *Pg, *Hk, *Tw, *Rs64, *Zr, *Nn, *Sparts;
   float2 *Zm, *Vf, *Vt;
   double* ddisc;
   int* dk;
@@ -913,6 +955,7 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     dK = c.get<float2>((size_t)n * B * 4 * K1);
     G = c.get<double2>((size_t)mxEig * mxEig * B);
     Gparts = c.get<double2>((size_t)SPLIT_S * SPLIT_N * SPLIT_N * B);
+    Sparts = c.get<double2>((size_t)SPLITK_PARTS * B);
     Pg = c.get<double2>(mxPg * B);
     Hk = c.get<double2>(mxHk * B);
     Zm = c.get<float2>(mxZm * B);
@@ -942,6 +985,10 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
   Carver c1;
   c1.base = base;
   layout(c1);
+  struct SplitScope {
+    SplitScope(double2* p, long long cap) { t_split_parts = p; t_split_cap = cap; }
+    ~SplitScope() { t_split_parts = nullptr; t_split_cap = 0; }
+  } split_scope(Sparts, (long long)SPLITK_PARTS * B);
 #define HFITS(p, cnt, T, what) \
   if (!fits((p), (long long)(cnt), sizeof(T), (what), __LINE__)) return layer_fail(MPDO_EINVAL, t_oob.c_str())
   HFITS(ddisc, (size_t)B * (nsteps + 2 * n), double, "discarded weights");
@@ -1194,7 +1241,7 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
       g.C = GR; g.rsC = r; g.csC = 1; g.bsC = (long long)r * r;
       g.alpha_re = 1;
       g.herm = 1;
-      LCK(K_GEMM(C128_F64ACC, g, s));
+      LCK(herm_gemm_(__LINE__, C128_F64ACC, g, s));
     }
     {
       EigB eR{ps.WR, ps.lamR, eE.ws};

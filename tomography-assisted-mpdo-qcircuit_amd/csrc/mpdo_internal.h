@@ -23,6 +23,11 @@ struct GemmArgs {
   // operand storage overriding the precision mode's input type (fp64-accumulating modes only):
   // 0 = as the mode says, 1 = complex64, 2 = complex128
   int inA, inB;
+  // split reduction: ksplit > 1 runs ksplit slices of K each (the caller's K is one slice),
+  // slice t reading A + t ksA, B + t ksB and writing its partial product to C + t ksC (a
+  // Hermitian output is then left unmirrored: the caller sums the slices and mirrors)
+  int ksplit;
+  long long ksA, ksB, ksC;
 };
 
 enum Prec { C64_F32ACC = 0,   // complex64 in, fp32 MFMA accumulate, complex64 out
@@ -32,6 +37,8 @@ enum Prec { C64_F32ACC = 0,   // complex64 in, fp32 MFMA accumulate, complex64 o
             C64_F64ACC_C64OUT = 4 };  // complex64 in, fp64 accumulate, complex64 out
 
 hipError_t gemm(Prec p, const GemmArgs& g, hipStream_t s);
+// C[b][i][j] = conj(C[b][j][i]) for j > i (n x n row-major, batch stride bs)
+hipError_t herm_mirror_b(void* C, bool c128, int n, long long bs, int batch, hipStream_t s);
 
 // Convenience: C = alpha*A*B (+beta C) with row-major operands of the given leading dims.
 GemmArgs gemm_rowmajor(int M, int N, int K,

@@ -64,6 +64,7 @@ def load() -> ctypes.CDLL:
                                 VP, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, I,
                                 VP, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong,
                                 D, D, D, D, VP]),
+        'mpdo_debug_gram_split': (I, [I, I, I, I, VP, VP, VP, VP]),
         'mpdo_debug_gemm_ex': (I, [I, I, I, I, I, I, I, I, VP, ctypes.c_longlong, ctypes.c_longlong,
                                    ctypes.c_longlong, I, VP, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong,
                                    I, VP, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, D, D, D, D, VP]),
@@ -421,6 +422,18 @@ def debug_gemm(prec: int, A: torch.Tensor, B: torch.Tensor, C: torch.Tensor, M: 
                              complex(alpha).real, complex(alpha).imag, complex(beta).real, complex(beta).imag,
                              _stream(A.device))
     check(st, 'mpdo_debug_gemm')
+
+
+def debug_gram_split(X: torch.Tensor, slices: int) -> torch.Tensor:
+    """X (batch, n, K) complex64 on the device -> X X^H (batch, n, n) complex128 through the
+    split-reduction Hermitian path."""
+    lib = load()
+    b, n, K = X.shape
+    parts = torch.empty(b * slices * n * n, dtype=torch.complex128, device=X.device)
+    G = torch.empty(b, n, n, dtype=torch.complex128, device=X.device)
+    check(lib.mpdo_debug_gram_split(n, K, b, slices, ctypes.c_void_p(X.data_ptr()), ctypes.c_void_p(parts.data_ptr()),
+                                    ctypes.c_void_p(G.data_ptr()), _stream(X.device)), 'mpdo_debug_gram_split')
+    return G
 
 
 def debug_gemm_ex(prec: int, herm: bool, inA: int, inB: int, A: torch.Tensor, B: torch.Tensor, C: torch.Tensor,

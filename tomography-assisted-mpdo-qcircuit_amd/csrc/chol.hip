@@ -25,6 +25,7 @@ inline size_t a256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 constexpr int PC_NB = 32;      // columns per panel; the trailing update is one GEMM per panel
 constexpr int PC_THREADS = 256;
+constexpr int PC_ROWS = 8;     // rows per thread of the panel kernel: n <= PC_ROWS * PC_THREADS
 
 // Per-matrix state of the blocked pivoted Cholesky, kept in global memory across panels.
 struct PcState {
@@ -67,8 +68,10 @@ __global__ __launch_bounds__(PC_THREADS) void pchol_init_kernel(const double2* A
 // One panel of up to PC_NB pivot columns of matrix b (left-looking within the panel). A holds
 // the Gram matrix minus the outer products of every earlier panel (the trailing updates), so a
 // column needs only the panel's own earlier columns: L_ik = (A_ip - sum_{t in panel, t<k}
-// L_it conj(L_pt)) / sqrt(d_p). Rows that were pivots get 0. Two barriers per column: the
-// argmax candidates of the waves, then the pivot row's panel entries.
+// L_it conj(L_pt)) / sqrt(d_p). Rows that were pivots get 0. Thread tid owns rows tid + m
+// PC_THREADS (m < PC_ROWS) and keeps their remaining diagonal and pivot flags in registers for
+// the whole panel. Two barriers per column: the argmax candidates of the waves, then the pivot
+// row's panel entries.
 __global__ __launch_bounds__(PC_THREADS) void pchol_panel_kernel(const double2* A, int n, long long lda, long long bsA,
                                                                   int j0, double2* L, PcState st) {
   __shared__ double wv[PC_THREADS / 64];
@@ -81,14 +84,23 @@ __global__ __launch_bounds__(PC_THREADS) void pchol_panel_kernel(const double2* 
   double* d = st.d + (long long)b * n;
   unsigned char* used = st.used + (long long)b * n;
   const double thr = st.thr[b];
+  double dr[PC_ROWS];
+  bool ur[PC_ROWS];
+#pragma unroll
+  for (int m = 0; m < PC_ROWS; ++m) {
+    const int i = tid + m * PC_THREADS;
+    dr[m] = i < n ? d[i] : 0.0;
+    ur[m] = i < n ? used[i] != 0 : true;
+  }
   int k = j0;
   const int kend = min(j0 + PC_NB, n);
   for (; k < kend; ++k) {
-    // argmax of the remaining diagonal (ties: lowest index); each thread scans the rows it owns
+    // argmax of the remaining diagonal (ties: lowest index) over the rows this thread owns
     double best = -1.0;
     int bi = n;
-    for (int i = tid; i < n; i += PC_THREADS)
-      if (!used[i] && d[i] > best) { best = d[i]; bi = i; }
+#pragma unroll
+    for (int m = 0; m < PC_ROWS; ++m)
+      if (!ur[m] && dr[m] > best) { best = dr[m]; bi = tid + m * PC_THREADS; }
     for (int off = 32; off > 0; off >>= 1) {
       const double ov = __shfl_xor(best, off);
       const int oi = __shfl_xor(bi, off);
@@ -103,19 +115,20 @@ __global__ __launch_bounds__(PC_THREADS) void pchol_panel_kernel(const double2* 
       if (wv[w] > bv || (wv[w] == bv && wi[w] < bx)) { bv = wv[w]; bx = wi[w]; }
     const int p = (bv > thr && bx < n) ? bx : -1;
     if (p < 0) break;
-    if (tid == 0) {
-      used[p] = 1;
-      st.piv[(long long)b * n + k] = p;
-    }
+    if (tid == 0) st.piv[(long long)b * n + k] = p;
     for (int t = tid; t < k - j0; t += PC_THREADS) Lp[t] = Lb[(long long)(j0 + t) * n + p];
     __syncthreads();
     const double piv = sqrt(bv);
     const double2* Ap = A + (long long)p * lda;   // row p: A_ip = conj(A_pi)
-    for (int i = tid; i < n; i += PC_THREADS) {
+#pragma unroll
+    for (int m = 0; m < PC_ROWS; ++m) {
+      const int i = tid + m * PC_THREADS;
+      if (i >= n) continue;
       double2 v;
       if (i == p) {
         v = make_double2(piv, 0.0);
-      } else if (used[i]) {
+        ur[m] = true;
+      } else if (ur[m]) {
         v = make_double2(0.0, 0.0);
       } else {
         const double2 api = Ap[i];
@@ -126,9 +139,17 @@ __global__ __launch_bounds__(PC_THREADS) void pchol_panel_kernel(const double2* 
           acc.y -= x.y * y.x - x.x * y.y;
         }
         v = make_double2(acc.x / piv, acc.y / piv);
-        d[i] -= v.x * v.x + v.y * v.y;
+        dr[m] -= v.x * v.x + v.y * v.y;
       }
       Lb[(long long)k * n + i] = v;
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < PC_ROWS; ++m) {
+    const int i = tid + m * PC_THREADS;
+    if (i < n) {
+      d[i] = dr[m];
+      used[i] = ur[m] ? 1 : 0;
     }
   }
   if (tid == 0) {
@@ -260,6 +281,7 @@ size_t pchol_workspace_bytes(int n, int batch) {
 hipError_t pchol_b(int n, int batch, void* A, long long lda, long long bsA, double tol_rel, double2* L, int* piv,
                    int* rank, void* ws, hipStream_t st) {
   if (n <= 0 || batch <= 0) return hipSuccess;
+  if (n > PC_ROWS * PC_THREADS) return hipErrorInvalidValue;
   hipError_t err;
   char* p = (char*)ws;
   PcState pc;

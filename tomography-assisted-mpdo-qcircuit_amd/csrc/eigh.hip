@@ -243,8 +243,7 @@ __global__ __launch_bounds__(1024) void bj_solve(BJ s, int r, int inner_sweeps) 
   __shared__ double2 A[NS][LD];
   __shared__ double2 V[NS][LD];
   __shared__ Rot rot[NS / 2];
-  __shared__ double red[256];
-  __shared__ int cnt;
+  __shared__ int cnt, need;
   const int P = s.N / 32, pi = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
   int I, J;
   block_pair(P, r, pi, I, J);
@@ -259,7 +258,23 @@ __global__ __launch_bounds__(1024) void bj_solve(BJ s, int r, int inner_sweeps) 
   if (tid == 0) cnt = 0;
   __syncthreads();
   const double fl = s.floor_abs[b];
-  (void)red;
+  // no off-diagonal entry above the rotation thresholds: the sweep would rotate nothing, so the
+  // round's rotation of this block pair is the identity (W is not written; its users skip it)
+  if (tid == 0) need = 0;
+  __syncthreads();
+  for (int e = tid; e < NS * NS; e += blockDim.x) {
+    const int i = e / NS, j = e % NS;
+    if (j > i) {
+      const double2 c = A[i][j];
+      const double ac = sqrt(c.x * c.x + c.y * c.y);
+      if (ac > fl && ac > REL_TOL * sqrt(fabs(A[i][i].x * A[j][j].x))) need = 1;
+    }
+  }
+  __syncthreads();
+  if (need == 0) {
+    if (tid == 0) s.ident[b * (P / 2) + pi] = 1;
+    return;
+  }
   for (int sw = 0; sw < inner_sweeps; ++sw) {
     for (int rr = 0; rr < NS - 1; ++rr) jacobi_round_blk(A, V, rot, NS, rr, fl, &cnt);
   }

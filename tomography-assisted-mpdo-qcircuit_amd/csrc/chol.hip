@@ -24,8 +24,8 @@ namespace {
 inline size_t a256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 constexpr int PC_NB = 32;      // columns per panel; the trailing update is one GEMM per panel
-constexpr int PC_THREADS = 256;
-constexpr int PC_ROWS = 8;     // rows per thread of the panel kernel: n <= PC_ROWS * PC_THREADS
+constexpr int PC_THREADS = 512;
+constexpr int PC_ROWS = 4;     // rows per thread of the panel kernel: n <= PC_ROWS * PC_THREADS
 
 // Per-matrix state of the blocked pivoted Cholesky, kept in global memory across panels.
 struct PcState {

@@ -287,13 +287,12 @@ __global__ __launch_bounds__(1024) void bj_solve(BJ s, int r, int inner_sweeps) 
 }
 
 // X[rows, K u L] <- X[rows, K u L] * W_KL   (cols=1)   or
-// X[K u L, cols] <- W_KL^H * X[K u L, cols] (cols=0), in place, 64x64 tile per workgroup.
-// blockIdx.z selects the matrix: z < batch -> A[z], else V[z - batch] (columns only).
-__device__ __forceinline__ void rotate_body(BJ s, int r, int cols, int z) {
-  // only the 64x64 tile of M is staged in LDS (66.5 KiB, two workgroups per CU); the rotation W
-  // of the block pair is read from global memory, where every tile of the pair finds it in L2
-  __shared__ double2 X[NS][LD];
-  const int P = s.N / 32, pi = blockIdx.y, t0 = blockIdx.x * NS, tid = threadIdx.x;
+// X[K u L, cols] <- W_KL^H * X[K u L, cols] (cols=0), in place, the 64x64 tile `tile` of block
+// pair pi per workgroup. z selects the matrix: z < batch -> A[z], else V[z - batch] (columns
+// only). Only the tile is staged in LDS (X, 66.5 KiB: two workgroups per CU); the rotation W of
+// the block pair is read from global memory, where every tile of the pair finds it in L2.
+__device__ __forceinline__ void rotate_body(BJ s, int r, int cols, int z, int tile, int pi, double2 (*X)[LD]) {
+  const int P = s.N / 32, t0 = tile * NS, tid = threadIdx.x;
   bool isV = z >= s.batch;
   int b = isV ? z - s.batch : z;
   int I, J;
@@ -373,11 +372,6 @@ __device__ __forceinline__ void rotate_body(BJ s, int r, int cols, int z) {
       }
 }
 
-// X <- X J (cols = 1) or J^H X (cols = 0) on the block pair's columns/rows; blockIdx.z selects
-// the matrix (z < batch: A, else V)
-__global__ __launch_bounds__(256) void bj_rotate(BJ s, int r, int cols) { rotate_body(s, r, cols, blockIdx.z); }
-// V <- V J on the columns of the round's block pairs (blockIdx.z = matrix)
-__global__ __launch_bounds__(256) void bj_rotate_v(BJ s, int r) { rotate_body(s, r, 1, blockIdx.z + s.batch); }
 
 // Two-sided update of A by the round's block rotations, one 64x64 block pair per workgroup:
 // A[R, C] <- W_R^H A[R, C] W_C for the lower block triangle (R >= C in pair order), the block
@@ -422,11 +416,10 @@ __device__ inline void mfma_tile64(const double2 (*X)[LD], const double2* Wg, bo
   }
 }
 
-__global__ __launch_bounds__(256) void bj_sym(BJ s, int r) {
-  __shared__ double2 X[NS][LD];
-  const int P = s.N / 32, h = P / 2, tid = threadIdx.x, b = blockIdx.y;
+__device__ __forceinline__ void sym_body(BJ s, int r, int e, int b, double2 (*X)[LD]) {
+  const int P = s.N / 32, h = P / 2, tid = threadIdx.x;
   // lower-triangle block pair (pr >= pc) from the linear index
-  int e = blockIdx.x, pr = 0;
+  int pr = 0;
   while (e >= pr + 1) { e -= pr + 1; ++pr; }
   const int pc = e;
   (void)h;
@@ -485,6 +478,20 @@ __global__ __launch_bounds__(256) void bj_sym(BJ s, int r) {
         A[(long long)grow(i) * s.N + gcol(j)] = o;
         if (pr != pc) A[(long long)gcol(j) * s.N + grow(i)] = make_double2(o.x, -o.y);
       }
+}
+
+// One launch per round for both updates, which touch disjoint data: workgroups x < nsym update
+// A's block pair x (two-sided, sym_body), the rest V's tiles (V <- V J, rotate_body: tile
+// (x - nsym) % ntile of block pair (x - nsym) / ntile); blockIdx.y = matrix.
+__global__ __launch_bounds__(256) void bj_update(BJ s, int r, int nsym, int ntile) {
+  __shared__ double2 X[NS][LD];
+  const int x = blockIdx.x, b = blockIdx.y;
+  if (x < nsym) {
+    sym_body(s, r, x, b, X);
+  } else {
+    const int t = x - nsym;
+    rotate_body(s, r, 1, b + s.batch, t % ntile, t / ntile, X);
+  }
 }
 
 // eigenvalues = diag(A), sorted descending (stable), eigenvectors = matching columns of V
@@ -580,13 +587,13 @@ hipError_t eigh(int n, int batch, void* A, long long lda, long long bsA, double*
         prof::Scope ps(prof::EIG_SOLVE, st, solve_flops, solve_bytes);
         hipLaunchKernelGGL(bj_solve, dim3(P / 2, batch), dim3(g_bj_threads), 0, st, s, r, 1);
       }
-      {   // A <- J^H A J block pair by block pair (Hermitian: lower triangle + mirror)
-        prof::Scope ps(prof::EIG_ROTATE, st, tiles_sym * 2 * tile_flops, tiles_sym * tile_bytes);
-        hipLaunchKernelGGL(bj_sym, dim3(npairs_sym, batch), dim3(256), 0, st, s, r);
-      }
-      {   // V <- V J: the columns of the round's block pairs, all rows
-        prof::Scope ps(prof::EIG_ROTATE, st, tiles_r * tile_flops, tiles_r * tile_bytes);
-        hipLaunchKernelGGL(bj_rotate_v, dim3(s.N / NS, P / 2, batch), dim3(256), 0, st, s, r);
+      {   // A <- J^H A J block pair by block pair (Hermitian: lower triangle + mirror), and
+          // V <- V J on the columns of the round's block pairs, in one launch
+        prof::Scope ps(prof::EIG_ROTATE, st, (tiles_sym * 2 + tiles_r) * tile_flops,
+                       (tiles_sym + tiles_r) * tile_bytes);
+        const int ntile = s.N / NS;
+        hipLaunchKernelGGL(bj_update, dim3(npairs_sym + ntile * (P / 2), batch), dim3(256), 0, st, s, r, npairs_sym,
+                           ntile);
       }
     }
     if ((err = hipGetLastError()) != hipSuccess) return err;

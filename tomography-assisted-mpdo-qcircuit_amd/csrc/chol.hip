@@ -23,7 +23,7 @@ namespace {
 
 inline size_t a256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-constexpr int PC_NB = 32;      // columns per panel; the trailing update is one GEMM per panel
+constexpr int PC_NB = 64;      // columns per panel; the trailing update is one GEMM per panel
 constexpr int PC_THREADS = 512;
 constexpr int PC_ROWS = 4;     // rows per thread of the panel kernel: n <= PC_ROWS * PC_THREADS
 
@@ -310,7 +310,7 @@ hipError_t pchol_b(int n, int batch, void* A, long long lda, long long bsA, doub
     g.C = A; g.rsC = lda; g.csC = 1; g.bsC = bsA;
     g.alpha_re = -1.0; g.beta_re = 1.0;
     if ((err = gemm(C128_F64ACC, g, st)) != hipSuccess) return err;
-    if ((j0 / PC_NB) % 4 == 3) {   // every 4 panels: stop once every matrix has stopped
+    if ((j0 + nb) % 128 == 0) {   // every 128 columns: stop once every matrix has stopped
       if ((err = hipMemcpyAsync(hs.data(), pc.stop, sizeof(int) * batch, hipMemcpyDeviceToHost, st)) != hipSuccess)
         return err;
       if ((err = hipStreamSynchronize(st)) != hipSuccess) return err;

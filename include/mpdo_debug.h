@@ -58,7 +58,8 @@ mpdo_status mpdo_debug_layer_dry_run(int batch, int n, const mpdo_site* in, int 
                                      int kappa, const mpdo_site* out, unsigned long long* workspace_bytes);
 
 /* Tile shape of the fp64-accumulating GEMMs for tuning sweeps: 0 = 64x64x16 (default),
- * 1 = 64x64x32, 2 = 128x64x16, 3 = 64x128x16, 4 = 128x128x16; returns the previous choice
+ * 1 = 64x64x32, 2 = 128x64x16, 3 = 64x128x16, 4 = 128x128x16, 5 = 64x64x16 with two LDS
+ * stages; returns the previous choice
  * (an out-of-range v only reads it). */
 int mpdo_debug_gemm_tiles(int v);
 

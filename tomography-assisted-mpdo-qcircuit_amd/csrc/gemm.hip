@@ -49,15 +49,17 @@ template <> __device__ inline float2 cvt<double2, float2>(double2 v) { return ma
 
 // TInA, TInB: operand storage types (float2 or double2, independently); TA: accumulation
 // scalar; TOut: output storage type.
-template <typename TInA, typename TInB, typename TA, typename TOut, int BM, int BN, int BK>
+// NBUF = 2: two LDS stages, the next one stored while the current one feeds the matrix cores
+// (one barrier per stage instead of two).
+template <typename TInA, typename TInB, typename TA, typename TOut, int BM, int BN, int BK, int NBUF = 1>
 __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
   typedef typename Acc<TA>::V V;
   typedef typename Acc<TA>::C CA;   // complex in accumulation precision (LDS element)
   constexpr int WM = BM / 2, WN = BN / 2;      // per-wave tile
   constexpr int TM = WM / 16, TN = WN / 16;    // 16x16 MFMA tiles per wave
   constexpr int PA = BM * BK / 256, PB = BK * BN / 256;
-  __shared__ CA As[BK][BM + 1];
-  __shared__ CA Bs[BK][BN + 1];
+  __shared__ CA As[NBUF][BK][BM + 1];
+  __shared__ CA Bs[NBUF][BK][BN + 1];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * WM, wn = (wave & 1) * WN;
@@ -118,7 +120,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
       rb[i] = v;
     }
   };
-  auto store = [&]() {
+  auto store = [&](int buf) {
 #pragma unroll
     for (int i = 0; i < PA; ++i) {
       int e = aK ? tid * PA + i : i * 256 + tid;
@@ -126,7 +128,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
       int kk = aK ? e % BK : e / BM;
       CA v = cvt<TInA, CA>(ra[i]);
       v.y *= sA;
-      As[kk][row] = v;
+      As[buf][kk][row] = v;
     }
 #pragma unroll
     for (int i = 0; i < PB; ++i) {
@@ -135,24 +137,19 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
       int kk = bK ? e % BK : e / BN;
       CA v = cvt<TInB, CA>(rb[i]);
       v.y *= sB;
-      Bs[kk][col] = v;
+      Bs[buf][kk][col] = v;
     }
   };
 
   const int lr = lane & 15, lk = lane >> 4;
-  load(0);
-  for (int k0 = 0; k0 < g.K; k0 += BK) {
-    __syncthreads();
-    store();
-    __syncthreads();
-    if (k0 + BK < g.K) load(k0 + BK);
+  auto compute = [&](int buf) {
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       CA a[TM], bb[TN];
 #pragma unroll
-      for (int i = 0; i < TM; ++i) a[i] = As[kk + lk][wm + i * 16 + lr];
+      for (int i = 0; i < TM; ++i) a[i] = As[buf][kk + lk][wm + i * 16 + lr];
 #pragma unroll
-      for (int j = 0; j < TN; ++j) bb[j] = Bs[kk + lk][wn + j * 16 + lr];
+      for (int j = 0; j < TN; ++j) bb[j] = Bs[buf][kk + lk][wn + j * 16 + lr];
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -162,6 +159,31 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
           acci[i][j] = Acc<TA>::mfma(a[i].x, bb[j].y, acci[i][j]);
           acci[i][j] = Acc<TA>::mfma(a[i].y, bb[j].x, acci[i][j]);
         }
+    }
+  };
+  load(0);
+  if constexpr (NBUF == 1) {
+    for (int k0 = 0; k0 < g.K; k0 += BK) {
+      __syncthreads();
+      store(0);
+      __syncthreads();
+      if (k0 + BK < g.K) load(k0 + BK);
+      compute(0);
+    }
+  } else {
+    // stage it is read from buffer it % 2 while stage it + 1 is written to the other: every
+    // thread has finished reading a buffer (barrier at the end of the previous stage) before
+    // any thread writes it again
+    store(0);
+    __syncthreads();
+    int cur = 0;
+    for (int k0 = 0; k0 < g.K; k0 += BK) {
+      const bool more = k0 + BK < g.K;
+      if (more) load(k0 + BK);
+      compute(cur);
+      if (more) store(cur ^ 1);
+      __syncthreads();
+      cur ^= 1;
     }
   }
 
@@ -229,7 +251,7 @@ static hipError_t herm_mirror(const GemmArgs& g, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <typename TInA, typename TInB, typename TA, typename TOut, int BM, int BN, int BK>
+template <typename TInA, typename TInB, typename TA, typename TOut, int BM, int BN, int BK, int NBUF = 1>
 static hipError_t launch(const GemmArgs& g0, hipStream_t s) {
   if (g0.M <= 0 || g0.N <= 0 || g0.batch <= 0) return hipSuccess;
   GemmArgs g = g0;
@@ -239,14 +261,14 @@ static hipError_t launch(const GemmArgs& g0, hipStream_t s) {
   const int tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
   const int tiles = g.herm ? tm * (tm + 1) / 2 : tm * tn;
   dim3 grid(tiles, split, g.batch);
-  hipLaunchKernelGGL((gemm_kernel<TInA, TInB, TA, TOut, BM, BN, BK>), grid, dim3(256), 0, s, g);
+  hipLaunchKernelGGL((gemm_kernel<TInA, TInB, TA, TOut, BM, BN, BK, NBUF>), grid, dim3(256), 0, s, g);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !g0.herm || split > 1) return e;
   return herm_mirror<TOut>(g0, s);
 }
 
 // Tile shape of the fp64-accumulating GEMMs (BM x BN x BK): 0 = 64x64x16, 1 = 64x64x32,
-// 2 = 128x64x16, 3 = 64x128x16, 4 = 128x128x16.
+// 2 = 128x64x16, 3 = 64x128x16, 4 = 128x128x16, 5 = 64x64x16 with two LDS stages.
 int g_tile_variant = 0;
 
 template <typename TInA, typename TInB, typename TOut>
@@ -260,6 +282,10 @@ static hipError_t launch_f64(const GemmArgs& g, hipStream_t s) {
       default: break;
     }
   }
+  // a complex128 operand doubles the staging traffic per stage: two LDS stages hide it
+  // (43.9 -> 58.7 TFLOP/s at 8192 x 512 x 512); complex64 operands gain nothing from them
+  constexpr bool wide = std::is_same<TInA, double2>::value || std::is_same<TInB, double2>::value;
+  if (wide || g_tile_variant == 5) return launch<TInA, TInB, double, TOut, 64, 64, 16, 2>(g, s);
   return launch<TInA, TInB, double, TOut, 64, 64, 16>(g, s);
 }
 
@@ -318,7 +344,7 @@ hipError_t herm_mirror_b(void* C, bool c128, int n, long long bs, int batch, hip
 
 extern "C" int mpdo_debug_gemm_tiles(int v) {
   const int old = mpdo::g_tile_variant;
-  if (v >= 0 && v <= 4) mpdo::g_tile_variant = v;
+  if (v >= 0 && v <= 5) mpdo::g_tile_variant = v;
   return old;
 }
 

@@ -33,8 +33,9 @@ mpdo_status mpdo_debug_gram_split(int n, int K, int batch, int slices, const voi
 
 /* mpdo_debug_gemm with each setting explicit: herm = 1 for a Hermitian output (as prec + 10
  * above); inA / inB the storage of each operand (0 = as prec says, 1 = complex64, 2 = complex128;
- * a complex128 operand needs an fp64-accumulating prec). */
-mpdo_status mpdo_debug_gemm_ex(int prec, int herm, int inA, int inB, int M, int N, int K, int batch,
+ * a complex128 operand needs an fp64-accumulating prec); tri = 1 when op(B) is upper triangular,
+ * 2 when op(A) is lower triangular (each output tile reduces only over its nonzero k range). */
+mpdo_status mpdo_debug_gemm_ex(int prec, int herm, int inA, int inB, int tri, int M, int N, int K, int batch,
                                const void* A, long long rsA, long long csA, long long bsA, int conjA,
                                const void* B, long long rsB, long long csB, long long bsB, int conjB,
                                void* C, long long rsC, long long csC, long long bsC,

@@ -98,6 +98,35 @@ def test_gemm_mixed_operands(cuda, prec, inA, inB, M, N, K, tA, tB):
     assert err < (1e-13 if prec in (1, 2) else 1e-6), (prec, inA, inB, M, N, K, err)
 
 
+@pytest.mark.parametrize('tri', [1, 2])
+@pytest.mark.parametrize('M,N,K', [(300, 130, 130), (130, 257, 130), (64, 64, 64), (17, 40, 40), (40, 17, 40)])
+def test_gemm_triangular_operand(cuda, tri, M, N, K):
+    """tri = 1: op(B) upper triangular (K x N), tri = 2: op(A) lower triangular (M x K); each output
+    tile reduces only over the k range that can be nonzero. Complex64 data x complex128 factor
+    with a complex64 output, as in the Cholesky-QR products; vs numpy."""
+    from mpdo_amd import _lib
+    if (tri == 1 and K != N) or (tri == 2 and K != M):
+        pytest.skip('square triangular operand')
+    rng = np.random.default_rng(M + 3 * N + 7 * K + tri)
+    batch = 2
+    A = _crand(rng, batch, M, K)
+    B = _crand(rng, batch, K, N)
+    if tri == 1:
+        B = np.triu(B)
+        A = A.astype(np.complex64)
+        inA, inB = 1, 2
+    else:
+        A = np.tril(A)
+        B = B.astype(np.complex64)
+        inA, inB = 2, 1
+    C = torch.zeros(batch, M, N, dtype=torch.complex64, device=cuda)
+    _lib.debug_gemm_ex(3, False, inA, inB, torch.from_numpy(A).to(cuda), torch.from_numpy(B).to(cuda), C, M, N, K,
+                       (K, 1, M * K), (N, 1, K * N), (N, 1, M * N), batch=batch, tri=tri)
+    ref = A.astype(np.complex128) @ B.astype(np.complex128)
+    got = C.cpu().numpy()
+    assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-6
+
+
 @pytest.mark.parametrize('slices', [1, 2, 4, 8])
 @pytest.mark.parametrize('n,K', [(64, 4096), (200, 8192), (256, 2048), (33, 64)])
 def test_gram_split_reduction(cuda, n, K, slices):

@@ -930,7 +930,8 @@ mpdo_status mpdo_contract_state(int n, const mpdo_site* sites, void* out, void* 
 }
 
 // ------------------------------------------------------------------------------ test hooks
-mpdo_status mpdo_debug_gemm_ex(int prec, int herm, int inA, int inB, int M, int N, int K, int batch, const void* A,
+mpdo_status mpdo_debug_gemm_ex(int prec, int herm, int inA, int inB, int tri, int M, int N, int K, int batch,
+                               const void* A,
                                long long rsA, long long csA, long long bsA, int conjA, const void* B, long long rsB,
                                long long csB, long long bsB, int conjB, void* C, long long rsC, long long csC,
                                long long bsC, double alpha_re, double alpha_im, double beta_re, double beta_im,
@@ -938,7 +939,9 @@ mpdo_status mpdo_debug_gemm_ex(int prec, int herm, int inA, int inB, int M, int 
   if (prec < 0 || prec > 4) return fail(MPDO_EINVAL, "prec %d", prec);
   if (herm && M != N) return fail(MPDO_EINVAL, "Hermitian output needs M == N (%d, %d)", M, N);
   if (inA < 0 || inA > 2 || inB < 0 || inB > 2) return fail(MPDO_EINVAL, "operand storage %d %d", inA, inB);
+  if (tri < 0 || tri > 2 || (tri && herm)) return fail(MPDO_EINVAL, "tri %d", tri);
   GemmArgs g{};
+  g.tri = tri;
   g.herm = herm != 0;
   g.inA = inA; g.inB = inB;
   g.M = M; g.N = N; g.K = K; g.batch = batch;
@@ -977,7 +980,7 @@ mpdo_status mpdo_debug_gemm(int prec, int M, int N, int K, int batch, const void
                             int conjB, void* C, long long rsC, long long csC, long long bsC, double alpha_re,
                             double alpha_im, double beta_re, double beta_im, void* stream) {
   const int herm = prec >= 10;
-  return mpdo_debug_gemm_ex(herm ? prec - 10 : prec, herm, 0, 0, M, N, K, batch, A, rsA, csA, bsA, conjA, B, rsB, csB,
+  return mpdo_debug_gemm_ex(herm ? prec - 10 : prec, herm, 0, 0, 0, M, N, K, batch, A, rsA, csA, bsA, conjA, B, rsB, csB,
                             bsB, conjB, C, rsC, csC, bsC, alpha_re, alpha_im, beta_re, beta_im, stream);
 }
 

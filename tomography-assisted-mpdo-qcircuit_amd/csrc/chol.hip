@@ -206,6 +206,31 @@ __global__ __launch_bounds__(64) void inv64_kernel(double2* Lt, int R) {
   for (int r = 0; r < 64; ++r) T[(long long)r * R + j] = X[r][j];
 }
 
+// Pivot-order gathers of complex64 data: cols = 1: out[i][t] = M[i][piv[t]] (rows x n), cols = 0:
+// out[t][j] = M[piv[t]][j] (n x cols); zero for t >= rank
+__global__ void gather_piv_kernel(const float2* M, const int* piv, const int* rank, int n, long long other, int cols,
+                                  float2* out) {
+  const int b = blockIdx.y;
+  const int rb = rank[b];
+  const int* pb = piv + (long long)b * n;
+  const long long per = (long long)n * other;
+  const float2* Mb = M + (long long)b * per;
+  float2* Ob = out + (long long)b * per;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < per; e += (long long)gridDim.x * blockDim.x) {
+    float2 v = make_float2(0.f, 0.f);
+    if (cols) {
+      const long long i = e / n;
+      const int t = (int)(e % n);
+      if (t < rb) v = Mb[i * n + pb[t]];
+    } else {
+      const int t = (int)(e / other);
+      const long long j = e % other;
+      if (t < rb) v = Mb[(long long)pb[t] * other + j];
+    }
+    Ob[e] = v;
+  }
+}
+
 __global__ void zero_c128_kernel(double2* X, long long per) {
   double2* Xb = X + (long long)blockIdx.y * per;
   for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < per; e += (long long)gridDim.x * blockDim.x)
@@ -213,7 +238,9 @@ __global__ void zero_c128_kernel(double2* X, long long per) {
 }
 
 // X (n x n complex128, zero-filled before) from Linv = Lt^-1 (R x R): mode 0
-// X[piv[t]][u] = conj(Linv[u][t]), mode 1 X[u][piv[t]] = Linv[u][t], for t, u < rank
+// X[piv[t]][u] = conj(Linv[u][t]), mode 1 X[u][piv[t]] = Linv[u][t], for t, u < rank; modes 2
+// and 3 the same in pivot order (X[t][u] = conj(Linv[u][t]), upper triangular; X[u][t] =
+// Linv[u][t], lower triangular), for products with the pivot-gathered input
 __global__ void scatter_x_kernel(const double2* Linv, const int* piv, const int* rank, int n, int R, int mode,
                                  double2* X) {
   const int b = blockIdx.y;
@@ -226,7 +253,9 @@ __global__ void scatter_x_kernel(const double2* Linv, const int* piv, const int*
     const int t = (int)(e / rb), u = (int)(e % rb);
     const double2 v = Li[(long long)u * R + t];
     if (mode == 0) Xb[(long long)pb[t] * n + u] = make_double2(v.x, -v.y);
-    else Xb[(long long)u * n + pb[t]] = v;
+    else if (mode == 1) Xb[(long long)u * n + pb[t]] = v;
+    else if (mode == 2) Xb[(long long)t * n + u] = make_double2(v.x, -v.y);
+    else Xb[(long long)u * n + t] = v;
   }
 }
 
@@ -391,6 +420,14 @@ hipError_t pchol_inverse_b(int n, int batch, const double2* L, const int* piv, c
   if ((err = hipGetLastError()) != hipSuccess) return err;
   hipLaunchKernelGGL(scatter_x_kernel, dim3(grid_for(nn), batch), dim3(256), 0, st, (const double2*)Lt, piv, rank, n, R,
                      mode, X);
+  return hipGetLastError();
+}
+
+hipError_t gather_piv(const float2* M, const int* piv, const int* rank, int n, long long other, int cols, float2* out,
+                      int batch, hipStream_t st) {
+  if (n <= 0 || other <= 0 || batch <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_piv_kernel, dim3(grid_for((long long)n * other), batch), dim3(256), 0, st, M, piv, rank, n,
+                     other, cols, out);
   return hipGetLastError();
 }
 

@@ -74,6 +74,12 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
     while ((mt + 1) * (mt + 2) / 2 <= e) ++mt;
     m0 = mt * BM;
     n0 = (e - mt * (mt + 1) / 2) * BN;
+  } else if (g.tri == 1) {
+    // work grows with the n-tile: m fastest, so that consecutive workgroups (dealt round-robin to
+    // the XCDs) share an n-tile and every XCD gets every n-tile
+    const int ntm = (g.M + BM - 1) / BM;
+    m0 = (blockIdx.x % ntm) * BM;
+    n0 = (blockIdx.x / ntm) * BN;
   } else {
     const int ntn = (g.N + BN - 1) / BN;
     m0 = (blockIdx.x / ntn) * BM;
@@ -161,13 +167,15 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
         }
     }
   };
+  // a triangular operand leaves only part of the reduction nonzero for this tile
+  const int kend = g.tri == 1 ? min(g.K, n0 + BN) : g.tri == 2 ? min(g.K, m0 + BM) : g.K;
   load(0);
   if constexpr (NBUF == 1) {
-    for (int k0 = 0; k0 < g.K; k0 += BK) {
+    for (int k0 = 0; k0 < kend; k0 += BK) {
       __syncthreads();
       store(0);
       __syncthreads();
-      if (k0 + BK < g.K) load(k0 + BK);
+      if (k0 + BK < kend) load(k0 + BK);
       compute(0);
     }
   } else {
@@ -177,8 +185,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
     store(0);
     __syncthreads();
     int cur = 0;
-    for (int k0 = 0; k0 < g.K; k0 += BK) {
-      const bool more = k0 + BK < g.K;
+    for (int k0 = 0; k0 < kend; k0 += BK) {
+      const bool more = k0 + BK < kend;
       if (more) load(k0 + BK);
       compute(cur);
       if (more) store(cur ^ 1);
@@ -308,6 +316,7 @@ hipError_t gemm(Prec p, const GemmArgs& g, hipStream_t s) {
   if (g.batch > 65535 || g.ksplit > 64) return hipErrorInvalidValue;
   if (g.herm && g.M != g.N) return hipErrorInvalidValue;
   if (g.inA < 0 || g.inA > 2 || g.inB < 0 || g.inB > 2) return hipErrorInvalidValue;
+  if (g.tri < 0 || g.tri > 2 || (g.tri && (g.ksplit > 1 || g.herm))) return hipErrorInvalidValue;
   const bool in128 = (p == C128_F64ACC || p == C128_IN_C64_OUT);
   const bool a128 = g.inA ? g.inA == 2 : in128, b128 = g.inB ? g.inB == 2 : in128;
   if (p == C64_F32ACC && (a128 || b128)) return hipErrorInvalidValue;
@@ -317,7 +326,13 @@ hipError_t gemm(Prec p, const GemmArgs& g, hipStream_t s) {
   // a Hermitian output computes the 64 x 64 tiles on and below the diagonal only
   const double tt = (g.M + 63) / 64;
   const double mn_work = g.herm ? std::min(mn, 0.5 * tt * (tt + 1) * 4096.0) : mn;
-  const double kt = (double)g.K * (g.ksplit > 1 ? g.ksplit : 1);   // the whole reduction
+  double kt = (double)g.K * (g.ksplit > 1 ? g.ksplit : 1);   // the whole reduction
+  if (g.tri == 1 || g.tri == 2) {   // mean k range over the output tiles (64 wide)
+    const int ext = g.tri == 1 ? g.N : g.M, nt = (ext + 63) / 64;
+    double sk = 0;
+    for (int t = 0; t < nt; ++t) sk += std::min(g.K, (t + 1) * 64);
+    kt = sk / nt;
+  }
   const int shape[5] = {g.M, g.N, (int)kt, g.batch, (int)p + (g.herm ? 10 : 0)};
   prof::Scope ps(prof::GEMM, s, 8.0 * mn_work * kt * nb,
                  nb * (inbA * (double)g.M * kt + inbB * kt * g.N +

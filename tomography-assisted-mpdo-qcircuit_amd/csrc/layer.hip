@@ -559,26 +559,67 @@ mpdo_status one_pass_enough(int n, int batch, QRWS& w, bool& ok, hipStream_t s, 
 }
 #define K_PINV(n, batch, L, piv, rank, mode, X, w, s) k_pinv(n, batch, L, piv, rank, mode, X, w, s, __LINE__)
 
+hipError_t k_gather(const float2* M, const int* piv, const int* rank, int n, long long other, int cols, float2* out,
+                    int batch, hipStream_t s, int line) {
+  if (!fits(M, batch * n * other, sizeof(float2), "pivot gather input", line) ||
+      !fits(piv, (long long)batch * n, sizeof(int), "pivot gather pivots", line) ||
+      !fits(rank, batch, sizeof(int), "pivot gather ranks", line) ||
+      !fits(out, batch * n * other, sizeof(float2), "pivot gather output", line))
+    return hipErrorInvalidValue;
+  if (t_dry) return hipSuccess;
+  return gather_piv(M, piv, rank, n, other, cols, out, batch, s);
+}
+
+// Y (rows x cols, complex64) = G (rows x cols, complex64 data gathered to pivot order) times the
+// upper-triangular complex128 factor U (cols x cols)
+hipError_t tri_right_(int line, int rows, int cols, const float2* Gm, const double2* U, float2* Y, int batch,
+                      hipStream_t s) {
+  GemmArgs g{};
+  g.M = rows; g.N = cols; g.K = cols; g.batch = batch;
+  g.A = Gm; g.rsA = cols; g.csA = 1; g.bsA = (long long)rows * cols; g.inA = 1;
+  g.B = U; g.rsB = cols; g.csB = 1; g.bsB = (long long)cols * cols; g.inB = 2;
+  g.C = Y; g.rsC = cols; g.csC = 1; g.bsC = (long long)rows * cols;
+  g.alpha_re = 1;
+  g.tri = 1;
+  return k_gemm(C128_IN_C64_OUT, g, s, line);
+}
+// Yt (rows x cols, complex64) = the lower-triangular complex128 factor Lw (rows x rows) times G
+// (rows x cols, complex64 data gathered to pivot order)
+hipError_t tri_left_(int line, int rows, int cols, const double2* Lw, const float2* Gm, float2* Yt, int batch,
+                     hipStream_t s) {
+  GemmArgs g{};
+  g.M = rows; g.N = cols; g.K = rows; g.batch = batch;
+  g.A = Lw; g.rsA = rows; g.csA = 1; g.bsA = (long long)rows * rows; g.inA = 2;
+  g.B = Gm; g.rsB = cols; g.csB = 1; g.bsB = (long long)rows * cols; g.inB = 1;
+  g.C = Yt; g.rsC = cols; g.csC = 1; g.bsC = (long long)rows * cols;
+  g.alpha_re = 1;
+  g.tri = 2;
+  return k_gemm(C128_IN_C64_OUT, g, s, line);
+}
+
 // Q (rows x cols) with orthonormal columns, Rf (cols x cols) complex64 with M = Q Rf; M is tall.
-// M = Q1 L1^H (Q1 = M X1), Q1 = Q L2^H (Q = Q1 X2), so Rf = L2^H L1^H. The products with the
-// inverse factors X (whose entries grow with the condition number) take complex128 operands.
+// M = Q1 L1^H (Q1 = M X1), Q1 = Q L2^H (Q = Q1 X2), so Rf = L2^H L1^H. With M's columns gathered
+// to pivot order the inverse factor X is upper triangular (half the product's work); the
+// products with it (entries growing with the condition number) take it in complex128.
 mpdo_status qr_b(const float2* M, int rows, int cols, float2* Q, float2* Rf, int batch, QRWS& w, hipStream_t s) {
-  const long long cc = (long long)cols * cols, rc = (long long)rows * cols;
-  LCK(gram_cols_b(M, rows, cols, rc, w.G, batch, s));
+  const long long cc = (long long)cols * cols;
+  LCK(gram_cols_b(M, rows, cols, (long long)rows * cols, w.G, batch, s));
   LCK(K_PCHOL(cols, batch, w.G, FLOOR, w.L1, w.piv1, w.rank1, w, s));
-  LCK(K_PINV(cols, batch, w.L1, w.piv1, w.rank1, 0, w.F, w, s));
+  LCK(K_PINV(cols, batch, w.L1, w.piv1, w.rank1, 2, w.F, w, s));
+  LCK(k_gather(M, w.piv1, w.rank1, cols, rows, 1, w.A1, batch, s, __LINE__));
   bool one = false;
   LST(one_pass_enough(cols, batch, w, one, s, __LINE__));
   if (one) {   // Q = M X1, Rf = L1^H
-    LCK(mmx(C128_IN_C64_OUT, rows, cols, cols, M, cols, rc, 1, w.F, cols, cc, 2, Q, cols, rc, batch, s));
+    LCK(tri_right_(__LINE__, rows, cols, w.A1, w.F, Q, batch, s));
     LCK(k_narrow_factor(w.L1, cols, 0, Rf, batch, s, __LINE__));
     return MPDO_OK;
   }
-  LCK(mmx(C128_IN_C64_OUT, rows, cols, cols, M, cols, rc, 1, w.F, cols, cc, 2, w.A1, cols, rc, batch, s));
-  LCK(gram_cols_b(w.A1, rows, cols, rc, w.G, batch, s));
+  LCK(tri_right_(__LINE__, rows, cols, w.A1, w.F, Q, batch, s));   // Q1, held in Q
+  LCK(gram_cols_b(Q, rows, cols, (long long)rows * cols, w.G, batch, s));
   LCK(K_PCHOL(cols, batch, w.G, FLOOR, w.L2, w.piv2, w.rank2, w, s));
-  LCK(K_PINV(cols, batch, w.L2, w.piv2, w.rank2, 0, w.F, w, s));
-  LCK(mmx(C128_IN_C64_OUT, rows, cols, cols, w.A1, cols, rc, 1, w.F, cols, cc, 2, Q, cols, rc, batch, s));
+  LCK(K_PINV(cols, batch, w.L2, w.piv2, w.rank2, 2, w.F, w, s));
+  LCK(k_gather(Q, w.piv2, w.rank2, cols, rows, 1, w.A1, batch, s, __LINE__));
+  LCK(tri_right_(__LINE__, rows, cols, w.A1, w.F, Q, batch, s));
   // Rf[u][j] = sum_t conj(L2[t][u]) conj(L1[j][t]); L stored by columns: L[x][y] at L[y cols + x]
   GemmArgs g{};
   g.M = cols; g.N = cols; g.K = cols; g.batch = batch;
@@ -590,24 +631,27 @@ mpdo_status qr_b(const float2* M, int rows, int cols, float2* Q, float2* Rf, int
   return MPDO_OK;
 }
 // Qt (rows x cols) with orthonormal rows, Lf (rows x rows) complex64 with X = Lf Qt; X is wide.
-// X = L1 Qt1 (Qt1 = X1 X), Qt1 = L2 Qt (Qt = X2 Qt1), so Lf = L1 L2.
+// X = L1 Qt1 (Qt1 = X1 X), Qt1 = L2 Qt (Qt = X2 Qt1), so Lf = L1 L2. With X's rows gathered to
+// pivot order the inverse factor is lower triangular.
 mpdo_status lq_b(const float2* X, int rows, int cols, float2* Qt, float2* Lf, int batch, QRWS& w, hipStream_t s) {
-  const long long rr = (long long)rows * rows, rc = (long long)rows * cols;
-  LCK(gram_rows_b(X, rows, cols, rc, w.G, batch, s));
+  const long long rr = (long long)rows * rows;
+  LCK(gram_rows_b(X, rows, cols, (long long)rows * cols, w.G, batch, s));
   LCK(K_PCHOL(rows, batch, w.G, FLOOR, w.L1, w.piv1, w.rank1, w, s));
-  LCK(K_PINV(rows, batch, w.L1, w.piv1, w.rank1, 1, w.F, w, s));
+  LCK(K_PINV(rows, batch, w.L1, w.piv1, w.rank1, 3, w.F, w, s));
+  LCK(k_gather(X, w.piv1, w.rank1, rows, cols, 0, w.A1, batch, s, __LINE__));
   bool one = false;
   LST(one_pass_enough(rows, batch, w, one, s, __LINE__));
   if (one) {   // Qt = X1 X, Lf = L1
-    LCK(mmx(C128_IN_C64_OUT, rows, cols, rows, w.F, rows, rr, 2, X, cols, rc, 1, Qt, cols, rc, batch, s));
+    LCK(tri_left_(__LINE__, rows, cols, w.F, w.A1, Qt, batch, s));
     LCK(k_narrow_factor(w.L1, rows, 1, Lf, batch, s, __LINE__));
     return MPDO_OK;
   }
-  LCK(mmx(C128_IN_C64_OUT, rows, cols, rows, w.F, rows, rr, 2, X, cols, rc, 1, w.A1, cols, rc, batch, s));
-  LCK(gram_rows_b(w.A1, rows, cols, rc, w.G, batch, s));
+  LCK(tri_left_(__LINE__, rows, cols, w.F, w.A1, Qt, batch, s));   // Qt1, held in Qt
+  LCK(gram_rows_b(Qt, rows, cols, (long long)rows * cols, w.G, batch, s));
   LCK(K_PCHOL(rows, batch, w.G, FLOOR, w.L2, w.piv2, w.rank2, w, s));
-  LCK(K_PINV(rows, batch, w.L2, w.piv2, w.rank2, 1, w.F, w, s));
-  LCK(mmx(C128_IN_C64_OUT, rows, cols, rows, w.F, rows, rr, 2, w.A1, cols, rc, 1, Qt, cols, rc, batch, s));
+  LCK(K_PINV(rows, batch, w.L2, w.piv2, w.rank2, 3, w.F, w, s));
+  LCK(k_gather(Qt, w.piv2, w.rank2, rows, cols, 0, w.A1, batch, s, __LINE__));
+  LCK(tri_left_(__LINE__, rows, cols, w.F, w.A1, Qt, batch, s));
   // Lf[i][v] = sum_u L1[i][u] L2[u][v]; L stored by columns
   GemmArgs g{};
   g.M = rows; g.N = rows; g.K = rows; g.batch = batch;

@@ -28,6 +28,9 @@ struct GemmArgs {
   // Hermitian output is then left unmirrored: the caller sums the slices and mirrors)
   int ksplit;
   long long ksA, ksB, ksC;
+  // triangular operand: 1 = B upper triangular (B[k][j] = 0 for k > j), 2 = A lower triangular
+  // (A[i][k] = 0 for k > i); each output tile then reduces over the k range that can be nonzero
+  int tri;
 };
 
 enum Prec { C64_F32ACC = 0,   // complex64 in, fp32 MFMA accumulate, complex64 out
@@ -83,8 +86,16 @@ hipError_t pchol_cond2(int n, int batch, const double2* L, const int* piv, const
 // The Cholesky-QR factor of a pivoted factor, as complex128 X[b] (n x n): with Lt[t][u] =
 // L[piv[t]][u] (t, u < rank), mode 0 gives X[piv[t]][u] = conj(Lt^-1[u][t]) (Q = M X has
 // orthonormal columns when M^H M = L L^H and M = Q L^H), mode 1 X[u][piv[t]] = Lt^-1[u][t]
-// (Qt = X M when M M^H = L L^H, M = L Qt); zero elsewhere. `ws`: pchol_inverse_workspace_bytes.
+// (Qt = X M when M M^H = L L^H, M = L Qt); zero elsewhere. Modes 2 and 3 give the same factors
+// in pivot order, X[t][u] = conj(Lt^-1[u][t]) (upper triangular) and X[u][t] = Lt^-1[u][t] (lower
+// triangular), for Q = gather_piv(M, cols) X and Qt = X gather_piv(M, rows).
+// `ws`: pchol_inverse_workspace_bytes.
 size_t pchol_inverse_workspace_bytes(int n, int batch);
+// Pivot-order gather of complex64 data for the triangular products of modes 2/3: cols = 1
+// out[i][t] = M[i][piv[t]] (M: other x n), cols = 0 out[t][j] = M[piv[t]][j] (M: n x other); zero
+// for t >= rank. Contiguous batch entries of n * other elements.
+hipError_t gather_piv(const float2* M, const int* piv, const int* rank, int n, long long other, int cols, float2* out,
+                      int batch, hipStream_t st);
 hipError_t pchol_inverse_b(int n, int batch, const double2* L, const int* piv, const int* rank, int mode, double2* X,
                            void* ws, hipStream_t s);
 

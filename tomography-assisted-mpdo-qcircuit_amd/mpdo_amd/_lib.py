@@ -65,7 +65,7 @@ def load() -> ctypes.CDLL:
                                 VP, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong,
                                 D, D, D, D, VP]),
         'mpdo_debug_gram_split': (I, [I, I, I, I, VP, VP, VP, VP]),
-        'mpdo_debug_gemm_ex': (I, [I, I, I, I, I, I, I, I, VP, ctypes.c_longlong, ctypes.c_longlong,
+        'mpdo_debug_gemm_ex': (I, [I, I, I, I, I, I, I, I, I, VP, ctypes.c_longlong, ctypes.c_longlong,
                                    ctypes.c_longlong, I, VP, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong,
                                    I, VP, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, D, D, D, D, VP]),
         'mpdo_debug_eigh': (I, [I, I, VP, P(D), VP, P(I), VP]),
@@ -438,11 +438,11 @@ def debug_gram_split(X: torch.Tensor, slices: int) -> torch.Tensor:
 
 def debug_gemm_ex(prec: int, herm: bool, inA: int, inB: int, A: torch.Tensor, B: torch.Tensor, C: torch.Tensor,
                   M: int, N: int, K: int, sa: Tuple[int, int, int], sb: Tuple[int, int, int],
-                  sc: Tuple[int, int, int], conjA=False, conjB=False, alpha=1.0, beta=0.0, batch=1):
+                  sc: Tuple[int, int, int], conjA=False, conjB=False, alpha=1.0, beta=0.0, batch=1, tri=0):
     """debug_gemm with a Hermitian-output flag and per-operand storage (0 = as prec, 1 = complex64,
     2 = complex128)."""
     lib = load()
-    st = lib.mpdo_debug_gemm_ex(prec, int(herm), inA, inB, M, N, K, batch, ctypes.c_void_p(A.data_ptr()), sa[0], sa[1],
+    st = lib.mpdo_debug_gemm_ex(prec, int(herm), inA, inB, int(tri), M, N, K, batch, ctypes.c_void_p(A.data_ptr()), sa[0], sa[1],
                                 sa[2], int(conjA), ctypes.c_void_p(B.data_ptr()), sb[0], sb[1], sb[2], int(conjB),
                                 ctypes.c_void_p(C.data_ptr()), sc[0], sc[1], sc[2],
                                 complex(alpha).real, complex(alpha).imag, complex(beta).real, complex(beta).imag,

@@ -98,6 +98,25 @@ def test_gemm_mixed_operands(cuda, prec, inA, inB, M, N, K, tA, tB):
     assert err < (1e-13 if prec in (1, 2) else 1e-6), (prec, inA, inB, M, N, K, err)
 
 
+@pytest.mark.parametrize('M,N,K,tA', [(16, 4096, 256, True), (32, 640, 40, False), (5, 513, 17, True),
+                                      (31, 1000, 300, False)])
+def test_gemm_few_rows(cuda, M, N, K, tA):
+    """At most 32 output rows and a wide output: the 32 x 128 tile shape; vs numpy."""
+    from mpdo_amd import _lib
+    rng = np.random.default_rng(M * 11 + N + K)
+    batch = 3
+    A = _crand(rng, batch, *((K, M) if tA else (M, K))).astype(np.complex64)
+    B = _crand(rng, batch, K, N).astype(np.complex64)
+    C = torch.zeros(batch, M, N, dtype=torch.complex64, device=cuda)
+    sa = (1, M, M * K) if tA else (K, 1, M * K)
+    _lib.debug_gemm(4, torch.from_numpy(A).to(cuda), torch.from_numpy(B).to(cuda), C, M, N, K, sa, (N, 1, K * N),
+                    (N, 1, M * N), batch=batch)
+    opA = np.swapaxes(A, 1, 2) if tA else A
+    ref = opA.astype(np.complex128) @ B.astype(np.complex128)
+    got = C.cpu().numpy()
+    assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-6
+
+
 @pytest.mark.parametrize('tri', [1, 2])
 @pytest.mark.parametrize('M,N,K', [(300, 130, 130), (130, 257, 130), (64, 64, 64), (17, 40, 40), (40, 17, 40)])
 def test_gemm_triangular_operand(cuda, tri, M, N, K):

@@ -290,6 +290,9 @@ static hipError_t launch_f64(const GemmArgs& g, hipStream_t s) {
       default: break;
     }
   }
+  // at most 32 rows: 32 x 128 tiles (a 64-row tile would leave most matrix-core work idle)
+  if (g.M <= 32 && g.N >= 512 && !g.herm && !g.tri && g_tile_variant == 0)
+    return launch<TInA, TInB, double, TOut, 32, 128, 16>(g, s);
   // a complex128 operand doubles the staging traffic per stage: two LDS stages hide it
   // (43.9 -> 58.7 TFLOP/s at 8192 x 512 x 512); complex64 operands gain nothing from them
   constexpr bool wide = std::is_same<TInA, double2>::value || std::is_same<TInB, double2>::value;

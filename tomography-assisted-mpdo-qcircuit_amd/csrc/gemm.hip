@@ -50,9 +50,10 @@ template <> __device__ inline float2 cvt<double2, float2>(double2 v) { return ma
 // TInA, TInB: operand storage types (float2 or double2, independently); TA: accumulation
 // scalar; TOut: output storage type.
 // NBUF = 2: two LDS stages, the next one stored while the current one feeds the matrix cores
-// (one barrier per stage instead of two).
+// (one barrier per stage instead of two). Register budget for three workgroups per CU with one
+// LDS stage (33 KiB each; 60 -> 66 TFLOP/s at 2048 x 8192 x 512), two with two stages (66 KiB).
 template <typename TInA, typename TInB, typename TA, typename TOut, int BM, int BN, int BK, int NBUF = 1>
-__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
+__global__ __launch_bounds__(256, NBUF == 1 ? 3 : 2) void gemm_kernel(GemmArgs g) {
   typedef typename Acc<TA>::V V;
   typedef typename Acc<TA>::C CA;   // complex in accumulation precision (LDS element)
   constexpr int WM = BM / 2, WN = BN / 2;      // per-wave tile

@@ -50,17 +50,27 @@ template <> __device__ inline float2 cvt<double2, float2>(double2 v) { return ma
 // TInA, TInB: operand storage types (float2 or double2, independently); TA: accumulation
 // scalar; TOut: output storage type.
 // NBUF = 2: two LDS stages, the next one stored while the current one feeds the matrix cores
-// (one barrier per stage instead of two). Register budget for three workgroups per CU with one
-// LDS stage (33 KiB each; 60 -> 66 TFLOP/s at 2048 x 8192 x 512), two with two stages (66 KiB).
+// (one barrier per stage instead of two); a complex64 operand then stays complex64 in LDS and is
+// widened as it is read. Register budget for three workgroups per CU when the LDS allows it (one
+// stage: 33 KiB, 60 -> 66 TFLOP/s at 2048 x 8192 x 512; two stages with one complex64 operand:
+// 50 KiB), two for two complex128 stages (66 KiB).
+template <typename TInA, typename TInB, int NBUF>
+constexpr int gemm_wg_per_cu() {
+  return (NBUF == 1 || !std::is_same<TInA, TInB>::value) ? 3 : 2;
+}
 template <typename TInA, typename TInB, typename TA, typename TOut, int BM, int BN, int BK, int NBUF = 1>
-__global__ __launch_bounds__(256, NBUF == 1 ? 3 : 2) void gemm_kernel(GemmArgs g) {
+__global__ __launch_bounds__(256, (gemm_wg_per_cu<TInA, TInB, NBUF>())) void gemm_kernel(GemmArgs g) {
   typedef typename Acc<TA>::V V;
-  typedef typename Acc<TA>::C CA;   // complex in accumulation precision (LDS element)
+  typedef typename Acc<TA>::C CA;   // complex in accumulation precision
+  // LDS elements: the accumulation type, or the complex64 storage of a complex64 operand when
+  // two stages are kept
+  typedef typename std::conditional<NBUF == 2 && std::is_same<TInA, float2>::value, float2, CA>::type LA;
+  typedef typename std::conditional<NBUF == 2 && std::is_same<TInB, float2>::value, float2, CA>::type LB;
   constexpr int WM = BM / 2, WN = BN / 2;      // per-wave tile
   constexpr int TM = WM / 16, TN = WN / 16;    // 16x16 MFMA tiles per wave
   constexpr int PA = BM * BK / 256, PB = BK * BN / 256;
-  __shared__ CA As[NBUF][BK][BM + 1];
-  __shared__ CA Bs[NBUF][BK][BN + 1];
+  __shared__ LA As[NBUF][BK][BM + 1];
+  __shared__ LB Bs[NBUF][BK][BN + 1];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * WM, wn = (wave & 1) * WN;
@@ -133,7 +143,7 @@ __global__ __launch_bounds__(256, NBUF == 1 ? 3 : 2) void gemm_kernel(GemmArgs g
       int e = aK ? tid * PA + i : i * 256 + tid;
       int row = aK ? e / BK : e % BM;
       int kk = aK ? e % BK : e / BM;
-      CA v = cvt<TInA, CA>(ra[i]);
+      LA v = cvt<TInA, LA>(ra[i]);
       v.y *= sA;
       As[buf][kk][row] = v;
     }
@@ -142,7 +152,7 @@ __global__ __launch_bounds__(256, NBUF == 1 ? 3 : 2) void gemm_kernel(GemmArgs g
       int e = bK ? tid * PB + i : i * 256 + tid;
       int col = bK ? e / BK : e % BN;
       int kk = bK ? e % BK : e / BN;
-      CA v = cvt<TInB, CA>(rb[i]);
+      LB v = cvt<TInB, LB>(rb[i]);
       v.y *= sB;
       Bs[buf][kk][col] = v;
     }
@@ -154,9 +164,9 @@ __global__ __launch_bounds__(256, NBUF == 1 ? 3 : 2) void gemm_kernel(GemmArgs g
     for (int kk = 0; kk < BK; kk += 4) {
       CA a[TM], bb[TN];
 #pragma unroll
-      for (int i = 0; i < TM; ++i) a[i] = As[buf][kk + lk][wm + i * 16 + lr];
+      for (int i = 0; i < TM; ++i) a[i] = cvt<LA, CA>(As[buf][kk + lk][wm + i * 16 + lr]);
 #pragma unroll
-      for (int j = 0; j < TN; ++j) bb[j] = Bs[buf][kk + lk][wn + j * 16 + lr];
+      for (int j = 0; j < TN; ++j) bb[j] = cvt<LB, CA>(Bs[buf][kk + lk][wn + j * 16 + lr]);
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll

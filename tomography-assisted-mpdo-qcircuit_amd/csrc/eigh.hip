@@ -248,11 +248,20 @@ __global__ __launch_bounds__(1024) void bj_solve(BJ s, int r, int inner_sweeps) 
   int I, J;
   block_pair(P, r, pi, I, J);
   const double2* Ab = s.A + (long long)b * s.N * s.N;
+  // A keeps its lower block triangle only: the (I, J) block (I < J) is the conjugate transpose
+  // of the stored (J, I) block; every 32 x 32 block is read along its stored rows (coalesced)
+  for (int q = tid; q < NS * NS; q += blockDim.x) {
+    const int sb = q >> 10, e = q & 1023, sr = sb >> 1, sc = sb & 1;
+    const int R = sr ? J : I, C = sc ? J : I, u = e >> 5, v = e & 31;
+    if (R >= C) {
+      A[sr * 32 + u][sc * 32 + v] = Ab[(long long)(R * 32 + u) * s.N + C * 32 + v];
+    } else {
+      const double2 t = Ab[(long long)(C * 32 + u) * s.N + R * 32 + v];
+      A[sr * 32 + v][sc * 32 + u] = make_double2(t.x, -t.y);
+    }
+  }
   for (int e = tid; e < NS * NS; e += blockDim.x) {
-    int i = e / NS, j = e % NS;
-    int gi = (i < 32 ? I * 32 + i : J * 32 + i - 32);
-    int gj = (j < 32 ? I * 32 + j : J * 32 + j - 32);
-    A[i][j] = Ab[(long long)gi * s.N + gj];
+    const int i = e / NS, j = e % NS;
     V[i][j] = make_double2(i == j ? 1.0 : 0.0, 0);
   }
   if (tid == 0) cnt = 0;
@@ -374,9 +383,9 @@ __device__ __forceinline__ void rotate_body(BJ s, int r, int cols, int z, int ti
 
 
 // Two-sided update of A by the round's block rotations, one 64x64 block pair per workgroup:
-// A[R, C] <- W_R^H A[R, C] W_C for the lower block triangle (R >= C in pair order), the block
-// A[C, R] written as its conjugate transpose (A stays Hermitian), so every block of A is read
-// and written once per round. R = I_pr u J_pr, C = I_pc u J_pc (two 32-wide blocks each).
+// A[R, C] <- W_R^H A[R, C] W_C for the block pairs pr >= pc, R = I_pr u J_pr, C = I_pc u J_pc
+// (two 32-wide blocks each). A is Hermitian and only its lower block triangle is kept (32 x 32
+// blocks (a, b) with a >= b), so every stored block is read and written once per round.
 __device__ inline void mfma_tile64(const double2 (*X)[LD], const double2* Wg, bool w_left, doublex4 (&re)[2][2],
                                    doublex4 (&im)[2][2], int r0, int c0, int li, int lk) {
   // w_left: out = W^H X (P[i][k] = conj(W[k][i])); else out = X W (Q[k][j] = W[k][j])
@@ -431,11 +440,17 @@ __device__ __forceinline__ void sym_body(BJ s, int r, int e, int b, double2 (*X)
   double2* A = s.A + (long long)b * s.N * s.N;
   const double2* Wr = s.W + ((long long)b * (P / 2) + pr) * NS * NS;
   const double2* Wc = s.W + ((long long)b * (P / 2) + pc) * NS * NS;
-  auto grow = [&](int i) { return i < 32 ? Ir * 32 + i : Jr * 32 + i - 32; };
-  auto gcol = [&](int j) { return j < 32 ? Ic * 32 + j : Jc * 32 + j - 32; };
+  // the four 32 x 32 sub-blocks (row block Rb, column block Cb) of the tile, each held in its
+  // stored orientation: directly when Rb >= Cb, else as the conjugate transpose of (Cb, Rb)
   for (int q = tid; q < NS * NS; q += blockDim.x) {
-    int i = q / NS, j = q % NS;
-    X[i][j] = A[(long long)grow(i) * s.N + gcol(j)];
+    const int sb = q >> 10, e = q & 1023, sr = sb >> 1, sc = sb & 1;
+    const int Rb = sr ? Jr : Ir, Cb = sc ? Jc : Ic, u = e >> 5, v = e & 31;
+    if (Rb >= Cb) {
+      X[sr * 32 + u][sc * 32 + v] = A[(long long)(Rb * 32 + u) * s.N + Cb * 32 + v];
+    } else {
+      const double2 t = A[(long long)(Cb * 32 + u) * s.N + Rb * 32 + v];
+      X[sr * 32 + v][sc * 32 + u] = make_double2(t.x, -t.y);
+    }
   }
   __syncthreads();
   const int wave = tid / 64, lane = tid % 64, li = lane % 16, lk = lane / 16;
@@ -467,17 +482,27 @@ __device__ __forceinline__ void sym_body(BJ s, int r, int e, int b, double2 (*X)
           im[a][c][q] = v.y;
         }
   }
+  __syncthreads();
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int c = 0; c < 2; ++c)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int i = r0 + 16 * a + 4 * q + lk, j = c0 + 16 * c + li;
-        const double2 o = make_double2(re[a][c][q], im[a][c][q]);
-        A[(long long)grow(i) * s.N + gcol(j)] = o;
-        if (pr != pc) A[(long long)gcol(j) * s.N + grow(i)] = make_double2(o.x, -o.y);
-      }
+      for (int q = 0; q < 4; ++q)
+        X[r0 + 16 * a + 4 * q + lk][c0 + 16 * c + li] = make_double2(re[a][c][q], im[a][c][q]);
+  __syncthreads();
+  // back in the stored orientations; on the diagonal tile (pr == pc) the upper sub-block's
+  // storage is the lower sub-block of the same tile, written directly
+  for (int q = tid; q < NS * NS; q += blockDim.x) {
+    const int sb = q >> 10, e = q & 1023, sr = sb >> 1, sc = sb & 1;
+    const int Rb = sr ? Jr : Ir, Cb = sc ? Jc : Ic, u = e >> 5, v = e & 31;
+    if (Rb >= Cb) {
+      A[(long long)(Rb * 32 + u) * s.N + Cb * 32 + v] = X[sr * 32 + u][sc * 32 + v];
+    } else if (pr != pc) {
+      const double2 t = X[sr * 32 + v][sc * 32 + u];
+      A[(long long)(Cb * 32 + u) * s.N + Rb * 32 + v] = make_double2(t.x, -t.y);
+    }
+  }
 }
 
 // One launch per round for both updates, which touch disjoint data: workgroups x < nsym update

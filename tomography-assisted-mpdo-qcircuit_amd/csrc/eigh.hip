@@ -519,7 +519,8 @@ __global__ __launch_bounds__(256) void bj_update(BJ s, int r, int nsym, int ntil
   }
 }
 
-// eigenvalues = diag(A), sorted descending (stable), eigenvectors = matching columns of V
+// eigenvalues = diag(A), sorted descending (stable); the matching column order of V goes to
+// order_ws for bj_gather_v
 __global__ __launch_bounds__(256) void bj_finish(BJ s, double* w, long long ldw, double2* Vg, long long ldv,
                                                  long long bsV, int* order_ws) {
   const int b = blockIdx.x, tid = threadIdx.x;
@@ -540,9 +541,18 @@ __global__ __launch_bounds__(256) void bj_finish(BJ s, double* w, long long ldw,
     int o = order[j];
     w[(long long)b * ldw + j] = Ab[(long long)o * s.N + o].x;
   }
+  (void)Vb; (void)Vg; (void)ldv; (void)bsV;
+}
+
+// eigenvectors = V's columns in the order bj_finish found (grid over the elements of each matrix)
+__global__ void bj_gather_v(BJ s, double2* Vg, long long ldv, long long bsV, const int* order_ws) {
+  const int b = blockIdx.y;
+  const double2* Vb = s.V + (long long)b * s.N * s.N;
+  const int* order = order_ws + (long long)b * s.N;
   double2* Vo = Vg + (long long)b * bsV;
-  for (long long e = tid; e < (long long)s.n * s.n; e += blockDim.x) {
-    int i = (int)(e / s.n), j = (int)(e % s.n);
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < (long long)s.n * s.n;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(e / s.n), j = (int)(e % s.n);
     Vo[(long long)i * ldv + j] = Vb[(long long)i * s.N + order[j]];
   }
 }
@@ -630,6 +640,11 @@ hipError_t eigh(int n, int batch, void* A, long long lda, long long bsA, double*
     if (!any) break;
   }
   hipLaunchKernelGGL(bj_finish, dim3(batch), dim3(256), 0, st, s, w, ldw, (double2*)V, ldv, bsV, order);
+  {
+    const long long nn = (long long)n * n;
+    const int blocks = (int)std::min<long long>((nn + 255) / 256, 1024);
+    hipLaunchKernelGGL(bj_gather_v, dim3(blocks, batch), dim3(256), 0, st, s, (double2*)V, ldv, bsV, order);
+  }
   if (sweeps_out) *sweeps_out = sw + 1;
   prof::eig_log(n, batch, sw + 1);
   return hipGetLastError();

@@ -7,10 +7,14 @@ depolarizing noise p = 0.002, chi = 256, kappa = 8, 32 copies whose angles diffe
 U[-0.01, 0.01] (mpdo_amd.workloads.config4). One step = one circuit-layer of all 32 copies:
 the composed single-qubit channel of every site, CNOT + two-qubit depolarizing on every pair of
 the layer's parity and the truncation (QR sweep, chi sweep, kappa cut), as one call of
-mpdo_layer_batched. The chains enter in the steady state of the circuit: synthetic complex64
-site tensors (seeded, decaying bond and inner weights) at the shapes a long run of this circuit
-settles into; successive steps alternate the parity and feed each other. Inputs are resident in
-HBM when the timed region starts.
+mpdo_layer_batched. The chains start from synthetic complex64 site tensors (seeded, decaying bond
+and inner weights) at the shapes a long run of this circuit settles into; successive layers
+alternate the parity and feed each other. The warmup layers bring the spectra to the circuit's
+own steady state as well: the gate-time split keeps ~458 of its 1024 directions on the synthetic
+start, then 780, 950, and from the sixth layer on 1017-1024 (the split no longer truncates and
+every later eigenproblem has order ~1024, twice the cost of the first layer), so the default
+times two layers after six warmup layers. Inputs are resident in HBM when the timed region
+starts.
 
 Prints one JSON line: circuit-layers/s (32 x steps / seconds), the roofline of the dominant
 kernel class (algorithmic flops per launch / launch duration, HIP events on the launch stream),
@@ -136,8 +140,8 @@ def cpu_baseline(Ti, Tj, i, threads: int = 16):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split('\n\n')[0])
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=1)
-    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=2)
+    ap.add_argument('--warmup', type=int, default=6)
     ap.add_argument('--copies', type=int, default=32)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-threads', type=int, default=16)

@@ -295,93 +295,6 @@ __global__ __launch_bounds__(1024) void bj_solve(BJ s, int r, int inner_sweeps) 
   }
 }
 
-// X[rows, K u L] <- X[rows, K u L] * W_KL   (cols=1)   or
-// X[K u L, cols] <- W_KL^H * X[K u L, cols] (cols=0), in place, the 64x64 tile `tile` of block
-// pair pi per workgroup. z selects the matrix: z < batch -> A[z], else V[z - batch] (columns
-// only). Only the tile is staged in LDS (X, 66.5 KiB: two workgroups per CU); the rotation W of
-// the block pair is read from global memory, where every tile of the pair finds it in L2.
-__device__ __forceinline__ void rotate_body(BJ s, int r, int cols, int z, int tile, int pi, double2 (*X)[LD]) {
-  const int P = s.N / 32, t0 = tile * NS, tid = threadIdx.x;
-  bool isV = z >= s.batch;
-  int b = isV ? z - s.batch : z;
-  int I, J;
-  block_pair(P, r, pi, I, J);
-  if (s.ident[b * (P / 2) + pi]) return;   // X J = X (uniform over the workgroup)
-  double2* M = (isV ? s.V : s.A) + (long long)b * s.N * s.N;
-  const double2* Wb = s.W + ((long long)b * (P / 2) + pi) * NS * NS;
-  for (int e = tid; e < NS * NS; e += blockDim.x) {
-    int i = e / NS, j = e % NS;
-    if (cols) {
-      int gj = (j < 32 ? I * 32 + j : J * 32 + j - 32);
-      X[i][j] = M[(long long)(t0 + i) * s.N + gj];
-    } else {
-      int gi = (i < 32 ? I * 32 + i : J * 32 + i - 32);
-      X[i][j] = M[(long long)gi * s.N + t0 + j];
-    }
-  }
-  __syncthreads();
-  // out = P Q (64x64 complex) on the fp64 matrix cores: P = X, Q = W for column updates, P = W^H,
-  // Q = X for row updates. Wave w computes the 32x32 quadrant (w/2, w%2) as 2x2 tiles of 16x16,
-  // four real v_mfma_f64_16x16x4f64 per tile and k-step (Re += Pr Qr - Pi Qi, Im += Pr Qi + Pi Qr).
-  // Operand layouts: P[i][k] in lane i + 16k, Q[k][j] in lane j + 16k; result D[i][j] in lane
-  // j + 16 (i % 4), item i / 4.
-  const int wave = tid / 64, lane = tid % 64, li = lane % 16, lk = lane / 16;
-  const int r0 = (wave / 2) * 32, c0 = (wave % 2) * 32;
-  doublex4 re[2][2], im[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      re[a][c] = doublex4{0, 0, 0, 0};
-      im[a][c] = doublex4{0, 0, 0, 0};
-    }
-  for (int kk = 0; kk < NS; kk += 4) {
-    const int k = kk + lk;
-    double pr[2], pi_[2], pn[2], qr[2], qi[2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-      const int row = r0 + 16 * a + li;
-      double2 v;
-      if (cols) v = X[row][k];
-      else { v = Wb[k * NS + row]; v.y = -v.y; }
-      pr[a] = v.x; pi_[a] = v.y; pn[a] = -v.y;
-    }
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const int col = c0 + 16 * c + li;
-      double2 v = cols ? Wb[k * NS + col] : X[k][col];
-      qr[c] = v.x; qi[c] = v.y;
-    }
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        re[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(pr[a], qr[c], re[a][c], 0, 0, 0);
-        re[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(pn[a], qi[c], re[a][c], 0, 0, 0);
-        im[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(pr[a], qi[c], im[a][c], 0, 0, 0);
-        im[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(pi_[a], qr[c], im[a][c], 0, 0, 0);
-      }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int c = 0; c < 2; ++c)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int i = r0 + 16 * a + 4 * q + lk, j = c0 + 16 * c + li;
-        const double2 o = make_double2(re[a][c][q], im[a][c][q]);
-        if (cols) {
-          int gj = (j < 32 ? I * 32 + j : J * 32 + j - 32);
-          M[(long long)(t0 + i) * s.N + gj] = o;
-        } else {
-          int gi = (i < 32 ? I * 32 + i : J * 32 + i - 32);
-          M[(long long)gi * s.N + t0 + j] = o;
-        }
-      }
-}
-
-
 // Two-sided update of A by the round's block rotations, one 64x64 block pair per workgroup:
 // A[R, C] <- W_R^H A[R, C] W_C for the block pairs pr >= pc, R = I_pr u J_pr, C = I_pc u J_pc
 // (two 32-wide blocks each). A is Hermitian and only its lower block triangle is kept (32 x 32
@@ -505,18 +418,70 @@ __device__ __forceinline__ void sym_body(BJ s, int r, int e, int b, double2 (*X)
   }
 }
 
-// One launch per round for both updates, which touch disjoint data: workgroups x < nsym update
-// A's block pair x (two-sided, sym_body), the rest V's tiles (V <- V J, rotate_body: tile
-// (x - nsym) % ntile of block pair (x - nsym) / ntile); blockIdx.y = matrix.
-__global__ __launch_bounds__(256) void bj_update(BJ s, int r, int nsym, int ntile) {
+// A's two-sided update by the round's rotations: workgroup x updates block pair x (sym_body);
+// blockIdx.y = matrix. V's update (independent of it) is bj_rotate_v_direct.
+__global__ __launch_bounds__(256) void bj_update(BJ s, int r) {
   __shared__ double2 X[NS][LD];
-  const int x = blockIdx.x, b = blockIdx.y;
-  if (x < nsym) {
-    sym_body(s, r, x, b, X);
-  } else {
-    const int t = x - nsym;
-    rotate_body(s, r, 1, b + s.batch, t % ntile, t / ntile, X);
+  sym_body(s, r, blockIdx.x, blockIdx.y, X);
+}
+
+// V <- V W on the columns of the round's block pairs without LDS: each wave reads its rows of the
+// 64 x 64 tile and the pair's rotation straight into the matrix-core operands (no LDS, so three
+// workgroups per CU), and every wave has read the whole tile before any wave writes it back.
+__global__ __launch_bounds__(256, 3) void bj_rotate_v_direct(BJ s, int r, int ntile) {
+  const int P = s.N / 32, t = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+  const int tile = t % ntile, pi = t / ntile, t0 = tile * NS;
+  if (s.ident[b * (P / 2) + pi]) return;   // V W = V (uniform over the workgroup)
+  int I, J;
+  block_pair(P, r, pi, I, J);
+  double2* M = s.V + (long long)b * s.N * s.N;
+  const double2* Wb = s.W + ((long long)b * (P / 2) + pi) * NS * NS;
+  const int wave = tid / 64, lane = tid % 64, li = lane % 16, lk = lane / 16;
+  const int r0 = (wave / 2) * 32, c0 = (wave % 2) * 32;
+  doublex4 re[2][2], im[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      re[a][c] = doublex4{0, 0, 0, 0};
+      im[a][c] = doublex4{0, 0, 0, 0};
+    }
+#pragma unroll 4
+  for (int kk = 0; kk < NS; kk += 4) {
+    const int k = kk + lk;
+    const int gk = k < 32 ? I * 32 + k : J * 32 + k - 32;
+    double pr[2], pim[2], pn[2], qr[2], qi[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const double2 v = M[(long long)(t0 + r0 + 16 * a + li) * s.N + gk];
+      pr[a] = v.x; pim[a] = v.y; pn[a] = -v.y;
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const double2 v = Wb[k * NS + c0 + 16 * c + li];
+      qr[c] = v.x; qi[c] = v.y;
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        re[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(pr[a], qr[c], re[a][c], 0, 0, 0);
+        re[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(pn[a], qi[c], re[a][c], 0, 0, 0);
+        im[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(pr[a], qi[c], im[a][c], 0, 0, 0);
+        im[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(pim[a], qr[c], im[a][c], 0, 0, 0);
+      }
   }
+  __syncthreads();   // the tile is rewritten in place: every read above precedes every write below
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = r0 + 16 * a + 4 * q + lk, j = c0 + 16 * c + li;
+        const int gj = j < 32 ? I * 32 + j : J * 32 + j - 32;
+        M[(long long)(t0 + i) * s.N + gj] = make_double2(re[a][c][q], im[a][c][q]);
+      }
 }
 
 // eigenvalues = diag(A), sorted descending (stable); the matching column order of V goes to
@@ -623,12 +588,12 @@ hipError_t eigh(int n, int batch, void* A, long long lda, long long bsA, double*
         hipLaunchKernelGGL(bj_solve, dim3(P / 2, batch), dim3(g_bj_threads), 0, st, s, r, 1);
       }
       {   // A <- J^H A J block pair by block pair (Hermitian: lower triangle + mirror), and
-          // V <- V J on the columns of the round's block pairs, in one launch
+          // V <- V J on the columns of the round's block pairs
         prof::Scope ps(prof::EIG_ROTATE, st, (tiles_sym * 2 + tiles_r) * tile_flops,
                        (tiles_sym + tiles_r) * tile_bytes);
         const int ntile = s.N / NS;
-        hipLaunchKernelGGL(bj_update, dim3(npairs_sym + ntile * (P / 2), batch), dim3(256), 0, st, s, r, npairs_sym,
-                           ntile);
+        hipLaunchKernelGGL(bj_update, dim3(npairs_sym, batch), dim3(256), 0, st, s, r);
+        hipLaunchKernelGGL(bj_rotate_v_direct, dim3(ntile * (P / 2), batch), dim3(256), 0, st, s, r, ntile);
       }
     }
     if ((err = hipGetLastError()) != hipSuccess) return err;

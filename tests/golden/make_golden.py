@@ -1,5 +1,5 @@
-"""Writes the oracle's outputs for the configs[0] workload (8 qubits, 4 layers, chi=32,
-kappa=4) to config0_oracle.npz: the full 256x256 density matrix, the trace and the discarded
+"""Writes the oracle's outputs for the 8-qubit brickwork workload (4 layers, chi=32,
+kappa=4) to brickwork8_chi32_oracle.npz: the full 256x256 density matrix, the trace and the discarded
 weight of every cut. Run from the repository root: python tests/golden/make_golden.py"""
 import os
 import sys
@@ -16,12 +16,12 @@ from oracle import mpdo_oracle as O  # noqa: E402
 from oracle import dense as D  # noqa: E402
 
 if __name__ == '__main__':
-    w = W.config0()
+    w = W.brickwork8_chi32()
     st = bridge.run_oracle(w)
     rho = O.density_matrix(st)
     dense = D.dense_evolve(bridge.oracle_ops(w), w.n, w.noise_type, w.ideal, bridge.oracle_noise(w))
     kinds = np.array([d[0] for d in st.discarded])
-    np.savez_compressed(os.path.join(HERE, "config0_oracle.npz"), rho=rho,
+    np.savez_compressed(os.path.join(HERE, "brickwork8_chi32_oracle.npz"), rho=rho,
                         trace=np.trace(rho).real, disc_kind=kinds,
                         disc_index=np.array([d[1] for d in st.discarded]),
                         disc_weight=np.array([d[2] for d in st.discarded]),

@@ -143,12 +143,12 @@ def test_realnoise_builder_rotations(cuda):
     assert _rho_err(rho, O.density_matrix(st)) < 1e-5
 
 
-def test_config0_golden(cuda):
-    """configs[0]: 8 qubits, 4 brickwork layers, dep1 p=1e-3, dep2 p=1e-2, chi=32, kappa=4; the
+def test_brickwork8_chi32_golden(cuda):
+    """8 qubits, 4 brickwork layers, dep1 p=1e-3, dep2 p=1e-2, chi=32, kappa=4; the
     full 256x256 density matrix against the oracle fixture (tests/golden/make_golden.py)."""
     from mpdo_amd import workloads as W
-    g = np.load(os.path.join(HERE, 'golden', 'config0_oracle.npz'))
-    w = W.config0()
+    g = np.load(os.path.join(HERE, 'golden', 'brickwork8_chi32_oracle.npz'))
+    w = W.brickwork8_chi32()
     c, state = _run_product(w, cuda)
     rho = c.cal_dm().cpu().numpy()
     assert _rho_err(rho, g['rho']) < 1e-5

@@ -288,10 +288,10 @@ def test_kappa_cut_of_a_single_site_keeps_its_top_eigenvectors():
 
 
 def test_golden_fixture_is_reproduced():
-    """tests/golden/config0_oracle.npz (written by tests/golden/make_golden.py) is what the oracle
+    """tests/golden/brickwork8_chi32_oracle.npz (written by tests/golden/make_golden.py) is what the oracle
     gives for the 8-qubit configuration now."""
     import bridge
     from mpdo_amd import workloads as W
-    g = np.load(os.path.join(HERE, 'golden', 'config0_oracle.npz'))
-    st = bridge.run_oracle(W.config0())
+    g = np.load(os.path.join(HERE, 'golden', 'brickwork8_chi32_oracle.npz'))
+    st = bridge.run_oracle(W.brickwork8_chi32())
     assert np.allclose(O.density_matrix(st), g['rho'], atol=1e-12)

@@ -61,10 +61,10 @@ def brickwork(n: int, depth: int, rng: np.random.Generator, two: Sequence[str] =
     return ops
 
 
-def config0(seed: int = 101) -> Workload:
+def brickwork8_chi32(seed: int = 101) -> Workload:
     """8 qubits, 4 layers, CNOT brickwork, dep1 p=1e-3 (1q), dep2 p=1e-2 (2q), chi=32, kappa=4."""
     rng = np.random.Generator(np.random.PCG64(seed))
-    return Workload('config0', 8, brickwork(8, 4, rng), 'idealNoise', False, 32, 4,
+    return Workload('brickwork8_chi32', 8, brickwork(8, 4, rng), 'idealNoise', False, 32, 4,
                     chip=dict(dpc_error_rate=0.01), single_qubit_noise='depolarizing', p1=1e-3, depth=4)
 
 

@@ -203,7 +203,8 @@ def test_eigh_rank_deficient(cuda, n, rank):
         assert np.abs(V[b].conj().T @ V[b] - np.eye(n)).max() <= 1e-11
 
 
-@pytest.mark.parametrize('n,rank', [(100, 100), (256, 256), (512, 300), (1024, 450), (200, 40)])
+@pytest.mark.parametrize('n,rank', [(100, 100), (256, 256), (512, 300), (1024, 450), (200, 40), (640, 600),
+                                    (1024, 1024)])
 def test_eigh_gram(cuda, n, rank):
     """The pivoted-Cholesky-preconditioned solver on graded Gram matrices (12 decades) of full and
     deficient rank: eigenvalues above its floor as numpy's; eigenvectors with residual within the

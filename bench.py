@@ -224,7 +224,7 @@ def main():
 
     # the dominant kernel: mpdo::gemm_kernel, the kernel with the largest share of the steady-state
     # step (30% of kernel time over its operand/tile instantiations; the block-Jacobi rounds are
-    # three kernels, bj_update 24%, bj_solve 21%, bj_rotate_v_direct 18%, whose per-round tile
+    # three kernels, bj_update 24%, bj_solve 20.5%, bj_rotate_v_direct 18%, whose per-round tile
     # counts include the identity rotations they skip, so their flop totals are upper bounds and
     # are not used for a roofline)
     name = 'gemm'

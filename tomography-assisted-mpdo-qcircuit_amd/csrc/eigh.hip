@@ -121,6 +121,83 @@ __device__ void jacobi_round_blk(double2 (*A)[LD], double2 (*V)[LD], Rot* rot, i
   __syncthreads();
 }
 
+// The same round without V: A <- J^H A J, and the round's rotations go to the log rl[0 .. N/2)
+// as (cs, sn, wr, wi) (the identity (1, 0, 1, 0) where a pair is not rotated), for replay_round.
+__device__ void jacobi_round_log(double2 (*A)[LD], Rot* rot, double4* rl, int N, int r, double floor_abs, int* cnt) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int np = N / 2;
+  if (tid < np) {
+    int p, q;
+    pair_of(N, r, tid, p, q);
+    if (p > q) { int t = p; p = q; q = t; }
+    double a = A[p][p].x, b = A[q][q].x;
+    double2 c = A[p][q];
+    double ac = sqrt(c.x * c.x + c.y * c.y);
+    Rot R; R.on = 0; R.cs = 1; R.sn = 0; R.wr = 1; R.wi = 0;
+    if (ac > floor_abs && ac > REL_TOL * sqrt(fabs(a * b))) {
+      double zeta = (b - a) / (2.0 * ac);
+      double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+      R.cs = 1.0 / sqrt(1.0 + t * t);
+      R.sn = t * R.cs;
+      R.wr = c.x / ac; R.wi = c.y / ac;
+      R.on = 1;
+      atomicAdd(cnt, 1);
+    }
+    rot[tid] = R;
+    rl[tid] = make_double4(R.cs, R.sn, R.wr, R.wi);
+  }
+  __syncthreads();
+  for (int task = tid; task < np * np; task += nt) {
+    const int bi = task / np, bj = task % np;
+    const Rot Ri = rot[bi], Rj = rot[bj];
+    if (!Ri.on && !Rj.on) continue;
+    int pi, qi, pj, qj;
+    pair_of(N, r, bi, pi, qi);
+    if (pi > qi) { int t = pi; pi = qi; qi = t; }
+    pair_of(N, r, bj, pj, qj);
+    if (pj > qj) { int t = pj; pj = qj; qj = t; }
+    double2 b00 = A[pi][pj], b01 = A[pi][qj], b10 = A[qi][pj], b11 = A[qi][qj];
+    if (Rj.on) {   // B J_j
+      double2 x0 = rot_x(Rj, b00, b01, false, false), y0 = rot_x(Rj, b00, b01, false, true);
+      double2 x1 = rot_x(Rj, b10, b11, false, false), y1 = rot_x(Rj, b10, b11, false, true);
+      b00 = x0; b01 = y0; b10 = x1; b11 = y1;
+    }
+    if (Ri.on) {   // J_i^H B
+      double2 x0 = rot_x(Ri, b00, b10, true, false), y0 = rot_x(Ri, b00, b10, true, true);
+      double2 x1 = rot_x(Ri, b01, b11, true, false), y1 = rot_x(Ri, b01, b11, true, true);
+      b00 = x0; b10 = y0; b01 = x1; b11 = y1;
+    }
+    if (bi == bj) {   // the rotated pair: exactly diagonal, real diagonal
+      b01 = make_double2(0, 0);
+      b10 = make_double2(0, 0);
+      b00.y = 0;
+      b11.y = 0;
+    }
+    A[pi][pj] = b00; A[pi][qj] = b01; A[qi][pj] = b10; A[qi][qj] = b11;
+  }
+  __syncthreads();
+}
+
+// V <- V J with round r's logged rotations (the V update of jacobi_round_blk, same arithmetic in
+// the same order per element, so the accumulated V is the same to the bit)
+__device__ void replay_round(double2 (*V)[LD], const double4* rl, int N, int r) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int np = N / 2;
+  for (int task = tid; task < np * N; task += nt) {
+    const int i = task / N, k = task % N;
+    const double4 g = rl[i];
+    if (g.x == 1.0 && g.y == 0.0 && g.z == 1.0 && g.w == 0.0) continue;   // not rotated
+    Rot R; R.cs = g.x; R.sn = g.y; R.wr = g.z; R.wi = g.w; R.on = 1;
+    int p, q;
+    pair_of(N, r, i, p, q);
+    if (p > q) { int t = p; p = q; q = t; }
+    const double2 x = V[k][p], y = V[k][q];
+    V[k][p] = rot_x(R, x, y, false, false);
+    V[k][q] = rot_x(R, x, y, false, true);
+  }
+  __syncthreads();
+}
+
 __device__ double max_diag(double2 (*A)[LD], int N, double* red) {
   const int tid = threadIdx.x;
   double m = 0;
@@ -202,6 +279,7 @@ struct BJ {
   int* flag;    // [batch]
   int* ident;   // [batch][N/64]: the round's rotation of block pair i is the identity
   double* floor_abs;   // [batch]: ABS_TOL * max diagonal of the whole matrix
+  double4* rlog;       // [batch][N/64][63][32]: the rotations of each subproblem's sweep
 };
 
 __global__ __launch_bounds__(256) void bj_floor(BJ s) {
@@ -236,12 +314,14 @@ __device__ inline void block_pair(int P, int r, int i, int& I, int& J) {
   if (I > J) { int t = I; I = J; J = t; }
 }
 
-// one workgroup per (block pair, matrix): diagonalise the 64x64 principal submatrix (any
-// multiple of 64 threads up to 1024: the round's rotations, 2x2-block updates and V columns are
-// strided over the workgroup)
-__global__ __launch_bounds__(1024) void bj_solve(BJ s, int r, int inner_sweeps) {
-  __shared__ double2 A[NS][LD];
-  __shared__ double2 V[NS][LD];
+// one workgroup per (block pair, matrix): one sweep of the Jacobi method on the 64x64 principal
+// submatrix (any multiple of 64 threads up to 1024: the round's rotations, 2x2-block updates and
+// V columns are strided over the workgroup). The sweep's rotations are logged to global memory
+// and then replayed on the identity in the same LDS array, which is free once A's rounds are
+// done: one 64x64 complex128 array (67 KB of LDS with the rotations) lets two workgroups share a
+// CU, so a round's 512 subproblems at order 1024 run in one wave instead of two.
+__global__ __launch_bounds__(1024) void bj_solve(BJ s, int r) {
+  __shared__ double2 A[NS][LD];   // the subproblem, then W
   __shared__ Rot rot[NS / 2];
   __shared__ int cnt, need;
   const int P = s.N / 32, pi = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
@@ -259,10 +339,6 @@ __global__ __launch_bounds__(1024) void bj_solve(BJ s, int r, int inner_sweeps) 
       const double2 t = Ab[(long long)(C * 32 + u) * s.N + R * 32 + v];
       A[sr * 32 + v][sc * 32 + u] = make_double2(t.x, -t.y);
     }
-  }
-  for (int e = tid; e < NS * NS; e += blockDim.x) {
-    const int i = e / NS, j = e % NS;
-    V[i][j] = make_double2(i == j ? 1.0 : 0.0, 0);
   }
   if (tid == 0) cnt = 0;
   __syncthreads();
@@ -284,11 +360,15 @@ __global__ __launch_bounds__(1024) void bj_solve(BJ s, int r, int inner_sweeps) 
     if (tid == 0) s.ident[b * (P / 2) + pi] = 1;
     return;
   }
-  for (int sw = 0; sw < inner_sweeps; ++sw) {
-    for (int rr = 0; rr < NS - 1; ++rr) jacobi_round_blk(A, V, rot, NS, rr, fl, &cnt);
-  }
+  double4* rl = s.rlog + ((long long)b * (P / 2) + pi) * (NS - 1) * (NS / 2);
+  for (int rr = 0; rr < NS - 1; ++rr) jacobi_round_log(A, rot, rl + rr * (NS / 2), NS, rr, fl, &cnt);
+  // W = J_0 J_1 ... J_62 accumulated in A's storage (every round above ended with a barrier)
+  for (int e = tid; e < NS * NS; e += blockDim.x) A[e / NS][e % NS] = make_double2(e / NS == e % NS ? 1.0 : 0.0, 0);
+  __threadfence_block();   // the log entries written by other threads of the workgroup
+  __syncthreads();
+  for (int rr = 0; rr < NS - 1; ++rr) replay_round(A, rl + rr * (NS / 2), NS, rr);
   double2* Wb = s.W + ((long long)b * (P / 2) + pi) * NS * NS;
-  for (int e = tid; e < NS * NS; e += blockDim.x) Wb[e] = V[e / NS][e % NS];
+  for (int e = tid; e < NS * NS; e += blockDim.x) Wb[e] = A[e / NS][e % NS];
   if (tid == 0) {
     if (cnt > 0) atomicOr(&s.flag[b], 1);
     s.ident[b * (P / 2) + pi] = cnt == 0;
@@ -531,7 +611,8 @@ size_t eigh_workspace_bytes(int n, int batch) {
   size_t N = (size_t)((n + 63) / 64) * 64;
   size_t mats = align256(N * N * sizeof(double2) * batch);
   size_t w = align256((N / 64) * NS * NS * sizeof(double2) * batch);
-  return 2 * mats + w + align256(sizeof(int) * batch) + align256(sizeof(int) * (N / 64) * batch) +
+  size_t rlog = align256((N / 64) * (NS - 1) * (NS / 2) * sizeof(double4) * batch);
+  return 2 * mats + w + rlog + align256(sizeof(int) * batch) + align256(sizeof(int) * (N / 64) * batch) +
          align256(sizeof(double) * batch) + align256(sizeof(int) * N * batch) + 256;
 }
 
@@ -564,6 +645,7 @@ hipError_t eigh(int n, int batch, void* A, long long lda, long long bsA, double*
   s.A = (double2*)p; p += mats;
   s.V = (double2*)p; p += mats;
   s.W = (double2*)p; p += align256((size_t)(s.N / 64) * NS * NS * sizeof(double2) * batch);
+  s.rlog = (double4*)p; p += align256((size_t)(s.N / 64) * (NS - 1) * (NS / 2) * sizeof(double4) * batch);
   s.flag = (int*)p; p += align256(sizeof(int) * batch);
   s.ident = (int*)p; p += align256(sizeof(int) * (s.N / 64) * batch);
   s.floor_abs = (double*)p; p += align256(sizeof(double) * batch);
@@ -585,7 +667,7 @@ hipError_t eigh(int n, int batch, void* A, long long lda, long long bsA, double*
     for (int r = 0; r < P - 1; ++r) {
       {
         prof::Scope ps(prof::EIG_SOLVE, st, solve_flops, solve_bytes);
-        hipLaunchKernelGGL(bj_solve, dim3(P / 2, batch), dim3(g_bj_threads), 0, st, s, r, 1);
+        hipLaunchKernelGGL(bj_solve, dim3(P / 2, batch), dim3(g_bj_threads), 0, st, s, r);
       }
       {   // A <- J^H A J block pair by block pair (Hermitian: lower triangle + mirror), and
           // V <- V J on the columns of the round's block pairs

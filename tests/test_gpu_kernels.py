@@ -184,6 +184,33 @@ def test_eigh(cuda, n):
         assert np.abs(V[b].conj().T @ V[b] - np.eye(n)).max() <= 1e-12
 
 
+@pytest.mark.parametrize('n', [130, 300])
+def test_eigh_block_threads(cuda, n):
+    """The blocked solver's subproblem kernel strides its rounds, its rotation log and the replay
+    of that log into W over any workgroup size: 256, 512 and 1024 threads give the same
+    eigenpairs to the bit (each element sees the same operations in the same order)."""
+    from mpdo_amd import _lib
+    lib = _lib.load()
+    rng = np.random.default_rng(n)
+    X = _crand(rng, 2, n, n + 3) * np.logspace(0, -5, n)[None, :, None]
+    H = X @ np.swapaxes(X.conj(), 1, 2)
+    outs = []
+    old = lib.mpdo_debug_bj_threads(0)
+    try:
+        for t in (256, 512, 1024):
+            lib.mpdo_debug_bj_threads(t)
+            w, V, _ = _lib.debug_eigh(torch.from_numpy(H.copy()).to(cuda))
+            outs.append((w.cpu().numpy(), V.cpu().numpy()))
+    finally:
+        lib.mpdo_debug_bj_threads(old)
+    for w, V in outs[:2]:
+        assert np.array_equal(w, outs[2][0]) and np.array_equal(V, outs[2][1])
+    for b in range(2):
+        ref = np.sort(np.linalg.eigvalsh(H[b]))[::-1]
+        assert np.abs(outs[2][0][b] - ref).max() <= 1e-13 * ref[0]
+        assert np.abs(H[b] @ outs[2][1][b] - outs[2][1][b] * outs[2][0][b][None, :]).max() <= 1e-12 * ref[0]
+
+
 @pytest.mark.parametrize('n,rank', [(40, 5), (100, 10), (256, 30), (200, 0)])
 def test_eigh_rank_deficient(cuda, n, rank):
     """Gram matrices of rank << n (zero null space plus rounding noise) must converge."""

@@ -79,7 +79,7 @@ mpdo_status mpdo_apply_2q(const mpdo_site* tmin, const mpdo_site* tmax, const fl
 
 /*
  * Left-to-right QR sweep. Replaces TNNOptimizer.py:87-108 (qr_left2right) with
- * decompositions.py:360-406 (qr): for i = 0..n-2, T_i as [(l s a) x r] = Q R with
+ * decompositions.py:149-195 (qr): for i = 0..n-2, T_i as [(l s a) x r] = Q R with
  * min(rows, cols) columns, T_i <- Q, T_{i+1} <- R T_{i+1}. mpdo_qr_left2right_dims fills the
  * output dims (deterministic); out[i].data must hold them.
  */
@@ -89,7 +89,7 @@ mpdo_status mpdo_qr_left2right(int n, const mpdo_site* in, mpdo_site* out, void*
 /*
  * Truncating SVD sweep with the sqrt(S) split.
  * direction 0 replaces TNNOptimizer.py:111-134 (svd_right2left), direction 1 replaces
- * TNNOptimizer.py:137-161 (svd_left2right); keep rule decompositions.py:328-345 with
+ * TNNOptimizer.py:137-161 (svd_left2right); keep rule decompositions.py:116-133 with
  * relative=True; max_singular_values <= 0 means None, max_truncation_err < 0 means None.
  * left_canonical != 0 asserts that sites 0..n-2 are left-orthonormal (true right after
  * mpdo_qr_left2right, the bondTruncate pairing of TNNOptimizer.py:72-84), which lets each step

@@ -60,9 +60,9 @@ def gate_matrix(name: str, params: Sequence[float] = ()) -> np.ndarray:
         'X': [[0, 1], [1, 0]],                                       # XGates.py:15-45
         'Y': [[0, -1j], [1j, 0]],                                    # YGates.py:15-45
         'Z': [[1, 0], [0, -1]],                                      # ZGates.py:15-45
-        'S': [[1, 0], [0, 1j]],                                      # PhaseGates.py:15-47
-        'SDG': [[1, 0], [0, -1j]],                                   # PhaseGates.py:50-80
-        'T': [[1, 0], [0, (1 + 1j) * _S2]],                          # PhaseGates.py:83-113
+        'S': [[1, 0], [0, 1j]],                                      # PhaseGates.py:16-48
+        'SDG': [[1, 0], [0, -1j]],                                   # PhaseGates.py:49-81
+        'T': [[1, 0], [0, (1 + 1j) * _S2]],                          # PhaseGates.py:82-114
         'MEASUREX': [[_S2, _S2], [_S2, -_S2]],                       # SingleGates.py:240-271
         'MEASUREY': [[_S2, -1j * _S2], [_S2, 1j * _S2]],             # SingleGates.py:274-305
         'MEASUREZ': [[1, 0], [0, 1]],                                # SingleGates.py:308-339
@@ -80,7 +80,7 @@ def gate_matrix(name: str, params: Sequence[float] = ()) -> np.ndarray:
     if n == 'RZ':                                                    # ZGates.py:48-84
         t = p[0]
         return np.array([[np.exp(-0.5j * t), 0], [0, np.exp(0.5j * t)]], dtype=np.complex128)
-    if n in ('P', 'U1'):                                             # PhaseGates.py:116-152, SingleGates.py:83-119
+    if n in ('P', 'U1'):                                             # PhaseGates.py:115-153, SingleGates.py:83-119
         return np.array([[1, 0], [0, np.exp(1j * p[0])]], dtype=np.complex128)
     if n == 'U2':                                                    # SingleGates.py:122-159 (uses phi, lam)
         phi, lam = p
@@ -103,7 +103,7 @@ def gate_matrix(name: str, params: Sequence[float] = ()) -> np.ndarray:
         two = [[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]]
     elif n == 'ISWAP':                                               # DoubleGates.py:85-117
         two = [[1, 0, 0, 0], [0, 0, 1j, 0], [0, 1j, 0, 0], [0, 0, 0, 1]]
-    elif n == 'CP':                                                  # PhaseGates.py:155-191
+    elif n == 'CP':                                                  # PhaseGates.py:154-190
         two = np.diag([1, 1, 1, np.exp(1j * p[0])])
     elif n == 'RXX':                                                 # XGates.py:123-160
         t = p[0]
@@ -244,7 +244,7 @@ def chi_to_kraus(chi: np.ndarray) -> np.ndarray:
 
     Eigen-decompose chi (numpy eig), sort eigenvalues descending (numpy complex sort), count
     |lambda| > 1e-12, and build E_i = sum_j sqrt(lambda_i) U_ji P_j for the first ``count``
-    sorted eigenpairs, zeroing coefficients below 1e-12 (RealNoise.py:360-370)."""
+    sorted eigenpairs, zeroing coefficients below 1e-12 (RealNoise.py:68-104)."""
     N = int(round(math.log(chi.shape[0], 4)))
     basis = [''.join(g) for g in itertools.product('IXYZ', repeat=N)]
     s, u = np.linalg.eig(chi)
@@ -275,7 +275,7 @@ def load_chi(path: str) -> np.ndarray:
 # ----------------------------------------------------------------------------------------------
 def keep_count(s: np.ndarray, max_singular_values: Optional[int], max_truncation_error: Optional[float],
                relative: bool) -> int:
-    """decompositions.py:328-345: number of singular values kept.
+    """decompositions.py:116-133: number of singular values kept.
 
     trunc_errs = sqrt(cumsum(s^2)); with an error bound, keep idx+1 for the first idx (excluding
     the last) where trunc_errs[-1] - trunc_errs[idx] <= err (err * s[0] when relative); result
@@ -290,12 +290,13 @@ def keep_count(s: np.ndarray, max_singular_values: Optional[int], max_truncation
             if te[-1] - te[i] <= bound:
                 k_err = i + 1
                 break
-    # the reference slices s[:k] (decompositions.py:348-351), which caps k at len(s)
+    # the reference slices s[:k] (decompositions.py:135-139), which caps k at len(s)
     return min(msv, k_err, n)
 
 
 def randomized_svd(M: np.ndarray, n_components: int, n_over: int = 7, rng_draw=None):
-    """decompositions.py:234-259 (the branch taken when numel >= 10000 and a cap is given).
+    """decompositions.py:23-48, the branch decompositions.py:111-114 takes when numel >= 10000 and a
+    cap is given.
 
     ``rng_draw(m, k)`` supplies the Gaussian test matrix; the reference draws it from a fresh
     default-seeded CPU ``torch.Generator`` (parity unpinned: the draw itself is torch behaviour)."""
@@ -315,7 +316,7 @@ def randomized_svd(M: np.ndarray, n_components: int, n_over: int = 7, rng_draw=N
 
 def svd(M: np.ndarray, max_singular_values=None, max_truncation_error=None, relative=False,
         randomized: bool = False, rng_draw=None):
-    """decompositions.py:262-357 on an already-flattened matrix: returns (u, s, vh, s_rest).
+    """decompositions.py:51-146 on an already-flattened matrix: returns (u, s, vh, s_rest).
 
     ``randomized`` reproduces the reference's branch for numel >= 10000 with a cap; the oracle's
     default is the exact SVD (the reference's algorithm with exact SVD)."""
@@ -328,7 +329,7 @@ def svd(M: np.ndarray, max_singular_values=None, max_truncation_error=None, rela
 
 
 def qr(M: np.ndarray):
-    """decompositions.py:360-406: reduced QR, no sign fixing (non_negative_diagonal=False)."""
+    """decompositions.py:149-195: reduced QR, no sign fixing (non_negative_diagonal=False)."""
     return np.linalg.qr(M, mode='reduced')
 
 
@@ -784,7 +785,7 @@ def trace_rho_rho(st0: MPDO, st1: Optional[MPDO] = None, reduced: Sequence[int] 
 
 
 def trace_composited_rho2(states: Sequence[MPDO], reduced: Sequence[int] = ()) -> float:
-    """dmOperations.py:91-132: Tr(rho^2) of rho = sum_i rho_i / S (diagonal and cross terms)."""
+    """dmOperations.py:90-132: Tr(rho^2) of rho = sum_i rho_i / S (diagonal and cross terms)."""
     S = len(states)
     tot = sum(trace_rho_rho(states[i], states[j], reduced) for i in range(S) for j in range(S))
     return tot / S ** 2

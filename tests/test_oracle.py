@@ -42,7 +42,7 @@ def test_rotations_are_exponentials(theta):
     assert np.allclose(O.gate_matrix('RY', [theta]), expm(-0.5j * theta * Y))
     assert np.allclose(O.gate_matrix('RZ', [theta]), expm(-0.5j * theta * Z))
     assert np.allclose(O.gate_matrix('RZZ', [theta]).reshape(4, 4), expm(-0.5j * theta * np.kron(Z, Z)))
-    # PhaseGates.py:155-191 / DoubleGates.py:155-190,229-262
+    # PhaseGates.py:154-190 / DoubleGates.py:155-190,229-262
     assert np.allclose(O.gate_matrix('CP', [theta]).reshape(4, 4), expm(1j * theta * np.diag([0, 0, 0, 1])))
     xy = (np.kron(X, X) + np.kron(Y, Y)) / 2
     assert np.allclose(O.gate_matrix('XXPLUSYY', [theta, 0.0]).reshape(4, 4), expm(-0.5j * theta * xy))

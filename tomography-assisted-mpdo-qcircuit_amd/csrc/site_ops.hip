@@ -491,7 +491,7 @@ hipError_t permute(const float2* in, float2* out, int rank, const int* dims, con
 }
 
 // ---------------------------------------------------------------------------------------
-// decompositions.svd keep-count (decompositions.py:328-345) on s = sqrt(max(lambda, 0)),
+// decompositions.svd keep-count (decompositions.py:116-133) on s = sqrt(max(lambda, 0)),
 // lambda the descending Gram eigenvalues; entries past `nev` are zero singular values.
 // Writes k and the discarded weight sum_{j >= k} s_j^2 (the reference computes and drops it).
 // ---------------------------------------------------------------------------------------

@@ -47,7 +47,7 @@ def checkConnectivity(_qubits: List[SiteNode]) -> bool:
 
 
 def qr_left2right(_qubits: List[SiteNode]) -> None:
-    """TNNOptimizer.py:87-108 / decompositions.py:360-406."""
+    """TNNOptimizer.py:87-108 / decompositions.py:149-195."""
     _validate_qubit_list(_qubits, 'qr_left2right')
     if len(_qubits) == 0:
         return

@@ -114,7 +114,7 @@ def trace_composited_rho(*dmNodes: DMNodes) -> torch.Tensor:
 
 
 def trace_composited_rho2(*dmNodes: DMNodes) -> torch.Tensor:
-    """dmOperations.py:91-132: Tr(rho^2) of rho = (rho_0 + ... + rho_{S-1}) / S as
+    """dmOperations.py:90-132: Tr(rho^2) of rho = (rho_0 + ... + rho_{S-1}) / S as
     (sum_i Tr rho_i^2 + 2 sum_{i<j} Tr rho_i rho_j) / S^2. The reference takes the same sum from a
     dense 2^k matrix when at most 12 qubits stay untraced; the value is the same."""
     S = len(dmNodes)

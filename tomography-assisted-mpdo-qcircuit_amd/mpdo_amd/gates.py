@@ -143,9 +143,9 @@ HGate = _fixed('H', [[_S2, _S2], [_S2, -_S2]])                        # SingleGa
 XGate = _fixed('X', [[0, 1], [1, 0]])                                 # XGates.py:15-45
 YGate = _fixed('Y', [[0, -1j], [1j, 0]])                              # YGates.py:15-45
 ZGate = _fixed('Z', [[1, 0], [0, -1]])                                # ZGates.py:15-45
-SGate = _fixed('S', [[1, 0], [0, 1j]])                                # PhaseGates.py:15-47
-SDGGate = _fixed('Sdg', [[1, 0], [0, -1j]])                           # PhaseGates.py:50-80
-TGate = _fixed('T', [[1, 0], [0, (1 + 1j) * _S2]])                    # PhaseGates.py:83-113
+SGate = _fixed('S', [[1, 0], [0, 1j]])                                # PhaseGates.py:16-48
+SDGGate = _fixed('Sdg', [[1, 0], [0, -1j]])                           # PhaseGates.py:49-81
+TGate = _fixed('T', [[1, 0], [0, (1 + 1j) * _S2]])                    # PhaseGates.py:82-114
 CXGate = _fixed('CX', np.eye(4)[[0, 1, 3, 2]], False)                 # XGates.py:88-120
 CNOTGate = _fixed('CNOT', np.eye(4)[[0, 1, 3, 2]], False)             # DoubleGates.py:50-82
 CYGate = _fixed('CY', [[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 0, -1j], [0, 0, 1j, 0]], False)   # YGates.py:88-121
@@ -184,7 +184,7 @@ def _s(t):
 RXGate = _param('RX', lambda t: [[_c(t), -1j * _s(t)], [-1j * _s(t), _c(t)]])            # XGates.py:48-85
 RYGate = _param('RY', lambda t: [[_c(t), -_s(t)], [_s(t), _c(t)]])                        # YGates.py:48-85
 RZGate = _param('RZ', lambda t: [[np.exp(-0.5j * t), 0], [0, np.exp(0.5j * t)]])          # ZGates.py:48-84
-PGate = _param('P', lambda t: [[1, 0], [0, np.exp(1j * t)]])                              # PhaseGates.py:116-152
+PGate = _param('P', lambda t: [[1, 0], [0, np.exp(1j * t)]])                              # PhaseGates.py:115-153
 U1Gate = _param('U1', lambda t: [[1, 0], [0, np.exp(1j * t)]])                            # SingleGates.py:83-119
 U2Gate = _param('U2', lambda ph, la: np.array([[1, -np.exp(1j * la)],
                                                [np.exp(1j * ph), np.exp(1j * (la + ph))]]) * _S2,
@@ -192,7 +192,7 @@ U2Gate = _param('U2', lambda ph, la: np.array([[1, -np.exp(1j * la)],
 U3Gate = _param('U3', lambda t, ph, la: [[_c(t), -np.exp(1j * la) * _s(t)],
                                          [np.exp(1j * ph) * _s(t), np.exp(1j * (la + ph)) * _c(t)]],
                 nparams=3)                                                                # SingleGates.py:162-201
-CPGate = _param('CP', lambda t: np.diag([1, 1, 1, np.exp(1j * t)]), single=False)        # PhaseGates.py:155-191
+CPGate = _param('CP', lambda t: np.diag([1, 1, 1, np.exp(1j * t)]), single=False)        # PhaseGates.py:154-190
 RXXGate = _param('RXX', lambda t: [[_c(t), 0, 0, -1j * _s(t)], [0, _c(t), -1j * _s(t), 0],
                                    [0, -1j * _s(t), _c(t), 0], [-1j * _s(t), 0, 0, _c(t)]],
                  single=False)                                                            # XGates.py:123-160

@@ -82,16 +82,16 @@ def _exp_channel(filename: Optional[str], default: str, dtype) -> torch.Tensor:
     if filename is None:
         filename = os.path.join(_DATA, default)
     st = np.stack(noisyTensor(readExpChi(filename)))            # (K, 2, 2, 2, 2)
-    return torch.from_numpy(np.moveaxis(st, 0, -1).copy()).to(dtype)   # RealNoise.py:437 'ijlmn -> jlmni'
+    return torch.from_numpy(np.moveaxis(st, 0, -1).copy()).to(dtype)   # RealNoise.py:169 'ijlmn -> jlmni'
 
 
 def czExp_channel(filename: Optional[str] = None, dtype=torch.complex128, device: Union[int, str] = 'cpu'):
-    """RealNoise.py:432-438 (the bundled czDefault.mat when no file is given)."""
+    """RealNoise.py:164-170 (the bundled czDefault.mat when no file is given)."""
     return _exp_channel(filename, 'czDefault.mat', dtype).to(device)
 
 
 def cpExp_channel(filename: Optional[str] = None, dtype=torch.complex128, device: Union[int, str] = 'cpu'):
-    """RealNoise.py:441-447. The reference ships no cpDefault.mat; a file must be given."""
+    """RealNoise.py:173-179. The reference ships no cpDefault.mat; a file must be given."""
     if filename is None:
         raise FileNotFoundError('no default CP chi matrix is bundled (the reference has none either)')
     return _exp_channel(filename, 'cpDefault.mat', dtype).to(device)

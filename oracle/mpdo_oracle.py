@@ -299,7 +299,8 @@ def randomized_svd(M: np.ndarray, n_components: int, n_over: int = 7, rng_draw=N
     cap is given.
 
     ``rng_draw(m, k)`` supplies the Gaussian test matrix; the reference draws it from a fresh
-    default-seeded CPU ``torch.Generator`` (parity unpinned: the draw itself is torch behaviour)."""
+    default-seeded CPU ``torch.Generator`` (parity unpinned for the draw itself, torch behaviour;
+    the algorithm is checked by closed forms in tests/test_oracle.py)."""
     m, n = M.shape
     k = n_components + n_over
     Q = rng_draw(m, k) if rng_draw is not None else (

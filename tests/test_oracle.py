@@ -161,6 +161,31 @@ def test_keep_count_hand_cases(s, msv, err, rel, want):
     assert O.keep_count(np.asarray(s), msv, err, rel) == want
 
 
+@pytest.mark.parametrize('m,n', [(120, 80), (60, 90)])   # n_iter 'auto': 3 power steps if m >= n, else 2
+def test_randomized_svd_closed_forms(m, n):
+    """decompositions.py:23-48 restated (the Gaussian draw is the oracle's own, not torch's):
+    (1) for M of rank 6 <= n_components + 7 the sketch spans M's range, so the singular values
+    and the reconstruction u s vh are exact; (2) on a full-rank spectrum 0.7^j the power steps
+    Q <- M (M^H Q) resolve the 10 leading singular values to the accuracy the sketch guarantees
+    ((sigma_18 / sigma_10)^(2q+1) for the range, squared for the values). A transposed Q in
+    B = Q^H M breaks (1); a missing conjugate in M^H leaves errors near 1e-5 in (2)."""
+    rng = np.random.default_rng(3 + m)
+    X = (rng.standard_normal((m, 6)) + 1j * rng.standard_normal((m, 6))) @ \
+        (rng.standard_normal((6, n)) + 1j * rng.standard_normal((6, n)))
+    u, s, vh = O.randomized_svd(X, 4)
+    ref = np.linalg.svd(X, compute_uv=False)
+    assert np.allclose(s[:6], ref[:6], rtol=1e-10)
+    assert np.abs(s[6:]).max() <= 1e-10 * ref[0]
+    assert np.abs((u * s) @ vh - X).max() <= 1e-10 * ref[0]
+    assert np.abs(u.conj().T @ u - np.eye(u.shape[1])).max() <= 1e-10
+    r = min(m, n)
+    U, _ = np.linalg.qr(rng.standard_normal((m, r)) + 1j * rng.standard_normal((m, r)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, r)) + 1j * rng.standard_normal((n, r)))
+    sig = 0.7 ** np.arange(r)
+    u, s, vh = O.randomized_svd((U * sig) @ V.conj().T, 10)
+    assert np.abs(s[:10] - sig[:10]).max() <= 1e-8
+
+
 @pytest.mark.parametrize('m,n,rank', [(30, 50, 7), (64, 20, 20), (12, 12, 3)])
 def test_svd_product_matches_explicit_svd(m, n, rank):
     rng = np.random.default_rng(m * n)

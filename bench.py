@@ -4,10 +4,10 @@
 Workload (the 32-qubit precision configuration): depth-30 brickwork of RY(theta) RZ(phi) on every
 qubit then CNOT on (0,1),(2,3),... / (1,2),(3,4),... alternately, single- and two-qubit
 depolarizing noise p = 0.002, chi = 256, kappa = 8, 32 copies whose angles differ by
-U[-0.01, 0.01] (mpdo_amd.workloads.config4). One step = one circuit-layer of all 32 copies:
-the composed single-qubit channel of every site, CNOT + two-qubit depolarizing on every pair of
-the layer's parity and the truncation (QR sweep, chi sweep, kappa cut), as one call of
-mpdo_layer_batched. The chains start from synthetic complex64 site tensors (seeded, decaying bond
+U[-0.01, 0.01] (mpdo_amd.workloads.brickwork32_chi256_family). One step = one circuit-layer of
+all 32 copies: the composed single-qubit channel of every site, CNOT + two-qubit depolarizing on
+every pair of the layer's parity and the truncation (QR sweep, chi sweep, kappa cut), as one call
+of mpdo_layer_batched. The chains start from synthetic complex64 site tensors (seeded, decaying bond
 and inner weights) at the shapes a long run of this circuit settles into; successive layers
 alternate the parity and feed each other. The warmup layers bring the spectra to the circuit's
 own steady state as well: the gate-time split keeps ~458 of its 1024 directions on the synthetic
@@ -89,7 +89,7 @@ def _layers(copies, count, dev):
     import mpdo_amd as M
     from mpdo_amd import batched
     from mpdo_amd import workloads as W
-    ws = W.config4(copies=copies, n=N_QUBITS)
+    ws = W.brickwork32_chi256_family(copies=copies, n=N_QUBITS)
     cs = []
     for w in ws:
         c = M.TensorCircuit(device=str(dev), **w.circuit_kwargs())

@@ -1,0 +1,44 @@
+"""The benchmark's contract pieces that need no GPU: it refuses more than one GPU before touching
+one, the roofline's HBM traffic is the figure of the committed counter profile, and the workload
+family it times is the documented circuit."""
+import os
+import re
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_refuses_several_gpus():
+    p = subprocess.run([sys.executable, os.path.join(ROOT, 'bench.py'), '--gpus', '2'], capture_output=True,
+                       text=True, timeout=300)
+    assert p.returncode == 2
+    assert 'measures one GPU' in p.stderr
+    assert p.stdout == ''   # no JSON line
+
+
+def test_traffic_is_the_profile_figure():
+    import bench
+    with open(bench.TRAFFIC_FILE) as f:
+        text = f.read()
+    want = float(re.search(r'traffic_bytes_per_launch = ([0-9.e+]+)', text).group(1))
+    assert bench._traffic_per_launch() == want
+    # FETCH_SIZE and WRITE_SIZE rows of the table add up to it
+    rows = re.findall(r'\| (FETCH_SIZE|WRITE_SIZE) \| [0-9.]+ \| [0-9.]+ \| ([0-9.e+]+) \|', text)
+    assert len(rows) == 2
+    assert abs(sum(float(v) for _, v in rows) - want) <= 1e-4 * want
+
+
+def test_timed_workload_family():
+    """32-qubit-style family at a small size: copies share qubits and gate names, copy 0 keeps the
+    base angles, the others differ by at most 0.01 per angle; chi 256, kappa 8, p = 0.002."""
+    from mpdo_amd import workloads as W
+    ws = W.brickwork32_chi256_family(copies=3, n=6, depth=4)
+    assert [w.name for w in ws] == ['brickwork32_chi256_0', 'brickwork32_chi256_1', 'brickwork32_chi256_2']
+    for w in ws:
+        assert (w.n, w.chi, w.kappa, w.depth, w.noise_type) == (6, 256, 8, 4, 'idealNoise')
+        assert w.chip == dict(dpc_error_rate=0.002) and w.p1 == 0.002
+        assert [(o[0], o[1]) for o in w.ops] == [(o[0], o[1]) for o in ws[0].ops]
+    for w in ws[1:]:
+        d = [abs(a - b) for o, o0 in zip(w.ops, ws[0].ops) for a, b in zip(o[2], o0[2])]
+        assert d and max(d) <= 0.01 + 1e-12 and max(d) > 0

@@ -68,17 +68,17 @@ def brickwork8_chi32(seed: int = 101) -> Workload:
                     chip=dict(dpc_error_rate=0.01), single_qubit_noise='depolarizing', p1=1e-3, depth=4)
 
 
-def config1(seed: int = 202, n: int = 20, depth: int = 20) -> Workload:
+def brickwork20_damping_chi64(seed: int = 202, n: int = 20, depth: int = 20) -> Workload:
     """20 qubits, depth 20, CNOT (even) / CZ (odd) brickwork, amplitude damping 0.005 plus
     dephasing 0.005 after every gate (1q: the reference composite; 2q: noisy identities on
     both qubits), chi=64, kappa=4."""
     rng = np.random.Generator(np.random.PCG64(seed))
     ops = brickwork(n, depth, rng, two=('CNOT', 'CZ'), noisy_identity_after_2q=True)
-    return Workload('config1', n, ops, 'unified', False, 64, 4,
+    return Workload('brickwork20_damping_chi64', n, ops, 'unified', False, 64, 4,
                     chip=dict(decay_param=0.005, dephasing_param=0.005), depth=depth)
 
 
-def config2(seed: int = 303) -> Workload:
+def hea16_chi64(seed: int = 303) -> Workload:
     """16-qubit hardware-efficient ansatz: 8 layers RY, RZ (angles U[-pi, pi)) + CNOT chain,
     depolarizing p=0.005 per gate, chi=64, kappa=4."""
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -92,11 +92,11 @@ def config2(seed: int = 303) -> Workload:
         for a in range(n - 1):
             ops.append(('CNOT', [a, a + 1], []))
         ops.append(('TRUNCATE', [], []))
-    return Workload('config2', n, ops, 'idealNoise', False, 64, 4, chip=dict(dpc_error_rate=0.005),
+    return Workload('hea16_chi64', n, ops, 'idealNoise', False, 64, 4, chip=dict(dpc_error_rate=0.005),
                     single_qubit_noise='depolarizing', p1=0.005, depth=depth)
 
 
-def config3(seed: int = 404, count: int = 256) -> List[Workload]:
+def random_brickwork_batch(seed: int = 404, count: int = 256) -> List[Workload]:
     """Independent random brickwork circuits: n ~ U{8..24}, depth ~ U{4..16},
     chi ~ {16,32,64,128}, kappa ~ {2,4}, depolarizing or amplitude damping (Bernoulli 1/2)
     with p ~ U[1e-4, 1e-2]."""
@@ -111,16 +111,16 @@ def config3(seed: int = 404, count: int = 256) -> List[Workload]:
         p = float(rng.uniform(1e-4, 1e-2))
         ops = brickwork(n, depth, rng, noisy_identity_after_2q=not dep)
         if dep:
-            w = Workload(f'config3_{c}', n, ops, 'idealNoise', False, chi, kappa, chip=dict(dpc_error_rate=p),
+            w = Workload(f'random_brickwork_{c}', n, ops, 'idealNoise', False, chi, kappa, chip=dict(dpc_error_rate=p),
                          single_qubit_noise='depolarizing', p1=p, depth=depth)
         else:
-            w = Workload(f'config3_{c}', n, ops, 'unified', False, chi, kappa,
+            w = Workload(f'random_brickwork_{c}', n, ops, 'unified', False, chi, kappa,
                          chip=dict(decay_param=p, dephasing_param=0.0), depth=depth)
         out.append(w)
     return out
 
 
-def config4(seed: int = 505, copies: int = 32, n: int = 32, depth: int = 30) -> List[Workload]:
+def brickwork32_chi256_family(seed: int = 505, copies: int = 32, n: int = 32, depth: int = 30) -> List[Workload]:
     """32 qubits, depth 30 RY/RZ + CNOT brickwork, dep1/dep2 p=0.002, chi=256, kappa=8; copies
     with the same base angles plus delta ~ U[-0.01, 0.01]."""
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -134,7 +134,7 @@ def config4(seed: int = 505, copies: int = 32, n: int = 32, depth: int = 30) -> 
             dr = np.random.Generator(np.random.PCG64(seed * 1000 + c))
             delta = dr.uniform(-0.01, 0.01, size=(depth, n, 2))
         ops = brickwork(n, depth, r, delta=delta)
-        out.append(Workload(f'config4_{c}', n, ops, 'idealNoise', False, 256, 8, chip=dict(dpc_error_rate=0.002),
+        out.append(Workload(f'brickwork32_chi256_{c}', n, ops, 'idealNoise', False, 256, 8, chip=dict(dpc_error_rate=0.002),
                             single_qubit_noise='depolarizing', p1=0.002, depth=depth))
     return out
 

@@ -11,21 +11,28 @@ of mpdo_layer_batched. The chains start from synthetic complex64 site tensors (s
 and inner weights) at the shapes a long run of this circuit settles into; successive layers
 alternate the parity and feed each other. The warmup layers bring the spectra to the circuit's
 own steady state as well: the gate-time split keeps ~458 of its 1024 directions on the synthetic
-start, then 780, 950, and from the sixth layer on 1017-1024 (the split no longer truncates and
-every later eigenproblem has order ~1024, twice the cost of the first layer), so the default
-times two layers after six warmup layers. Inputs are resident in HBM when the timed region
-starts.
+start, then 780, 950, and from the sixth layer on 1017-1024. Inputs are resident in HBM when the
+timed region starts; the timed steps run without the library's launch-timing hooks.
 
-Prints one JSON line: circuit-layers/s (32 x steps / seconds), the roofline of the dominant
-kernel class (algorithmic flops per launch / launch duration, HIP events on the launch stream),
-and the CPU oracle timed on a bounded sample of the same workload.
+After the timed steps, on the input of the first timed layer (kept aside):
+  * one profiled pass of the same layer (HIP events on the launch stream around every launch):
+    the roofline of the dominant kernel and of the block-Jacobi kernels, with the work the latter
+    executed counted from their skip flags;
+  * the plain per-circuit path (the per-gate entry points, one gate at a time, copy 0) in fp32
+    accumulation: its circuit-layers/s, and the batched/plain ratio;
+  * accuracy: copy 0 of the batched layer against the per-gate path in fp64 accumulation on the
+    same input (all 496 <Z_i Z_j>, 32 <Z_i>, Tr rho~, chi/kappa discarded weights), and one
+    chi=256, kappa=8 layer at 10 qubits against its complex128 oracle fixture.
+Then the CPU oracle on a bounded sample of the same workload.
+
+Prints one JSON line.
 """
 from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
+import platform
 import sys
 import time
 
@@ -41,7 +48,12 @@ N_QUBITS, CHI, KAPPA, P_NOISE, FIRST_LAYER = 32, 256, 8, 0.002, 10
 # HBM bytes per GEMM launch of the timed step, from the two rocprofv3 --pmc passes (FETCH_SIZE,
 # WRITE_SIZE) of this benchmark, scaled by the copy calibration; the file records the commands
 TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'gemm_kernel_traffic_fused_layer_n32_chi256_kappa8_b32.md')
+LAYER10_FIXTURE = os.path.join(ROOT, 'tests', 'golden', 'layer10_chi256_kappa8_oracle.npz')
 CALIB_BYTES = 1 << 30
+
+
+def _log(msg):
+    print(f'# {msg}', file=sys.stderr, flush=True)
 
 
 def _traffic_per_launch():
@@ -56,40 +68,13 @@ def _traffic_per_launch():
     return float(m.group(1)) if m else None
 
 
-def _steady_chains(dims, copies, dev, seed=2024):
-    """(copies, l, 2, a, r) complex64 per site: random tensors with geometric weights 0.97^l,
-    0.97^r on the bonds and 0.5^a on the inner index, each site scaled so that the running
-    transfer-matrix trace of its chain stays 1 (Tr rho = 1, as for a state evolved from |0..0>
-    by trace-preserving channels: the gate-time split's absolute error bound then acts at the
-    scale it has in a real run)."""
-    import torch
-    g = torch.Generator(device=dev)
-    g.manual_seed(seed)
-    out = []
-    env = torch.ones(copies, 1, 1, dtype=torch.complex128, device=dev)
-    for (l, a, r) in dims:
-        T = torch.complex(torch.randn(copies, l, 2, a, r, generator=g, device=dev),
-                          torch.randn(copies, l, 2, a, r, generator=g, device=dev))
-        wl = 0.97 ** torch.arange(l, device=dev, dtype=torch.float32)
-        wr = 0.97 ** torch.arange(r, device=dev, dtype=torch.float32)
-        wa = 0.5 ** torch.arange(a, device=dev, dtype=torch.float32)
-        T = T * (wl[:, None, None, None] * wa[None, None, :, None] * wr[None, None, None, :])
-        T64 = T.to(torch.complex128)
-        # env'[r, r'] = sum env[l, l'] T[l, s, a, r] conj(T[l', s, a, r'])
-        nxt = torch.einsum('bxy,bxsar,bysaz->brz', env, T64, T64.conj())
-        f = torch.einsum('brr->b', nxt).real
-        T = (T64 / torch.sqrt(f).reshape(copies, 1, 1, 1, 1)).to(torch.complex64)
-        env = nxt / f.reshape(copies, 1, 1)
-        out.append(T.contiguous())
-    return out
-
-
-def _layers(copies, count, dev):
-    """Fused-layer operands of layers FIRST_LAYER .. FIRST_LAYER + count - 1 of the workload."""
+def _layers(copies, count, dev, n=N_QUBITS, first=FIRST_LAYER):
+    """Fused-layer operands of layers first .. first + count - 1 of the workload (the n-qubit
+    member of its circuit family)."""
     import mpdo_amd as M
     from mpdo_amd import batched
     from mpdo_amd import workloads as W
-    ws = W.brickwork32_chi256_family(copies=copies, n=N_QUBITS)
+    ws = W.brickwork32_chi256_family(copies=copies, n=n)
     cs = []
     for w in ws:
         c = M.TensorCircuit(device=str(dev), **w.circuit_kwargs())
@@ -98,43 +83,92 @@ def _layers(copies, count, dev):
     segs = batched._segments(cs[0])
     out = []
     for k in range(count):
-        start, stop = segs[(FIRST_LAYER + k) % len(segs)]
+        start, stop = segs[(first + k) % len(segs)]
         layer, why = batched.fused_layer_of(cs, start, stop)
         if layer is None:
-            raise RuntimeError(f'layer {FIRST_LAYER + k} does not fuse: {why}')
+            raise RuntimeError(f'layer {first + k} does not fuse: {why}')
         out.append(layer)
     return out
 
 
-def cpu_baseline(Ti, Tj, i, threads: int = 16):
-    """The oracle (numpy complex128) on one pair of the benchmark's first circuit-layer (sites i,
-    i+1 of copy 0, the same tensors the GPU step starts from): the composed single-qubit channels
-    of both sites and the gate-time split of the pair (Circuit.py:74-178); a layer has 16 such
-    pairs plus the QR, chi and kappa steps, so 16 x this time is a lower bound on the oracle's
-    time per circuit-layer."""
+def _zz_codes(n):
+    """<Z_i> (n strings), <Z_i Z_j> for every i < j, then the identity (Tr rho~)."""
+    codes = []
+    for i in range(n):
+        c = [-1] * n
+        c[i] = 2
+        codes.append(c)
+    for i in range(n):
+        for j in range(i + 1, n):
+            c = [-1] * n
+            c[i] = c[j] = 2
+            codes.append(c)
+    codes.append([-1] * n)
+    return codes
+
+
+def _cpu_model():
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or 'unknown'
+
+
+def cpu_baseline(Ti, Tj, k1i, k1j, g2, threads):
+    """The oracle (numpy complex128) on one pair (sites i, i+1 of copy 0, the tensors the first
+    timed GPU layer starts from) through every step the layer applies to it: both sites' single-qubit
+    channels (RY, RZ + depolarizing, composed into one Choi-reduced Kraus set as the GPU path does),
+    the CNOT + two-qubit depolarizing gate-time split (Circuit.py:74-136), the QR step through the
+    pair (TNNOptimizer.py:87-108), the chi step at the pair bond (TNNOptimizer.py:111-134,
+    decompositions.py:51-146) and the kappa cuts of both sites (TNNOptimizer.py:164-197). A layer
+    has 16 such pairs plus 15 chi steps and 15 QR steps between pairs, so 1 / (16 x the pair's
+    time) bounds the oracle's rate from above."""
     import numpy as np
     from threadpoolctl import threadpool_limits
 
     from oracle import mpdo_oracle as O
 
     Ti, Tj = Ti.astype(np.complex128), Tj.astype(np.complex128)
-    noise = O.NoiseSpec(chip='best', dpc_error_rate=P_NOISE, single='depolarizing', p1=P_NOISE)
-    E1 = noise.single_kraus(O.gate_matrix('RY', [1.1]))
-    E2 = noise.single_kraus(O.gate_matrix('RZ', [0.3]))
-    G2 = noise.double_kraus(O.gate_matrix('CNOT'))
+    t = {}
     with threadpool_limits(threads):
         t0 = time.perf_counter()
-        a = O.apply_1q(O.apply_1q(Ti, E1), E2)
-        b = O.apply_1q(O.apply_1q(Tj, E1), E2)
-        O.apply_2q(a, b, G2)
-        dt = time.perf_counter() - t0
+        a = O.apply_1q(Ti, k1i)
+        b = O.apply_1q(Tj, k1j)
+        t['1q'] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        a, b, _ = O.apply_2q(a, b, g2)
+        t['split'] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        l, _, a1, m = a.shape
+        q, R = O.qr(a.reshape(l * 2 * a1, m))
+        a = q.reshape(l, 2, a1, q.shape[1])
+        b = np.tensordot(R, b, axes=([1], [0]))
+        t['qr'] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        l, _, a1, m = a.shape
+        _, _, a2, r = b.shape
+        us, svh, s, _ = O._sweep_split(a.reshape(l * 2 * a1, m), b.reshape(m, 2 * a2 * r), CHI, None, {})
+        k = s.shape[0]
+        a, b = us.reshape(l, 2, a1, k), svh.reshape(k, 2, a2, r)
+        t['chi'] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        st = O.MPDO([a, b], [True, True], [True])
+        O.svd_kappa_left2right(st, KAPPA)
+        t['kappa'] = time.perf_counter() - t0
+    total = sum(t.values())
     pairs = len(range(0, N_QUBITS - 1, 2))
-    return {'value': 1.0 / (pairs * dt), 'unit': 'circuit-layers/s', 'cores': threads, 'kind': 'port',
-            'sample': (f'oracle (numpy complex128, {threads} BLAS threads) on one middle pair of the first timed '
-                       f'circuit-layer (n=32, chi=256, kappa=8, copy 0, sites {i},{i + 1} of shapes '
-                       f'{tuple(Ti.shape)}, {tuple(Tj.shape)}): both sites\' RY, RZ + depolarizing channels and '
-                       f'the CNOT + depolarizing gate-time split took {dt:.1f} s; value = 1 / ({pairs} x that), an '
-                       f'upper bound on the oracle\'s rate since the QR, chi and kappa steps are not timed')}
+    shapes = f'{tuple(Ti.shape)}, {tuple(Tj.shape)}'
+    return {'value': 1.0 / (pairs * total), 'unit': 'circuit-layers/s', 'cores': threads, 'kind': 'port',
+            'cpu': _cpu_model(), 'dtype': 'complex128 (numpy, LAPACK)',
+            'sample': (f'oracle on one middle pair of the first timed circuit-layer (n=32, chi=256, kappa=8, copy 0, '
+                       f'input shapes {shapes}): '
+                       + ', '.join(f'{k} {v:.1f} s' for k, v in t.items())
+                       + f', total {total:.1f} s; value = 1 / ({pairs} x total), an upper bound on the oracle\'s '
+                       'rate (the 15 chi and QR steps between pairs are not timed)')}
 
 
 def main():
@@ -144,7 +178,11 @@ def main():
     ap.add_argument('--warmup', type=int, default=6)
     ap.add_argument('--copies', type=int, default=32)
     ap.add_argument('--no-cpu-baseline', action='store_true')
-    ap.add_argument('--cpu-threads', type=int, default=16)
+    ap.add_argument('--cpu-threads', type=int, default=None,
+                    help='threads of the CPU oracle sample (default: the cores this process may run on)')
+    ap.add_argument('--fused-only', action='store_true',
+                    help='time the fused steps only (no profiled pass, plain path or accuracy check): the '
+                         'command of the rocprofv3 counter passes')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='before the warmup, run torch.neg over 1 GiB of float32 three times (1 GiB read and '
                          '1 GiB written per launch): the reference launch that scales the HBM counters of a '
@@ -156,9 +194,11 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    import numpy as np
     import torch
 
     from mpdo_amd import _lib, batched
+    from mpdo_amd import workloads as W
 
     if not torch.cuda.is_available():
         raise SystemExit('bench.py needs the MI355X (cuda:0)')
@@ -170,20 +210,17 @@ def main():
             torch.neg(src, out=dst)
         torch.cuda.synchronize()
         del src, dst
-        print(f'# copy calibration: 3 launches of torch.neg, {CALIB_BYTES} bytes read and {CALIB_BYTES} '
-              'bytes written each', file=sys.stderr, flush=True)
+        _log(f'copy calibration: 3 launches of torch.neg, {CALIB_BYTES} bytes read and {CALIB_BYTES} bytes '
+             'written each')
     lib = _lib.load()
     prev_mode = lib.mpdo_set_math_mode(0)
     B = args.copies
     layers = _layers(B, args.warmup + args.steps, dev)
     dims = batched.steady_dims(N_QUBITS, CHI, KAPPA, layers[0].kraus1.shape[2], layers[0].gates2.shape[-1],
                                layers[0].parity)
-    sites = _steady_chains(dims, B, dev)
+    sites = W.steady_chains_device(dims, B, dev)
     torch.cuda.synchronize()
-    # the CPU sample's pair: sites i, i+1 of copy 0 at the input of the first timed layer
-    pair_i = N_QUBITS // 2 - 2 + layers[args.warmup].parity
-    print(f'# {B} copies, {N_QUBITS} qubits, chi={CHI}, kappa={KAPPA}; warmup {args.warmup}, steps {args.steps}',
-          file=sys.stderr, flush=True)
+    _log(f'{B} copies, {N_QUBITS} qubits, chi={CHI}, kappa={KAPPA}; warmup {args.warmup}, steps {args.steps}')
 
     def step(k, sites):
         L = layers[k]
@@ -194,56 +231,132 @@ def main():
         t = time.perf_counter()
         sites = step(k, sites)
         torch.cuda.synchronize()
-        print(f'# warmup step {k}: {time.perf_counter() - t:.2f} s', file=sys.stderr, flush=True)
+        _log(f'warmup step {k}: {time.perf_counter() - t:.2f} s')
 
-    cpu_pair = None
-    if not args.no_cpu_baseline:
-        cpu_pair = (sites[pair_i][0].cpu().numpy(), sites[pair_i + 1][0].cpu().numpy())
-    _lib.prof_enable(True)
+    saved = None if args.fused_only else [t.clone() for t in sites]   # input of the first timed layer
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
         sites = step(args.warmup + k, sites)
-        print(f'# timed step {k} issued', file=sys.stderr, flush=True)
+        _log(f'timed step {k} issued')
     torch.cuda.synchronize()
     T = time.perf_counter() - t0
-    prof = _lib.prof_read()
-    _lib.prof_enable(False)
-    print(f'# gate-time split kept counts of the last layer: {_lib.layer_kept()}', file=sys.stderr, flush=True)
-    eigs = {}
-    for (n_e, b_e, sw_e) in _lib.prof_eig_log():
-        c = eigs.setdefault(n_e, [0, 0])
-        c[0] += 1
-        c[1] += sw_e
-    print('# blocked eigensolver calls (order: calls, mean sweeps): ' +
-          ', '.join(f'{k}: {v[0]}, {v[1] / v[0]:.1f}' for k, v in sorted(eigs.items())), file=sys.stderr, flush=True)
-    shapes = sorted(_lib.prof_shapes('gemm'), key=lambda r: -r[2])
-    print('# GEMM shapes by time (M, N, K, batch, prec: launches, ms, TFLOP/s): ' +
-          '; '.join(f'{sh}: {c}, {t:.0f}, {f / (t * 1e-3) / 1e12:.1f}' for sh, c, t, f in shapes[:14]),
-          file=sys.stderr, flush=True)
-
-    # the dominant kernel: mpdo::gemm_kernel, the kernel with the largest share of the steady-state
-    # step (30% of kernel time over its operand/tile instantiations; the block-Jacobi rounds are
-    # three kernels, bj_update 24%, bj_solve 20.5%, bj_rotate_v_direct 18%, whose per-round tile
-    # counts include the identity rotations they skip, so their flop totals are upper bounds and
-    # are not used for a roofline)
-    name = 'gemm'
-    n_l, ms, fl, by = prof[name]
-    prec = 'fp64'
-    achieved = fl / (ms * 1e-3) / 1e12
-    kernels = {'gemm': 'mpdo::gemm_kernel (complex GEMM, fp64 MFMA accumulate)'}
-    traffic = _traffic_per_launch() if name == 'gemm' else None
-    roofline = {'bound': 'mfma', 'achieved': round(achieved, 3), 'peak': PEAK_TFLOPS[prec], 'unit': 'TFLOP/s',
-                'frac': round(achieved / PEAK_TFLOPS[prec], 4), 'traffic': traffic,
-                'kernel': kernels.get(name, name), 'launches': n_l, 'avg_launch_ms': round(ms / max(n_l, 1), 4),
-                'flops_per_launch': fl / max(n_l, 1), 'operand_bytes_per_launch': by / max(n_l, 1),
-                'share_of_kernel_time': round(ms / max(sum(v[1] for v in prof.values()), 1e-9), 3),
-                'classes_ms': {k: round(v[1], 1) for k, v in prof.items()}}
     value = B * args.steps / T
-    cpu = None
-    if not args.no_cpu_baseline:
-        print('# cpu baseline (oracle sample)', file=sys.stderr, flush=True)
-        cpu = cpu_baseline(cpu_pair[0], cpu_pair[1], pair_i, args.cpu_threads)
+    _log(f'timed: {T:.2f} s, {value:.3f} circuit-layers/s')
+    del sites
+
+    roofline, others, plain, accuracy, cpu = None, None, None, None, None
+    if not args.fused_only:
+        L0 = layers[args.warmup]
+        # ---- profiled pass: the first timed layer again, on its kept input
+        _lib.prof_enable(True)
+        outs, disc = _lib.layer_batched(saved, L0.kraus1, L0.parity, L0.gates2, CHI, KAPPA)
+        torch.cuda.synchronize()
+        prof = _lib.prof_read()
+        execd = _lib.prof_read_executed()
+        eig_log = _lib.prof_eig_log()
+        shapes = sorted(_lib.prof_shapes('gemm'), key=lambda r: -r[2])
+        _lib.prof_enable(False)
+        _log(f'profiled pass: gate-time split kept counts {_lib.layer_kept()}')
+        eigs = {}
+        for (n_e, b_e, sw_e) in eig_log:
+            c = eigs.setdefault(n_e, [0, 0])
+            c[0] += 1
+            c[1] += sw_e
+        _log('blocked eigensolver calls (order: calls, mean sweeps): ' +
+             ', '.join(f'{k}: {v[0]}, {v[1] / v[0]:.1f}' for k, v in sorted(eigs.items())))
+        _log('GEMM shapes by time (M, N, K, batch, prec: launches, ms, TFLOP/s): ' +
+             '; '.join(f'{sh}: {c}, {t:.0f}, {f / (t * 1e-3) / 1e12:.1f}' for sh, c, t, f in shapes[:14]))
+        total_ms = max(sum(v[1] for v in prof.values()), 1e-9)
+        n_l, ms, fl, by = prof['gemm']
+        achieved = fl / (ms * 1e-3) / 1e12
+        roofline = {'bound': 'mfma', 'achieved': round(achieved, 3), 'peak': PEAK_TFLOPS['fp64'], 'unit': 'TFLOP/s',
+                    'frac': round(achieved / PEAK_TFLOPS['fp64'], 4), 'traffic': _traffic_per_launch(),
+                    'kernel': 'mpdo::gemm_kernel (complex GEMM, fp64 MFMA accumulate)', 'launches': n_l,
+                    'avg_launch_ms': round(ms / max(n_l, 1), 4), 'flops_per_launch': fl / max(n_l, 1),
+                    'operand_bytes_per_launch': by / max(n_l, 1),
+                    'share_of_kernel_time': round(ms / total_ms, 3),
+                    'classes_ms': {k: round(v[1], 1) for k, v in prof.items()}}
+        names = {'eig_update': 'bj_update (two-sided 64x64 block-pair rotation of A, fp64 MFMA)',
+                 'eig_rotate': 'bj_rotate_v_direct (eigenvector tile update V W, fp64 MFMA)',
+                 'eig_solve': 'bj_solve (64x64 subproblem Jacobi sweep in LDS, fp64 vector ALU)'}
+        others = []
+        for cls, kname in names.items():
+            n_c, ms_c, fl_c, by_c = prof[cls]
+            ex_fl, ex_by = execd[cls]
+            if n_c == 0 or ms_c <= 0:
+                continue
+            a_c = ex_fl / (ms_c * 1e-3) / 1e12
+            others.append({'kernel': kname, 'bound': 'mfma' if cls != 'eig_solve' else 'fp64 vector ALU',
+                           'achieved': round(a_c, 3), 'peak': PEAK_TFLOPS['fp64'], 'unit': 'TFLOP/s',
+                           'frac': round(a_c / PEAK_TFLOPS['fp64'], 4), 'launches': n_c,
+                           'avg_launch_ms': round(ms_c / n_c, 4), 'executed_flops_per_launch': ex_fl / n_c,
+                           'executed_bytes_per_launch': ex_by / n_c,
+                           'hbm_gbs_of_executed_bytes': round(ex_by / (ms_c * 1e-3) / 1e9, 1),
+                           'share_of_kernel_time': round(ms_c / total_ms, 3)})
+        # ---- accuracy of the batched layer: copy 0 against the per-gate path (fp64 accumulation)
+        codes = _zz_codes(N_QUBITS)
+        v_f = np.array(_lib.pauli_expect([t[0].contiguous() for t in outs], codes))
+        chain0 = [t[0] for t in saved]
+        del outs
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        ref, gate, d_chi, d_kap = batched.per_gate_layer(chain0, L0, 0, CHI, KAPPA)
+        torch.cuda.synchronize()
+        t_ref = time.perf_counter() - t
+        v_g = np.array(_lib.pauli_expect(ref, codes))
+        tr = abs(v_g[-1])
+        got_chi = np.array([disc[0, i] for i in reversed(range(N_QUBITS - 1))])
+        chi_rel = float(np.max(np.abs(got_chi - np.array(d_chi)) / np.maximum(np.abs(d_chi), 1e-30)))
+        k_f, k_g = float(disc[0, N_QUBITS:2 * N_QUBITS].sum()), float(sum(d_kap.values()))
+        del ref
+        # ---- the plain per-circuit FP32 GPU path: same input, copy 0, fp32 accumulation
+        lib.mpdo_set_math_mode(1)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        pl, _, _, _ = batched.per_gate_layer(chain0, L0, 0, CHI, KAPPA)
+        torch.cuda.synchronize()
+        t_plain = time.perf_counter() - t
+        lib.mpdo_set_math_mode(0)
+        v_p = np.array(_lib.pauli_expect(pl, codes))
+        del pl
+        plain = {'value': round(1.0 / t_plain, 6), 'unit': 'circuit-layers/s', 'seconds_per_circuit_layer': round(t_plain, 3),
+                 'path': 'per-gate entry points one gate at a time (mpdo_apply_1q, mpdo_apply_2q, bondTruncate, '
+                         'svdKappa_left2right), one chain, fp32 MFMA accumulation (mpdo_set_math_mode(1)), on copy 0 '
+                         'of the first timed layer\'s input',
+                 'batched_vs_plain': round(value * t_plain, 2),
+                 'fp64_per_gate_seconds': round(t_ref, 3)}
+        accuracy = {'reference': 'per-gate path, fp64 accumulation, copy 0, same input and layer',
+                    'zz_z_trace_max_abs_err_over_trace': float(np.abs(v_f - v_g).max() / tr),
+                    'chi_weights_max_rel_err': chi_rel,
+                    'kappa_total_rel_err': abs(k_f - k_g) / max(k_g, 1e-30),
+                    'plain_fp32_zz_max_abs_err_over_trace': float(np.abs(v_p - v_g).max() / tr)}
+        # ---- one chi=256, kappa=8 layer at 10 qubits against its complex128 oracle fixture
+        if os.path.exists(LAYER10_FIXTURE):
+            g = np.load(LAYER10_FIXTURE)
+            dims10 = [tuple(int(x) for x in d) for d in batched.steady_dims(10, CHI, KAPPA, 4, 16, 1)]
+            ch = [torch.from_numpy(a).to(dev).unsqueeze(0).contiguous() for a in W.steady_state(dims10, seed=7)]
+            L10 = _layers(1, 1, dev, n=10, first=1)[0]    # the fixture's layer: pairs (1,2),(3,4),...
+            o10, _ = _lib.layer_batched(ch, L10.kraus1, L10.parity, L10.gates2, CHI, KAPPA)
+            c10 = _zz_codes(10)
+            v10 = np.array(_lib.pauli_expect([t[0].contiguous() for t in o10], c10))
+            ref10 = list(g['z']) + [g['zz'][i, j] for i in range(10) for j in range(i + 1, 10)] + [float(g['trace'])]
+            accuracy['oracle_complex128_layer10_zz_max_abs_err_over_trace'] = float(
+                np.abs(v10 - np.array(ref10)).max() / float(g['trace']))
+        _log(f'accuracy: {accuracy}')
+        _log(f'plain: {plain}')
+        cpu_in = None
+        if not args.no_cpu_baseline:
+            i = N_QUBITS // 2 - 2 + L0.parity
+            k1 = L0.kraus1[0].cpu().numpy()
+            cpu_in = (saved[i][0].cpu().numpy(), saved[i + 1][0].cpu().numpy(),
+                      np.transpose(k1[i], (1, 2, 0)), np.transpose(k1[i + 1], (1, 2, 0)),
+                      L0.gates2[(i - L0.parity) // 2].cpu().numpy())
+        del saved
+        if cpu_in is not None:
+            threads = args.cpu_threads or len(os.sched_getaffinity(0))
+            _log(f'cpu baseline (oracle sample, {threads} threads)')
+            cpu = cpu_baseline(*cpu_in, threads)
     lib.mpdo_set_math_mode(prev_mode)
     line = {'metric': METRIC, 'value': round(value, 6), 'unit': 'circuit-layers/s', 'n_gpus': 1,
             'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': round(T / args.steps * 1e3, 3),
@@ -254,7 +367,9 @@ def main():
                                     'every copy from the steady state (fused batched layer)'),
                        'n_qubits': N_QUBITS, 'chi': CHI, 'kappa': KAPPA, 'copies': B, 'global_batch': B,
                        'parallelism': 'none (one GPU)'},
-            'roofline': roofline, 'cpu_baseline': cpu}
+            'roofline': roofline, 'roofline_block_jacobi': others, 'plain': plain,
+            'batched_vs_plain': plain['batched_vs_plain'] if plain else None, 'accuracy': accuracy,
+            'cpu_baseline': cpu}
     print(json.dumps(line), flush=True)
 
 

@@ -82,20 +82,25 @@ int mpdo_debug_layer_kept(int* out, int cap);
  * recorded eigenvalues (per pair its 2 p1 values, descending) into out and returns their count. */
 int mpdo_debug_layer_gate_eigs(int record, double* out, int cap);
 
-/* Launch timing by kernel class (0 complex GEMM, 1 block-Jacobi subproblem solve, 2 block-Jacobi
- * rotation update, 3 single-workgroup Jacobi, 4 permutation): HIP events recorded on each
+/* Launch timing by kernel class (0 complex GEMM, 1 block-Jacobi subproblem solve (bj_solve),
+ * 2 block-Jacobi eigenvector update (bj_rotate_v_direct), 3 single-workgroup Jacobi and Cholesky
+ * panels, 4 permutation, 5 block-Jacobi two-sided matrix update (bj_update)): HIP events recorded on each
  * launch's own stream. enable(1) resets the totals and starts recording, enable(0) stops;
  * read() waits for the recorded launches and returns the totals of one class: launches, summed
  * duration in ms, algorithmic flops (the arithmetic the launches are asked to do) and operand
  * bytes. */
 mpdo_status mpdo_debug_prof_enable(int on);
 mpdo_status mpdo_debug_prof_read(int cls, long long* launches, double* ms, double* flops, double* bytes);
-/* The blocked eigensolver calls (order > 64) made while timing was on: (order, batch, sweeps)
- * triples, up to cap ints written; returns the number of calls. */
 /* Per-shape totals of one timing class while timing is on: shapes [5 n] (for GEMMs: M, N, K,
  * batch, precision mode), stats [3 n] (launches, ms, algorithmic flops); returns the number of
  * distinct shapes (fills at most cap). */
 int mpdo_debug_prof_shapes(int cls, int* shapes, double* stats, int cap);
+/* Work the launches of class cls executed while timing was on, as counted from the device's
+ * skip flags (the block-Jacobi kernels skip identity rotations; mpdo_debug_prof_read gives the
+ * work they were asked to do). Zero for classes without such flags. */
+mpdo_status mpdo_debug_prof_read_executed(int cls, double* flops, double* bytes);
+/* The blocked eigensolver calls (order > 64) made while timing was on: (order, batch, sweeps)
+ * triples, up to cap ints written; returns the number of calls. */
 int mpdo_debug_prof_eig_log(int* out, int cap);
 
 #ifdef __cplusplus

@@ -401,6 +401,27 @@ def apply_1q(T: np.ndarray, E: np.ndarray) -> np.ndarray:
     return np.einsum('ijk,ljar->liakr', E, T).reshape(l, 2, a * K, r)
 
 
+def compose_compressed(kraus_sets: Sequence[np.ndarray], rel_tol: float = 1e-14) -> np.ndarray:
+    """One Kraus tensor (2, 2, K') for the channels ``kraus_sets`` applied in order (each
+    (2,2[,K])), with K' the Choi rank of the composition: the stacked vec(E_k) rows (K x 4) are
+    replaced by the rows of S V^H of their SVD. The new set is related to the sequential one
+    (Circuit.py:174-178 flattens each new Kraus axis into I_i) by an isometry on the inner
+    index, which no gauge-invariant quantity, kept count or discarded weight of the path sees.
+    Test infrastructure for fixtures whose uncompressed inner index would not fit in memory;
+    pinned by the channel action on random density matrices."""
+    ops = [np.eye(2, dtype=np.complex128)]
+    for E in kraus_sets:
+        E = np.asarray(E, dtype=np.complex128)
+        if E.ndim == 2:
+            E = E[..., None]
+        ops = [E[:, :, k] @ F for F in ops for k in range(E.shape[2])]
+    rows = np.stack([F.reshape(4) for F in ops])                 # K x 4
+    _, s, vh = np.linalg.svd(rows, full_matrices=False)
+    keep = s > rel_tol * max(s[0], 1e-300)
+    new = (s[keep][:, None] * vh[keep]).reshape(-1, 2, 2)
+    return np.transpose(new, (1, 2, 0)).copy()
+
+
 def _split_sqrt(theta: np.ndarray, **kw):
     """tn.split_node: U*sqrt(S) and sqrt(S)*V^H (tensornetwork 0.4.6 convention)."""
     u, s, vh, s_rest = svd(theta, **kw)
@@ -754,6 +775,29 @@ def pauli_expect(st: MPDO, observables, oqs) -> float:
             P = PAULIS[1 + obs[oqs.index(i)]]
         env = np.einsum('ln,lsar,ts,ntam->rm', env, T, P, T.conj(), optimize=True)
     return float(np.real(env[0, 0]))
+
+
+def bitstring_probs(st: MPDO, bits: np.ndarray) -> np.ndarray:
+    """Unnormalised P(b) = Re Tr(|b><b| rho~) for each row b of ``bits`` (shape (count, n), 0/1),
+    the projector contraction of Circuit.py:297-332 (_conditional_prb with every qubit sampled in
+    the computational basis). All bitstrings advance together: env[c, l, l'] picks the physical
+    slice bits[c, i] of each site. Pinned against the diagonal of the dense density matrix."""
+    bits = np.asarray(bits, dtype=np.int64)
+    cnt = bits.shape[0]
+    env = np.ones((cnt, 1, 1), dtype=np.complex128)
+    for i, T in enumerate(st.sites):
+        l, _, a, r = T.shape
+        nxt = np.empty((cnt, r, r), dtype=np.complex128)
+        for s in (0, 1):
+            sel = np.nonzero(bits[:, i] == s)[0]
+            if sel.size == 0:
+                continue
+            Ts = T[:, s].reshape(l, a * r)                       # [l, (a r)]
+            X = np.einsum('cln,lk->cnk', env[sel], Ts)           # [c, l', (a r)]
+            X = X.reshape(sel.size, l, a, r)
+            nxt[sel] = np.einsum('cnar,nam->crm', X, T[:, s].conj())
+        env = nxt
+    return env[:, 0, 0].real.copy()
 
 
 def trace_rho(st: MPDO) -> float:

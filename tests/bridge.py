@@ -31,3 +31,60 @@ def run_oracle(w, chi='default', kappa='default', err=None):
              chi=w.chi if chi == 'default' else chi, kappa=w.kappa if kappa == 'default' else kappa,
              max_truncation_err=err)
     return st
+
+
+def run_oracle_fused(w, st=None, layers=None):
+    """oracle.evolve's rules (Circuit.py:74-224, 469-491) for a workload of gates and TRUNCATE
+    markers, with each qubit's consecutive single-qubit channels composed into one
+    Choi-rank-reduced Kraus set (oracle.compose_compressed) applied before the next two-qubit
+    gate on that qubit or the next truncation: the sequential sets would multiply the inner
+    dimension by up to 36 per layer, and by thousands before the chain is first connected
+    (no truncation until then). The composed set is related to the sequential ones by an
+    isometry on the inner index, which no kept count, discarded weight or gauge-invariant
+    output sees. ``st``: start state (default |0...0>); ``layers``: only the first that many
+    circuit-layers. The gate-time split's dropped weight is recorded as ('gate', min, weight)."""
+    from mpdo_amd import workloads as W
+    noise = oracle_noise(w)
+    nt = w.noise_type.lower()
+    unified, idealn = (not w.ideal and nt == 'unified'), (not w.ideal and nt == 'idealnoise')
+    st = O.product_state(w.n) if st is None else st
+    ops = w.ops if layers is None else [op for k in range(layers) for op in W.layer_ops(w.ops, k)]
+    pending = {q: [] for q in range(w.n)}
+
+    def flush(qs):
+        for q in qs:
+            if pending[q]:
+                st.sites[q] = O.apply_1q(st.sites[q], O.compose_compressed([E for E, _ in pending[q]]))
+                if any(g for _, g in pending[q]):
+                    st.has_I[q] = True
+                pending[q] = []
+
+    for name, q, p in ops:
+        if name == 'TRUNCATE':
+            flush(range(w.n))
+            if O.check_connectivity(st):
+                O.bond_truncate(st, w.chi)
+                if not w.ideal:
+                    O.svd_kappa_left2right(st, w.kappa)
+            continue
+        G = O.gate_matrix(name, p)
+        if G.ndim == 2:
+            gnoise = idealn or unified
+            pending[q[0]].append((noise.single_kraus(G) if gnoise else G, gnoise))
+            continue
+        a, b = sorted(q)
+        flush([a, b])
+        if idealn:
+            G = noise.double_kraus(G)
+        if q[0] > q[1]:
+            G = np.swapaxes(np.swapaxes(G, 0, 1), 2, 3)
+        x, y, s_rest = O.apply_2q(st.sites[a], st.sites[b], G, True)
+        st.sites[a], st.sites[b] = x, y
+        st.bond[a] = True
+        if G.ndim == 5:
+            st.has_I[b] = True
+        st.discarded.append(('gate', a, float(np.sum(np.abs(s_rest) ** 2))))
+    flush(range(w.n))
+    if not w.ideal and ops and ops[-1][0] != 'TRUNCATE':
+        O.svd_kappa_left2right(st, w.kappa)
+    return st

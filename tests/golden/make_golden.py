@@ -1,8 +1,29 @@
-"""Writes the oracle's outputs for the 8-qubit brickwork workload (4 layers, chi=32,
-kappa=4) to brickwork8_chi32_oracle.npz: the full 256x256 density matrix, the trace and the discarded
-weight of every cut. Run from the repository root: python tests/golden/make_golden.py"""
+"""Writes the oracle's outputs for the golden fixtures of the GPU parity tests. Run from the
+repository root (in the build container; the GPU tests only read the .npz files):
+
+    python tests/golden/make_golden.py [brickwork8 | brickwork20 | layer10]
+
+brickwork8  -> brickwork8_chi32_oracle.npz: 8 qubits, 4 CNOT brickwork layers, dep1 p=1e-3 /
+               dep2 p=1e-2, chi=32, kappa=4: the full 256x256 density matrix, the trace and the
+               discarded weight of every cut.
+brickwork20 -> brickwork20_damping_chi64_oracle.npz: 20 qubits, depth 20, CNOT (even) / CZ (odd)
+               layers, amplitude damping 0.005 + dephasing 0.005 after every gate, chi=64, kappa=4
+               (seed 202): <Z_i>, <Z_i Z_i+1>, Tr rho~, the probabilities of 1024 bitstrings drawn
+               U{0,1}^20 from PCG64(202) after the angles, and every discarded weight.
+family8     -> family8_chi64_kappa8_oracle.npz: the 8-qubit, depth-6 member (copy 0) of the 32-qubit
+               benchmark's circuit family (RY, RZ + depolarizing p=0.002, CNOT + two-qubit
+               depolarizing p=0.002) at chi=64, kappa=8: the 256x256 density matrix, the trace and
+               every discarded weight.
+layer10     -> layer10_chi256_kappa8_oracle.npz: one circuit-layer (RY, RZ + depolarizing p=0.002
+               on every qubit, CNOT + two-qubit depolarizing p=0.002 on pairs (1,2),(3,4),...,
+               truncate with chi=256, kappa=8) of the 10-qubit member of the 32-qubit benchmark's
+               circuit family, applied to the numpy-seeded steady-state chain
+               workloads.steady_state(batched.steady_dims(10, 256, 8, 4, 16, 1), seed=7): every
+               <Z_i Z_j>, <Z_i>, Tr rho~, the output shapes and every discarded weight.
+"""
 import os
 import sys
+import time
 
 import numpy as np
 
@@ -15,15 +36,94 @@ import bridge  # noqa: E402
 from oracle import mpdo_oracle as O  # noqa: E402
 from oracle import dense as D  # noqa: E402
 
-if __name__ == '__main__':
+# the 10-qubit steady-state layer fixture: chain shapes and seeds (shared with the GPU test)
+LAYER10_DIMS = [(1, 8, 64), (64, 8, 256), (256, 8, 256), (256, 8, 256), (256, 8, 256), (256, 8, 256),
+                (256, 8, 256), (256, 8, 256), (256, 8, 256), (256, 8, 1)]
+LAYER10_SEED = 7
+LAYER10_LAYER = 1          # layer index of the circuit whose ops are applied (pairs (1,2),(3,4),...)
+
+
+def _disc(st):
+    return dict(disc_kind=np.array([d[0] for d in st.discarded]),
+                disc_index=np.array([d[1] for d in st.discarded]),
+                disc_weight=np.array([d[2] for d in st.discarded]))
+
+
+def brickwork8():
     w = W.brickwork8_chi32()
     st = bridge.run_oracle(w)
     rho = O.density_matrix(st)
     dense = D.dense_evolve(bridge.oracle_ops(w), w.n, w.noise_type, w.ideal, bridge.oracle_noise(w))
-    kinds = np.array([d[0] for d in st.discarded])
     np.savez_compressed(os.path.join(HERE, "brickwork8_chi32_oracle.npz"), rho=rho,
-                        trace=np.trace(rho).real, disc_kind=kinds,
-                        disc_index=np.array([d[1] for d in st.discarded]),
-                        disc_weight=np.array([d[2] for d in st.discarded]),
-                        dims=np.array([t.shape for t in st.sites]))
+                        trace=np.trace(rho).real, dims=np.array([t.shape for t in st.sites]), **_disc(st))
     print('trace', np.trace(rho).real, 'max |rho - dense|', np.abs(rho - dense).max())
+
+
+def brickwork20_bits(n=20, count=1024, seed=202):
+    """The 1024 bitstrings of the 20-qubit configuration: drawn after the circuit's angles from
+    the same PCG64(seed) stream (qubit 0 first in each row)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    rng.uniform(0.0, 2 * np.pi, size=(20, n, 2))       # the angles of brickwork20_damping_chi64
+    return rng.integers(0, 2, size=(count, n))
+
+
+def brickwork20():
+    w = W.brickwork20_damping_chi64()
+    t = time.time()
+    st = bridge.run_oracle_fused(w)
+    print(f'evolve: {time.time() - t:.1f} s', flush=True)
+    n = w.n
+    z = np.array([O.pauli_expect(st, [2], [i]) for i in range(n)])
+    zz = np.array([O.pauli_expect(st, [2, 2], [i, i + 1]) for i in range(n - 1)])
+    bits = brickwork20_bits(n)
+    probs = O.bitstring_probs(st, bits)
+    np.savez_compressed(os.path.join(HERE, 'brickwork20_damping_chi64_oracle.npz'), z=z, zz=zz, bits=bits,
+                        probs=probs, trace=O.trace_rho(st), dims=np.array([t.shape for t in st.sites]), **_disc(st))
+    print('trace', O.trace_rho(st), 'sum probs', probs.sum(), f'total {time.time() - t:.1f} s')
+
+
+def family8_workload():
+    w = W.brickwork32_chi256_family(copies=1, n=8, depth=6)[0]
+    w.chi = 64
+    return w
+
+
+def family8():
+    w = family8_workload()
+    t = time.time()
+    st = bridge.run_oracle_fused(w)
+    rho = O.density_matrix(st)
+    np.savez_compressed(os.path.join(HERE, 'family8_chi64_kappa8_oracle.npz'), rho=rho, trace=np.trace(rho).real,
+                        dims=np.array([t.shape for t in st.sites]), **_disc(st))
+    print('trace', np.trace(rho).real, f'total {time.time() - t:.1f} s')
+
+
+def layer10_ops():
+    """The ops of layer LAYER10_LAYER of the 10-qubit member of the benchmark family (copy 0)."""
+    w = W.brickwork32_chi256_family(copies=1, n=10)[0]
+    return w, W.layer_ops(w.ops, LAYER10_LAYER)
+
+
+def layer10():
+    w, ops = layer10_ops()
+    sites = [t.astype(np.complex128) for t in W.steady_state(LAYER10_DIMS, seed=LAYER10_SEED)]
+    n = len(sites)
+    st = O.MPDO(sites, [True] * n, [True] * (n - 1))
+    t = time.time()
+    bridge.run_oracle_fused(W.Workload(w.name, n, ops, w.noise_type, w.ideal, w.chi, w.kappa, chip=w.chip,
+                                       single_qubit_noise=w.single_qubit_noise, p1=w.p1), st=st)
+    print(f'layer: {time.time() - t:.1f} s', flush=True)
+    z = np.array([O.pauli_expect(st, [2], [i]) for i in range(n)])
+    zz = np.zeros((n, n))
+    for i in range(n):
+        for j in range(i + 1, n):
+            zz[i, j] = zz[j, i] = O.pauli_expect(st, [2, 2], [i, j])
+    np.savez_compressed(os.path.join(HERE, 'layer10_chi256_kappa8_oracle.npz'), z=z, zz=zz, trace=O.trace_rho(st),
+                        dims=np.array([t.shape for t in st.sites]), **_disc(st))
+    print('trace', O.trace_rho(st), 'dims', [t.shape for t in st.sites], f'total {time.time() - t:.1f} s')
+
+
+if __name__ == '__main__':
+    which = sys.argv[1:] or ['brickwork8']
+    for name in which:
+        {'brickwork8': brickwork8, 'brickwork20': brickwork20, 'family8': family8, 'layer10': layer10}[name]()

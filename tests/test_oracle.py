@@ -320,3 +320,134 @@ def test_golden_fixture_is_reproduced():
     g = np.load(os.path.join(HERE, 'golden', 'brickwork8_chi32_oracle.npz'))
     st = bridge.run_oracle(W.brickwork8_chi32())
     assert np.allclose(O.density_matrix(st), g['rho'], atol=1e-12)
+
+
+def test_bitstring_probs_are_the_dense_diagonal():
+    """oracle.bitstring_probs (projector contraction of every qubit) against the diagonal of the
+    dense density matrix of a noisy 4-qubit evolution, all 16 bitstrings in a shuffled order."""
+    n = 4
+    ops = _random_ops(n, 2, 41)
+    ns = O.NoiseSpec(chip='best', dpc_error_rate=0.05, single='depolarizing', p1=0.03)
+    st = O.evolve(ops, O.product_state(n), 'idealNoise', False, ns)
+    rho = D.dense_evolve(ops, n, 'idealNoise', False, ns)
+    idx = np.random.default_rng(5).permutation(2 ** n)
+    bits = (idx[:, None] >> np.arange(n - 1, -1, -1)[None, :]) & 1      # qubit 0 = most significant
+    got = O.bitstring_probs(st, bits)
+    assert np.allclose(got, np.real(np.diag(rho))[idx], atol=1e-13)
+    assert np.isclose(got.sum(), np.real(np.trace(rho)))
+
+
+def test_expect_with_unsorted_qubits_matches_dense():
+    """oracle.expect with oqs listed out of order (O's first factor acts on the first listed
+    qubit): Tr(O rho) from the dense density matrix with O embedded on those qubits."""
+    n = 4
+    ops = _random_ops(n, 2, 43)
+    ns = O.NoiseSpec(chip='best', dpc_error_rate=0.05, single='depolarizing', p1=0.03)
+    st = O.evolve(ops, O.product_state(n), 'idealNoise', False, ns)
+    rho = D.dense_evolve(ops, n, 'idealNoise', False, ns)
+    rng = np.random.default_rng(6)
+    A = rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2))
+    B = rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2))
+    C = rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2))
+    for oq in ([2, 0], [3, 1, 0], [1, 3]):
+        mats = [A, B, C][:len(oq)]
+        Ofull = mats[0]
+        for m in mats[1:]:
+            Ofull = np.kron(Ofull, m)
+        # dense embedding: factor j acts on qubit oq[j]; identity elsewhere
+        op = np.ones((1, 1), dtype=np.complex128)
+        for q in range(n):
+            op = np.kron(op, mats[oq.index(q)] if q in oq else np.eye(2))
+        want = float(np.real(np.trace(op @ rho)))
+        got = O.expect(st, [Ofull], [oq])[0]
+        assert np.isclose(got, want, atol=1e-12), oq
+
+
+def _mirror(st):
+    """The same state with the chain read right to left (site i -> n-1-i, l <-> r)."""
+    sites = [np.transpose(T, (3, 1, 2, 0)).copy() for T in reversed(st.sites)]
+    return O.MPDO(sites, list(reversed(st.has_I)), list(reversed(st.bond)))
+
+
+@pytest.mark.parametrize('seed', [51, 52])
+def test_svd_left2right_pins(seed):
+    """svd_left2right (TNNOptimizer.py:137-161): (1) on a pure state whose sites 1..n-1 are
+    right-orthonormal, the first cut (bond 0|1) discards the Schmidt tail of psi[s0, rest];
+    (2) the whole sweep equals the right-to-left sweep of the mirrored chain, mirrored back
+    (same density matrix, same discarded weights in the mirrored order): a transposed split or
+    a sweep in the wrong direction fails one of the two."""
+    n = 4
+    ops = _random_ops(n, 3, seed)
+    st = O.evolve([o for o in ops if o.name != 'TRUNCATE'], O.product_state(n), 'no', True)
+    # right-orthonormal gauge for sites 1..n-1: mirrored QR sweep
+    m = _mirror(st)
+    O.qr_left2right(m)
+    st = _mirror(m)
+    rho = O.density_matrix(st)
+    w, V = np.linalg.eigh(rho)
+    psi = V[:, -1] * math.sqrt(max(w[-1], 0))
+    S = np.linalg.svd(psi.reshape(2, 2 ** (n - 1)), compute_uv=False)
+    st1 = st.copy()
+    O.svd_left2right(st1, 1)
+    d = [x for x in st1.discarded if x[0] == 'chi_l2r']
+    assert np.isclose(d[0][2], S[1] ** 2, atol=1e-12)
+    for chi in (1, 2):
+        a = st.copy()
+        O.svd_left2right(a, chi)
+        b = _mirror(st)
+        O.svd_right2left(b, chi)
+        db = [x[2] for x in b.discarded if x[0] == 'chi']
+        b = _mirror(b)
+        assert np.allclose(O.density_matrix(a), O.density_matrix(b), atol=1e-12)
+        da = [x[2] for x in a.discarded if x[0] == 'chi_l2r']
+        assert len(da) == n - 1
+        assert np.allclose(da, db, atol=1e-12)
+
+
+def test_compose_compressed_is_the_same_channel():
+    """oracle.compose_compressed: the composed, Choi-rank-reduced Kraus set acts on random
+    density matrices as the channels applied one after the other, with at most 4 operators;
+    a reversed composition order fails (the channels below do not commute)."""
+    rng = np.random.default_rng(9)
+    ns = O.NoiseSpec(chip='best', single='depolarizing', p1=0.02)
+    ud = O.NoiseSpec(chip='best', decay_param=0.03, dephasing_param=0.02)
+    sets = [ns.single_kraus(O.gate_matrix('RY', [0.7])), ns.single_kraus(O.gate_matrix('RZ', [1.9])),
+            ud.single_kraus(O.gate_matrix('RX', [0.4]))]
+    E = O.compose_compressed(sets)
+    assert E.shape[:2] == (2, 2) and E.shape[2] <= 4
+    for _ in range(3):
+        A = rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2))
+        rho = A @ A.conj().T
+        want = rho
+        for S in sets:
+            want = sum(S[:, :, k] @ want @ S[:, :, k].conj().T for k in range(S.shape[2]))
+        got = sum(E[:, :, k] @ rho @ E[:, :, k].conj().T for k in range(E.shape[2]))
+        assert np.allclose(got, want, atol=1e-13)
+        bad = O.compose_compressed(sets[::-1])
+        wrong = sum(bad[:, :, k] @ rho @ bad[:, :, k].conj().T for k in range(bad.shape[2]))
+        assert not np.allclose(wrong, want, atol=1e-6)
+
+
+@pytest.mark.parametrize('noise', ['idealNoise', 'unified'])
+def test_fused_channel_oracle_matches_sequential(noise):
+    """bridge.run_oracle_fused (single-qubit channel runs composed and Choi-reduced) against
+    oracle.evolve applying every Kraus set in sequence, with chi and kappa both truncating:
+    the same density matrix, the same discarded weights at every cut."""
+    import bridge
+    from mpdo_amd import workloads as W
+    rng = np.random.Generator(np.random.PCG64(61))
+    n = 5
+    ops = W.brickwork(n, 4, rng, two=('CNOT', 'CZ'), noisy_identity_after_2q=(noise == 'unified'))
+    if noise == 'unified':
+        w = W.Workload('t', n, ops, 'unified', False, 3, 2, chip=dict(decay_param=0.05, dephasing_param=0.03), depth=4)
+    else:
+        w = W.Workload('t', n, ops, 'idealNoise', False, 3, 2, chip=dict(dpc_error_rate=0.04),
+                       single_qubit_noise='depolarizing', p1=0.02, depth=4)
+    a = bridge.run_oracle(w)
+    b = bridge.run_oracle_fused(w)
+    assert np.allclose(O.density_matrix(a), O.density_matrix(b), atol=1e-12)
+    da = [d for d in a.discarded if d[0] in ('chi', 'kappa')]
+    db = [d for d in b.discarded if d[0] in ('chi', 'kappa')]
+    assert [d[:2] for d in da] == [d[:2] for d in db]
+    assert np.allclose([d[2] for d in da], [d[2] for d in db], atol=1e-12)
+    assert any(d[2] > 1e-6 for d in da)

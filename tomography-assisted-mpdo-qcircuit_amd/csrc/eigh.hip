@@ -664,18 +664,44 @@ hipError_t eigh(int n, int batch, void* A, long long lda, long long bsA, double*
     const double tile_flops = 8.0 * NS * NS * NS, tile_bytes = 3.0 * NS * NS * 16.0;
     const int npairs_sym = (P / 2) * (P / 2 + 1) / 2;
     const double tiles_sym = (double)npairs_sym * batch, tiles_r = (double)(s.N / NS) * (P / 2) * batch;
+    const int ntile = s.N / NS;
     for (int r = 0; r < P - 1; ++r) {
       {
         prof::Scope ps(prof::EIG_SOLVE, st, solve_flops, solve_bytes);
         hipLaunchKernelGGL(bj_solve, dim3(P / 2, batch), dim3(g_bj_threads), 0, st, s, r);
       }
-      {   // A <- J^H A J block pair by block pair (Hermitian: lower triangle + mirror), and
-          // V <- V J on the columns of the round's block pairs
-        prof::Scope ps(prof::EIG_ROTATE, st, (tiles_sym * 2 + tiles_r) * tile_flops,
-                       (tiles_sym + tiles_r) * tile_bytes);
-        const int ntile = s.N / NS;
+      {   // A <- J^H A J block pair by block pair (Hermitian: lower triangle + mirror)
+        prof::Scope ps(prof::EIG_UPDATE, st, tiles_sym * 2 * tile_flops, tiles_sym * tile_bytes);
         hipLaunchKernelGGL(bj_update, dim3(npairs_sym, batch), dim3(256), 0, st, s, r);
+      }
+      {   // V <- V J on the columns of the round's block pairs
+        prof::Scope ps(prof::EIG_ROTATE, st, tiles_r * tile_flops, tiles_r * tile_bytes);
         hipLaunchKernelGGL(bj_rotate_v_direct, dim3(ntile * (P / 2), batch), dim3(256), 0, st, s, r, ntile);
+      }
+      if (prof::enabled()) {
+        // the work the three launches executed: only subproblems with a rotation run their sweep,
+        // and only the tiles of rotated block pairs are multiplied (one readback per round, timing
+        // runs only)
+        std::vector<int> id((size_t)(P / 2) * batch);
+        if ((err = hipMemcpyAsync(id.data(), s.ident, sizeof(int) * id.size(), hipMemcpyDeviceToHost, st)) !=
+            hipSuccess)
+          return err;
+        if ((err = hipStreamSynchronize(st)) != hipSuccess) return err;
+        double nsolve = 0, nsym = 0, nsym_tiles = 0, nv = 0;
+        for (int b = 0; b < batch; ++b) {
+          const int* idb = id.data() + (size_t)b * (P / 2);
+          for (int i = 0; i < P / 2; ++i) nsolve += idb[i] == 0;
+          for (int pr = 0; pr < P / 2; ++pr)
+            for (int pc = 0; pc <= pr; ++pc) {
+              const int k = (idb[pr] == 0) + (idb[pc] == 0);
+              if (k) { nsym += 1; nsym_tiles += k; }
+            }
+        }
+        nv = nsolve * ntile;
+        const double per_solve = solve_flops / ((double)(P / 2) * batch);
+        prof::add_executed(prof::EIG_SOLVE, nsolve * per_solve, nsolve * 2.0 * NS * NS * 16.0);
+        prof::add_executed(prof::EIG_UPDATE, nsym_tiles * tile_flops, (2.0 * nsym + nsym_tiles) * NS * NS * 16.0);
+        prof::add_executed(prof::EIG_ROTATE, nv * tile_flops, nv * tile_bytes);
       }
     }
     if ((err = hipGetLastError()) != hipSuccess) return err;

@@ -7,11 +7,15 @@
 namespace mpdo {
 namespace prof {
 
-enum Cls { GEMM = 0, EIG_SOLVE = 1, EIG_ROTATE = 2, EIG_SMALL = 3, PERMUTE = 4, NCLS = 5 };
+// EIG_SOLVE: bj_solve; EIG_ROTATE: bj_rotate_v_direct; EIG_UPDATE: bj_update
+enum Cls { GEMM = 0, EIG_SOLVE = 1, EIG_ROTATE = 2, EIG_SMALL = 3, PERMUTE = 4, EIG_UPDATE = 5, NCLS = 6 };
 
 bool enabled();
 // one record per eigensolver call while timing is on: order, batch, sweeps
 void eig_log(int n, int batch, int sweeps);
+// work a class's launches actually executed (counted on the host from device flags read back
+// while timing is on), next to the work each launch was asked to do (the Scope's figures)
+void add_executed(int cls, double flops, double bytes);
 
 // RAII: records a start event on construction and a stop event on destruction; `shape` (five
 // ints, e.g. M, N, K, batch, precision of a GEMM) keys a per-shape breakdown of the class

@@ -33,6 +33,7 @@ bool g_on = false;
 std::vector<Rec> g_recs;
 std::vector<hipEvent_t> g_free;
 Acc g_acc[NCLS];
+double g_exec_flops[NCLS], g_exec_bytes[NCLS];
 std::vector<int> g_eig;   // (n, batch, sweeps) triples
 std::map<std::pair<int, std::array<int, 5>>, ShapeAcc> g_shapes;
 
@@ -84,6 +85,13 @@ void eig_log(int n, int batch, int sweeps) {
   g_eig.push_back(sweeps);
 }
 
+void add_executed(int cls, double flops, double bytes) {
+  if (!g_on || cls < 0 || cls >= NCLS) return;
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_exec_flops[cls] += flops;
+  g_exec_bytes[cls] += bytes;
+}
+
 Scope::Scope(int cls, hipStream_t s, double flops, double bytes, const int* shape)
     : cls_(cls), s_(s), flops_(flops), bytes_(bytes) {
   if (!g_on) return;
@@ -117,6 +125,7 @@ mpdo_status mpdo_debug_prof_enable(int on) {
   drain_locked();
   if (on) {
     for (auto& a : g_acc) a = Acc{};
+    for (int c = 0; c < NCLS; ++c) g_exec_flops[c] = g_exec_bytes[c] = 0;
     g_eig.clear();
     g_shapes.clear();
   }
@@ -134,6 +143,15 @@ mpdo_status mpdo_debug_prof_read(int cls, long long* launches, double* ms, doubl
   if (ms) *ms = a.ms;
   if (flops) *flops = a.flops;
   if (bytes) *bytes = a.bytes;
+  return MPDO_OK;
+}
+
+mpdo_status mpdo_debug_prof_read_executed(int cls, double* flops, double* bytes) {
+  using namespace mpdo::prof;
+  if (cls < 0 || cls >= NCLS) return MPDO_EINVAL;
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (flops) *flops = g_exec_flops[cls];
+  if (bytes) *bytes = g_exec_bytes[cls];
   return MPDO_OK;
 }
 

@@ -80,6 +80,7 @@ def load() -> ctypes.CDLL:
         'mpdo_debug_prof_eig_log': (I, [P(I), I]),
         'mpdo_debug_prof_shapes': (I, [I, P(I), P(D), I]),
         'mpdo_debug_prof_read': (I, [I, P(ctypes.c_longlong), P(D), P(D), P(D)]),
+        'mpdo_debug_prof_read_executed': (I, [I, P(D), P(D)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -489,7 +490,7 @@ def layer_dry_run(batch: int, dims: Sequence[Tuple[int, int, int]], K1: int, par
     return int(ws.value)
 
 
-PROF_CLASSES = ('gemm', 'eig_solve', 'eig_rotate', 'eig_small', 'permute')
+PROF_CLASSES = ('gemm', 'eig_solve', 'eig_rotate', 'eig_small', 'permute', 'eig_update')
 
 
 def prof_enable(on: bool) -> None:
@@ -506,6 +507,18 @@ def prof_read() -> dict:
         check(lib.mpdo_debug_prof_read(i, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(fl), ctypes.byref(by)),
               'mpdo_debug_prof_read')
         out[name] = (int(n.value), float(ms.value), float(fl.value), float(by.value))
+    return out
+
+
+def prof_read_executed() -> dict:
+    """{class: (flops, bytes)} the launches executed since prof_enable(True), counted from the
+    device's skip flags (block-Jacobi classes; zero for the others)."""
+    lib = load()
+    out = {}
+    for i, name in enumerate(PROF_CLASSES):
+        fl, by = ctypes.c_double(0), ctypes.c_double(0)
+        check(lib.mpdo_debug_prof_read_executed(i, ctypes.byref(fl), ctypes.byref(by)), 'mpdo_debug_prof_read_executed')
+        out[name] = (float(fl.value), float(by.value))
     return out
 
 

@@ -277,3 +277,31 @@ def _fusable_state(cs, states, layer: FusedLayer) -> Tuple[bool, str]:
     if out is None:
         return False, why
     return True, ''
+
+
+def per_gate_layer(chain: Sequence[torch.Tensor], layer: FusedLayer, b: int, chi: Optional[int],
+                   kappa: Optional[int]):
+    """The plain per-circuit path of one fused layer for chain ``b``: the same operands run one
+    gate at a time through the per-gate entry points in the reference's order (Circuit.py:469-489):
+    mpdo_apply_1q on every site, mpdo_apply_2q on every pair (Circuit.py:74-136, absolute error
+    GLOBAL_MINIMUM), then bondTruncate (TNNOptimizer.py:72-84) and svdKappa_left2right
+    (TNNOptimizer.py:164-197). ``chain``: the chain's (l, 2, a, r) site tensors (not modified).
+    Returns (sites, gate-time split weights by pair min site, chi-step weights in sweep order,
+    kappa weights by site)."""
+    from . import TNNOptimizer as _opt
+    from .circuit import GLOBAL_MINIMUM
+    n = len(chain)
+    nodes = [SiteNode(t.contiguous().clone(), i, has_I=True, bond_left=i > 0, bond_right=i < n - 1)
+             for i, t in enumerate(chain)]
+    ts = _lib.apply_1q([nd.tensor for nd in nodes], [layer.kraus1[b, q].permute(1, 2, 0) for q in range(n)])
+    for nd, t in zip(nodes, ts):
+        nd.tensor = t
+    gate = {}
+    for k, p in enumerate(range(layer.parity, n - 1, 2)):
+        g = layer.gates2[k] if layer.noisy2 else layer.gates2[k][..., 0]
+        x, y, d = _lib.apply_2q(nodes[p].tensor, nodes[p + 1].tensor, g, GLOBAL_MINIMUM)
+        nodes[p].tensor, nodes[p + 1].tensor = x, y
+        gate[p] = d
+    d_chi = _opt.bondTruncate(nodes, chi, return_discarded=True)
+    d_kap = _opt.svdKappa_left2right(nodes, kappa, return_discarded=True) if kappa is not None else []
+    return [nd.tensor.contiguous() for nd in nodes], gate, list(d_chi), dict(d_kap)

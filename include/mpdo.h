@@ -42,10 +42,13 @@ const char* mpdo_last_error(void);
 /* Frees the internal workspace of the current device. */
 void mpdo_release_workspace(void);
 /*
- * Accumulation precision of the complex64 tensor contractions (no counterpart in the
- * reference, which computes in the dtype it is given): 0 = fp64 matrix-core accumulation
- * (default; complex64 storage), 1 = fp32 matrix-core accumulation (twice the peak rate).
- * Gram matrices and eigensolves are always fp64. Returns the previous mode.
+ * Accumulation precision of the complex64 tensor contractions whose results are complex64 site
+ * data (no counterpart in the reference, which computes in the dtype it is given): 0 = fp64
+ * matrix-core accumulation (default; complex64 storage), 1 = fp32 matrix-core accumulation of
+ * four real products per complex product (twice the fp64 peak rate), 2 = fp32 accumulation of
+ * three real products (3M: Re = P1 - P2, Im = P3 - P1 - P2 with P1 = Ar Br, P2 = Ai Bi,
+ * P3 = (Ar + Ai)(Br + Bi); 8/3 of the fp64 rate). Gram matrices, eigensolves and the QR/LQ
+ * factors are always fp64. Returns the previous mode.
  */
 int mpdo_set_math_mode(int mode);
 
@@ -183,6 +186,16 @@ long long mpdo_env_right_elems(int n, const mpdo_site* sites);
 mpdo_status mpdo_env_right(int n, const mpdo_site* sites, void* out, void* stream);
 mpdo_status mpdo_sample_step(const mpdo_site* site, int groups, const void* prev, int prev_count, const int* sel,
                              int sampled, const void* R_next, void* out, double* probs, void* stream);
+
+/*
+ * Batched Hermitian eigendecomposition in fp64 (the Jacobi solver every SVD and QR of the path is
+ * built on; decompositions.py:51-195 computes those factorisations with torch.linalg.svd/qr).
+ * A: device complex128 [batch][n][n] row-major, Hermitian, destroyed; w: device double
+ * [batch][n], eigenvalues in descending order; V: device complex128 [batch][n][n], column j the
+ * eigenvector of w[j]. MPDO_ENOCONV when the sweep cap is hit. Synchronises once. The
+ * differentiable evolution builds its QR, SVD and kappa cuts on it.
+ */
+mpdo_status mpdo_eigh(int n, int batch, void* A, double* w, void* V, void* stream);
 
 /*
  * State vector of a pure MPDO (every inner dimension 1), amplitudes in ket-bit order with the

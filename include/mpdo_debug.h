@@ -82,6 +82,20 @@ int mpdo_debug_layer_kept(int* out, int cap);
  * recorded eigenvalues (per pair its 2 p1 values, descending) into out and returns their count. */
 int mpdo_debug_layer_gate_eigs(int record, double* out, int cap);
 
+/* The fused layer's gate-time cut decides "every direction kept in every chain" by one pivoted
+ * Cholesky of G - D* I (D* the largest eigenvalue the absolute-error rule may drop) and then needs
+ * no eigensolver; on = 0 always runs the eigensolver, on < 0 leaves the switch. Returns the
+ * previous setting. mpdo_debug_layer_gate_skips: how many pairs of the last fused layer on this
+ * thread were decided that way. */
+int mpdo_debug_layer_gate_skip(int on);
+int mpdo_debug_layer_gate_skips(void);
+
+/* The next `count` eigensolver calls of the per-gate entry points (mpdo_apply_2q, the sweeps,
+ * the kappa cut) run to completion and then return MPDO_ENOCONV, as if the sweep cap had been
+ * hit: the host-side route of the reference's perturb-and-retry (Circuit.py:118-136) without any
+ * kernel that fails to finish. 0 turns it off. */
+void mpdo_debug_fail_eig(int count);
+
 /* Launch timing by kernel class (0 complex GEMM, 1 block-Jacobi subproblem solve (bj_solve),
  * 2 block-Jacobi eigenvector update (bj_rotate_v_direct), 3 single-workgroup Jacobi and Cholesky
  * panels, 4 permutation, 5 block-Jacobi two-sided matrix update (bj_update)): HIP events recorded on each

@@ -12,8 +12,8 @@
   oracle fixture: 20 <Z_i>, 19 <Z_i Z_i+1>, 1024 bitstring probabilities, discarded weights.
 
 Tolerances: expectation values and probabilities within 1e-5 of Tr rho~ (complex64 data), chi
-and kappa discarded weights 1e-4 relative, gate-time split weights (bounded by the split's
-absolute error 2.7e-8 squared) within 1e-12 absolute.
+and kappa discarded weights 1e-4 relative, gate-time split weights (the rule drops tails up to
+about 2 x 2.7e-8 x |theta| in total) 1e-3 relative or 1e-12 absolute.
 """
 import os
 
@@ -68,7 +68,7 @@ def _check_layer_weights(disc_b, n, parity, gate, d_chi, d_kap):
     k_g = sum(d_kap.values())
     assert abs(k_f - k_g) <= 1e-4 * k_g + 1e-10
     for p, v in gate.items():
-        assert abs(disc_b[2 * n + p] - v) <= 1e-12
+        assert abs(disc_b[2 * n + p] - v) <= 1e-3 * abs(v) + 1e-12, (p, disc_b[2 * n + p], v)
 
 
 # ------------------------------------------------------------------------------------------
@@ -165,7 +165,7 @@ def test_layer10_chi256_kappa8_golden(cuda):
     assert abs(disc[0, n:2 * n].sum() - ref_kap) <= 1e-4 * ref_kap + 1e-10
     ref_gate = {int(i): w for k, i, w in zip(kinds, g['disc_index'], g['disc_weight']) if k == 'gate'}
     for p, w in ref_gate.items():
-        assert abs(disc[0, 2 * n + p] - w) <= 1e-12
+        assert abs(disc[0, 2 * n + p] - w) <= 1e-3 * abs(w) + 1e-12, (p, disc[0, 2 * n + p], w)
     # both chains against the per-gate entry points
     for b in range(B):
         res, gate, d_chi, d_kap = batched.per_gate_layer([chains[b][i] for i in range(n)], L, b, 256, 8)
@@ -261,3 +261,43 @@ def test_brickwork20_damping_chi64_golden(cuda):
     kf = sum(d[2] for d in c.discarded if d[0] == 'kappa')
     ko = sum(v for k, v in zip(kinds, g['disc_weight']) if k == 'kappa')
     assert abs(kf - ko) <= 1e-4 * ko + 1e-10
+
+
+# ------------------------------------------------------------------------------------------
+def test_gate_cut_keep_all_test_matches_eigensolver(cuda):
+    """The fused layer decides "the gate-time cut keeps every direction" by a pivoted Cholesky of
+    G - D* I instead of the eigensolver: on chains whose gate Grams are well conditioned (flat bond
+    weights) the decision is taken for some pairs, and the layer's outputs and discarded weights
+    are those of the same layer with the eigensolver forced on every pair, to the bit; on the
+    fixture chain (decaying weights: cuts that drop directions) both runs agree as well."""
+    from mpdo_amd import _lib
+    from mpdo_amd import workloads as W
+    lib = _lib.load()
+    n, B = 10, 2
+    _, L = _layer10_inputs(cuda, B)
+    import importlib.util
+    spec = importlib.util.spec_from_file_location('make_golden', os.path.join(HERE, 'golden', 'make_golden.py'))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    for decay in (1.0, 0.97):
+        chains = [[torch.from_numpy(t).to(cuda) for t in W.steady_state(mg.LAYER10_DIMS, seed=11 + b, decay=decay)]
+                  for b in range(B)]
+        sites = [torch.stack([chains[b][i] for b in range(B)]).contiguous() for i in range(n)]
+        old = lib.mpdo_debug_layer_gate_skip(1)
+        try:
+            outs_a, disc_a = _lib.layer_batched(sites, L.kraus1, L.parity, L.gates2, 256, 8)
+            skips = lib.mpdo_debug_layer_gate_skips()
+            kept_a = _lib.layer_kept()
+            lib.mpdo_debug_layer_gate_skip(0)
+            outs_b, disc_b = _lib.layer_batched(sites, L.kraus1, L.parity, L.gates2, 256, 8)
+            assert lib.mpdo_debug_layer_gate_skips() == 0
+            kept_b = _lib.layer_kept()
+        finally:
+            lib.mpdo_debug_layer_gate_skip(old)
+        print(f'decay {decay}: {skips} of {len(kept_a)} gate cuts decided without the eigensolver; kept {kept_b}')
+        if decay == 1.0:
+            assert skips > 0
+        assert kept_a == kept_b
+        for a, b in zip(outs_a, outs_b):
+            assert torch.equal(a, b)
+        assert np.array_equal(disc_a, disc_b)

@@ -11,18 +11,18 @@ def _crand(rng, *shape):
     return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
 
 
-@pytest.mark.parametrize('prec', [0, 1, 2, 3, 4])
+@pytest.mark.parametrize('prec', [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize('M,N,K', [(1, 1, 1), (16, 16, 4), (37, 53, 29), (130, 70, 200), (257, 129, 300)])
 @pytest.mark.parametrize('tA,tB,cA,cB', [(0, 0, 0, 0), (1, 0, 1, 0), (0, 1, 0, 1), (1, 1, 0, 0)])
 def test_gemm(cuda, prec, M, N, K, tA, tB, cA, cB):
     from mpdo_amd import _lib
     rng = np.random.default_rng(M * 1000 + N * 10 + K)
     batch = 2
-    din = np.complex64 if prec in (0, 1, 4) else np.complex128
+    din = np.complex64 if prec in (0, 1, 4, 5) else np.complex128
     A = _crand(rng, batch, *((K, M) if tA else (M, K))).astype(din)
     B = _crand(rng, batch, *((N, K) if tB else (K, N))).astype(din)
     C0 = _crand(rng, batch, M, N)
-    out_t = torch.complex64 if prec in (0, 3, 4) else torch.complex128
+    out_t = torch.complex64 if prec in (0, 3, 4, 5) else torch.complex128
     Cd = torch.from_numpy(C0).to(out_t).to(cuda)
     sa = (1, M, M * K) if tA else (K, 1, M * K)
     sb = (1, K, K * N) if tB else (N, 1, K * N)
@@ -34,10 +34,12 @@ def test_gemm(cuda, prec, M, N, K, tA, tB, cA, cB):
     opA = opA.conj() if cA else opA
     opB = opB.conj() if cB else opB
     ref = alpha * (opA.astype(np.complex128) @ opB.astype(np.complex128)) + beta * C0.astype(
-        np.complex64 if prec in (0, 3, 4) else np.complex128).astype(np.complex128)
+        np.complex64 if prec in (0, 3, 4, 5) else np.complex128).astype(np.complex128)
     got = Cd.cpu().numpy()
     err = np.abs(got - ref).max() / np.abs(ref).max()
-    tol = {0: 3e-6 * np.sqrt(K), 1: 1e-6, 2: 1e-13, 3: 1e-6, 4: 1e-6}[prec]
+    # fp32 accumulation (0: four real products, 5: three, whose imaginary part P3 - P1 - P2 loses
+    # up to a few ulp of |A||B| more)
+    tol = {0: 3e-6 * np.sqrt(K), 1: 1e-6, 2: 1e-13, 3: 1e-6, 4: 1e-6, 5: 6e-6 * np.sqrt(K)}[prec]
     assert err < tol, (prec, M, N, K, err)
 
 

@@ -15,6 +15,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -147,10 +148,18 @@ void carve_eig(Carver& c, int n, Eig& e) {
   e.w = c.get<double>((size_t)n);
   e.ws = c.get<char>(eigh_workspace_bytes(n, 1));
 }
+// test hook (mpdo_debug_fail_eig): the next g_fail_eig eigensolver calls of the per-gate entry
+// points run to completion and then report non-convergence, as a sweep cap would
+std::atomic<int> g_fail_eig{0};
+
 mpdo_status run_eig(int n, double2* G, Eig& e, hipStream_t s) {
   int sweeps = 0;
   CK(eigh(n, 1, G, n, 0, e.w, n, e.V, n, 0, e.ws, s, &sweeps));
   if (sweeps > 60) return fail(MPDO_ENOCONV, "Jacobi eigensolver did not converge (n=%d)", n);
+  int pending = g_fail_eig.load();
+  while (pending > 0 && !g_fail_eig.compare_exchange_weak(pending, pending - 1)) {
+  }
+  if (pending > 0) return fail(MPDO_ENOCONV, "Jacobi eigensolver did not converge (n=%d; forced by the test hook)", n);
   return MPDO_OK;
 }
 
@@ -239,15 +248,120 @@ mpdo_status layer_fail(mpdo_status s, const char* msg) { return fail(s, "%s", ms
 mpdo_status layer_ws(size_t bytes, hipStream_t s, char** base) { return ws_reserve(bytes, s, base); }
 }  // namespace mpdo
 
+namespace {
+// ---------------------------------------------------------------- Pauli strings
+// Every string advances through the chain together: per site, X[t] = E[t]^T T (one strided
+// batched GEMM over the strings, T shared), string t's operator on X[t]'s physical index, then
+// E[t] <- X[t]^T conj(T) (the next environment, complex64 stored, fp64 accumulated): the
+// arithmetic of dmOperations.py:174-201 for each string, with the launches shared.
+struct ExpectLayout {
+  float2 *E0, *E1, *X, *Xp;
+  int* codes;
+  double* res;
+  long long bsE, bsX;
+};
+
+void expect_dims(int n, const mpdo_site* sites, size_t& maxX, int& maxr) {
+  maxX = 1;
+  maxr = 1;
+  for (int i = 0; i < n; ++i) {
+    maxX = std::max(maxX, (size_t)sites[i].l * 2 * sites[i].a * sites[i].r);
+    maxr = std::max(maxr, std::max(sites[i].r, sites[i].l));
+  }
+}
+
+ExpectLayout expect_layout(Carver& c, int n, const mpdo_site* sites, int chunk) {
+  size_t maxX;
+  int maxr;
+  expect_dims(n, sites, maxX, maxr);
+  ExpectLayout L;
+  L.bsE = (long long)maxr * maxr;
+  L.bsX = (long long)maxX;
+  L.E0 = c.get<float2>((size_t)L.bsE * chunk);
+  L.E1 = c.get<float2>((size_t)L.bsE * chunk);
+  L.X = c.get<float2>((size_t)L.bsX * chunk);
+  L.Xp = c.get<float2>((size_t)L.bsX * chunk);
+  L.codes = c.get<int>((size_t)n * chunk);
+  L.res = c.get<double>((size_t)chunk);
+  return L;
+}
+
+}  // namespace
+
+namespace mpdo {
+size_t expect_chunk_bytes(int n, const mpdo_site* sites, int chunk) {
+  Carver c;
+  expect_layout(c, n, sites, chunk);
+  return c.off;
+}
+
+// argument checks; *per = scratch bytes per string (the layout of a chunk of one)
+mpdo_status expect_check(int n, const mpdo_site* sites, int nstrings, const int* codes, double* out, size_t* per) {
+  if (n <= 0 || nstrings <= 0) return MPDO_OK;
+  if (!sites || !codes || !out) return fail(MPDO_EINVAL, "null argument");
+  for (int i = 0; i < n; ++i) {
+    if (bad_site(sites[i])) return fail(MPDO_EINVAL, "site %d: bad descriptor", i);
+    if (i + 1 < n && sites[i].r != sites[i + 1].l) return fail(MPDO_EINVAL, "bond %d mismatch", i);
+  }
+  if (sites[0].l != 1 || sites[n - 1].r != 1) return fail(MPDO_EINVAL, "open boundary bonds must be 1");
+  for (long long e = 0; e < (long long)n * nstrings; ++e)
+    if (codes[e] < -1 || codes[e] > 4) return fail(MPDO_EINVAL, "operator code %d", codes[e]);
+  *per = expect_chunk_bytes(n, sites, 1);
+  return MPDO_OK;
+}
+
+// the strings in chunks of `chunk` through a scratch region laid out by expect_layout at `base`
+mpdo_status expect_run(int n, const mpdo_site* sites, int nstrings, const int* codes, double* out, char* base,
+                       int chunk, hipStream_t s) {
+  Carver c;
+  c.base = base;
+  const ExpectLayout L = expect_layout(c, n, sites, chunk);
+  for (int t0 = 0; t0 < nstrings; t0 += chunk) {
+    const int cnt = std::min(chunk, nstrings - t0);
+    CK(hipMemcpyAsync(L.codes, codes + (size_t)t0 * n, sizeof(int) * (size_t)cnt * n, hipMemcpyHostToDevice, s));
+    CK(env_one(L.E0, L.bsE, cnt, s));
+    float2* E = L.E0;
+    float2* En = L.E1;
+    for (int i = 0; i < n; ++i) {
+      const int l = sites[i].l, a = sites[i].a, r = sites[i].r;
+      const long long cols = 2LL * a * r;
+      // X[t][l', (s a r)] = sum_l E[t][l, l'] T[l, (s a r)]
+      GemmArgs g{};
+      g.M = l; g.N = (int)cols; g.K = l; g.batch = cnt;
+      g.A = E; g.rsA = 1; g.csA = l; g.bsA = L.bsE;
+      g.B = sites[i].data; g.rsB = cols; g.csB = 1; g.bsB = 0;
+      g.C = L.X; g.rsC = cols; g.csC = 1; g.bsC = L.bsX;
+      g.alpha_re = 1;
+      CK(gemm(C64_F64ACC_C64OUT, g, s));
+      CK(phys_op_codes(L.X, L.Xp, l, (long long)a * r, L.bsX, cnt, L.codes, n, i, s));
+      // En[t][r, r'] = sum_{l' s a} Xp[t][l' s a, r] conj(T[l' s a, r'])
+      GemmArgs h{};
+      h.M = r; h.N = r; h.K = l * 2 * a; h.batch = cnt;
+      h.A = L.Xp; h.rsA = 1; h.csA = r; h.bsA = L.bsX;
+      h.B = sites[i].data; h.rsB = r; h.csB = 1; h.bsB = 0; h.conjB = 1;
+      h.C = En; h.rsC = r; h.csC = 1; h.bsC = L.bsE;
+      h.alpha_re = 1;
+      CK(gemm(C64_F64ACC_C64OUT, h, s));
+      std::swap(E, En);
+    }
+    CK(take_re(E, L.bsE, cnt, L.res, s));
+    CK(hipMemcpyAsync(out + t0, L.res, sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
+    CK(hipStreamSynchronize(s));
+  }
+  return MPDO_OK;
+}
+}  // namespace mpdo
+
 // ==========================================================================================
 extern "C" {
 
 int mpdo_version(void) { return 1; }
 
 int mpdo_set_math_mode(int mode) {
-  int old = (g_tensor_prec == C64_F32ACC) ? 1 : 0;
+  const int old = g_tensor_prec == C64_F32ACC ? 1 : g_tensor_prec == C64_F32ACC_3M ? 2 : 0;
   if (mode == 0) g_tensor_prec = C64_F64ACC_C64OUT;
   else if (mode == 1) g_tensor_prec = C64_F32ACC;
+  else if (mode == 2) g_tensor_prec = C64_F32ACC_3M;
   return old;
 }
 const char* mpdo_last_error(void) { return g_err.c_str(); }
@@ -708,72 +822,15 @@ mpdo_status mpdo_svd_kappa(int nsites, const mpdo_site* in, mpdo_site* out, int 
 // ------------------------------------------------------------------------------ observables
 mpdo_status mpdo_pauli_expect(int n, const mpdo_site* sites, int nstrings, const int* codes, double* out,
                               void* stream) {
+  size_t per = 0;
+  ST(expect_check(n, sites, nstrings, codes, out, &per));
   if (n <= 0 || nstrings <= 0) return MPDO_OK;
-  if (!sites || !codes || !out) return fail(MPDO_EINVAL, "null argument");
-  size_t maxX = 0;
-  int maxr = 1;
-  for (int i = 0; i < n; ++i) {
-    if (bad_site(sites[i])) return fail(MPDO_EINVAL, "site %d: bad descriptor", i);
-    if (i + 1 < n && sites[i].r != sites[i + 1].l) return fail(MPDO_EINVAL, "bond %d mismatch", i);
-    maxX = std::max(maxX, (size_t)sites[i].l * 2 * sites[i].a * sites[i].r);
-    maxr = std::max(maxr, std::max(sites[i].r, sites[i].l));
-  }
-  if (sites[0].l != 1 || sites[n - 1].r != 1) return fail(MPDO_EINVAL, "open boundary bonds must be 1");
-  hipStream_t s = S(stream);
-  float2 *E0, *E1, *X, *Xp;
-  auto layout = [&](Carver& c) {
-    E0 = c.get<float2>((size_t)maxr * maxr);
-    E1 = c.get<float2>((size_t)maxr * maxr);
-    X = c.get<float2>(maxX);
-    Xp = c.get<float2>(maxX);
-  };
-  Carver c0;
-  layout(c0);
+  // strings per chunk: every string at once up to 512 MiB of environments and intermediates
+  const int chunk = (int)std::max<size_t>(1, std::min<size_t>({(size_t)nstrings, (size_t)EXPECT_MAX_CHUNK,
+                                                                  (size_t)(512u << 20) / per}));
   char* base;
-  ST(ws_reserve(c0.off, s, &base));
-  Carver c1;
-  c1.base = base;
-  layout(c1);
-  const float2 P[5][4] = {
-      {{0, 0}, {1, 0}, {1, 0}, {0, 0}},     // X
-      {{0, 0}, {0, -1}, {0, 1}, {0, 0}},    // Y
-      {{1, 0}, {0, 0}, {0, 0}, {-1, 0}},    // Z
-      {{1, 0}, {0, 0}, {0, 0}, {0, 0}},     // |0><0|
-      {{0, 0}, {0, 0}, {0, 0}, {1, 0}}};    // |1><1|
-  std::vector<float2> res(nstrings);
-  for (int t = 0; t < nstrings; ++t) {
-    const int* cd = codes + (size_t)t * n;
-    float2 one = make_float2(1.f, 0.f);
-    CK(hipMemcpyAsync(E0, &one, sizeof(float2), hipMemcpyHostToDevice, s));
-    float2* E = E0;
-    float2* En = E1;
-    for (int i = 0; i < n; ++i) {
-      const int l = sites[i].l, a = sites[i].a, r = sites[i].r;
-      const int cols = 2 * a * r;
-      // X[l', (s a r)] = sum_l E[l, l'] T[l, (s a r)]
-      GemmArgs g = gemm_rowmajor(l, cols, l, E, l, true, false, sites[i].data, cols, false, false, X, cols);
-      CK(gemm(C64_F64ACC_C64OUT, g, s));
-      const float2* Xs = X;
-      if (cd[i] >= 0) {
-        if (cd[i] > 4) return fail(MPDO_EINVAL, "operator code %d", cd[i]);
-        CK(phys_op(X, Xp, l, (long long)a * r, P[cd[i]], s));
-        Xs = Xp;
-      }
-      // En[r, r'] = sum_{l' s a} Xs[l' s a, r] conj(T[l' s a, r'])
-      GemmArgs h{};
-      h.M = r; h.N = r; h.K = l * 2 * a; h.batch = 1;
-      h.A = Xs; h.rsA = 1; h.csA = r; h.conjA = 0;
-      h.B = sites[i].data; h.rsB = r; h.csB = 1; h.conjB = 1;
-      h.C = En; h.rsC = r; h.csC = 1;
-      h.alpha_re = 1;
-      CK(gemm(C64_F64ACC_C64OUT, h, s));
-      std::swap(E, En);
-    }
-    CK(hipMemcpyAsync(&res[t], E, sizeof(float2), hipMemcpyDeviceToHost, s));
-    CK(hipStreamSynchronize(s));
-  }
-  for (int t = 0; t < nstrings; ++t) out[t] = res[t].x;
-  return MPDO_OK;
+  ST(ws_reserve(expect_chunk_bytes(n, sites, chunk), S(stream), &base));
+  return expect_run(n, sites, nstrings, codes, out, base, chunk, S(stream));
 }
 
 mpdo_status mpdo_reduced_dm(int n, const mpdo_site* sites, const int* keep, void* out, void* stream) {
@@ -929,14 +986,32 @@ mpdo_status mpdo_contract_state(int n, const mpdo_site* sites, void* out, void* 
   return MPDO_OK;
 }
 
+// ------------------------------------------------------------------------------ eigensolver
+mpdo_status mpdo_eigh(int n, int batch, void* A, double* w, void* V, void* stream) {
+  if (n <= 0 || batch <= 0) return MPDO_OK;
+  if (!A || !w || !V) return fail(MPDO_EINVAL, "null argument");
+  hipStream_t s = S(stream);
+  Carver c0;
+  c0.get<char>(eigh_workspace_bytes(n, batch));
+  char* base;
+  ST(ws_reserve(c0.off, s, &base));
+  int sw = 0;
+  CK(eigh(n, batch, A, n, (long long)n * n, w, n, V, n, (long long)n * n, base, s, &sw));
+  CK(hipStreamSynchronize(s));
+  if (sw > 60) return fail(MPDO_ENOCONV, "Jacobi eigensolver did not converge (n=%d, batch %d)", n, batch);
+  return MPDO_OK;
+}
+
 // ------------------------------------------------------------------------------ test hooks
+void mpdo_debug_fail_eig(int count) { g_fail_eig.store(count > 0 ? count : 0); }
+
 mpdo_status mpdo_debug_gemm_ex(int prec, int herm, int inA, int inB, int tri, int M, int N, int K, int batch,
                                const void* A,
                                long long rsA, long long csA, long long bsA, int conjA, const void* B, long long rsB,
                                long long csB, long long bsB, int conjB, void* C, long long rsC, long long csC,
                                long long bsC, double alpha_re, double alpha_im, double beta_re, double beta_im,
                                void* stream) {
-  if (prec < 0 || prec > 4) return fail(MPDO_EINVAL, "prec %d", prec);
+  if (prec < 0 || prec > 5) return fail(MPDO_EINVAL, "prec %d", prec);
   if (herm && M != N) return fail(MPDO_EINVAL, "Hermitian output needs M == N (%d, %d)", M, N);
   if (inA < 0 || inA > 2 || inB < 0 || inB > 2) return fail(MPDO_EINVAL, "operand storage %d %d", inA, inB);
   if (tri < 0 || tri > 2 || (tri && herm)) return fail(MPDO_EINVAL, "tri %d", tri);

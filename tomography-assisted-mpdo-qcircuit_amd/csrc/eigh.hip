@@ -749,7 +749,65 @@ __global__ void pad_eigs_kernel(const double* wm, int r, double* w, long long ld
   for (int j = threadIdx.x; j < n; j += blockDim.x) w[(long long)b * ldw + j] = j < r ? wm[(long long)b * r + j] : 0.0;
 }
 
+// Gs[b] = G[b] - shift_b I with shift_b = D*_b + margin: D*_b = tr G - (sqrt(tr G) - eps)^2 is
+// the largest eigenvalue the absolute-error cut may drop (te[-1] - sqrt(te[-1]^2 - lambda) <= eps),
+// margin = 1e-9 D* + 1e-13 tr G keeps rounding on the side of running the eigensolver
+__global__ void shift_gram_kernel(const double2* G, long long lda, long long bsA, int n, double eps, double2* Gs) {
+  __shared__ double red[256];
+  const int b = blockIdx.x;
+  const double2* Gb = G + (long long)b * bsA;
+  double2* Sb = Gs + (long long)b * n * n;
+  double t = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) t += Gb[(long long)i * lda + i].x;
+  red[threadIdx.x] = t;
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  const double tr = red[0];
+  const double rt = sqrt(fmax(tr, 0.0));
+  const double dstar = rt > eps ? tr - (rt - eps) * (rt - eps) : tr;
+  const double shift = dstar * (1.0 + 1e-9) + 1e-13 * tr;
+  for (long long e = threadIdx.x; e < (long long)n * n; e += blockDim.x) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    double2 v = Gb[(long long)i * lda + j];
+    if (i == j) v.x -= shift;
+    Sb[e] = v;
+  }
+}
+
 }  // namespace
+
+hipError_t gram_keeps_all(int n, int batch, const void* A, long long lda, long long bsA, double eps, void* ws,
+                          hipStream_t st, bool* all) {
+  *all = false;
+  if (n <= NS || n > PC_MAX || batch <= 0) return hipSuccess;
+  hipError_t err;
+  char* p = (char*)ws;
+  const size_t mat = align256((size_t)n * n * sizeof(double2) * batch);
+  p += mat;                                     // (eigh_gram's L)
+  double2* Gs = (double2*)p; p += mat;          // (M)
+  double2* L = (double2*)p; p += mat;           // (U)
+  p += mat;                                     // (Y)
+  p += align256((size_t)n * sizeof(double) * batch);
+  int* piv = (int*)p; p += align256((size_t)n * sizeof(int) * batch);
+  int* rank = (int*)p; p += align256(sizeof(int) * batch);
+  void* pws = p;
+  hipLaunchKernelGGL(shift_gram_kernel, dim3(batch), dim3(256), 0, st, (const double2*)A, lda, bsA, n, eps, Gs);
+  if ((err = hipGetLastError()) != hipSuccess) return err;
+  // positive definite <=> the pivoted Cholesky completes all n columns with pivots above 1e-14 of
+  // the largest diagonal
+  if ((err = pchol_b(n, batch, Gs, n, (long long)n * n, 1e-14, L, piv, rank, pws, st)) != hipSuccess) return err;
+  std::vector<int> hr(batch);
+  if ((err = hipMemcpyAsync(hr.data(), rank, sizeof(int) * batch, hipMemcpyDeviceToHost, st)) != hipSuccess)
+    return err;
+  if ((err = hipStreamSynchronize(st)) != hipSuccess) return err;
+  bool ok = true;
+  for (int b = 0; b < batch; ++b) ok = ok && hr[b] == n;
+  *all = ok;
+  return hipSuccess;
+}
 
 size_t eigh_gram_workspace_bytes(int n, int batch) {
   const size_t mat = align256((size_t)n * n * sizeof(double2) * batch);

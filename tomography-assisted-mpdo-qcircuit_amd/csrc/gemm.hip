@@ -2,6 +2,8 @@
 //
 // Complex products are formed as four real MFMA products per k-step
 //   Cre += Are*Bre - Aim*Bim,  Cim += Are*Bim + Aim*Bre
+// (or, in the 3M mode for complex64 with fp32 accumulation, three: Are*Bre, Aim*Bim and
+// (Are + Aim)*(Bre + Bim), combined once at the end)
 // with v_mfma_f32_16x16x4_f32 (complex64, fp32 accumulate) or v_mfma_f64_16x16x4_f64
 // (fp64 accumulate; complex64 inputs are widened on the way into LDS, which makes Gram
 // matrices of complex64 data exact up to fp64 rounding).
@@ -54,12 +56,16 @@ template <> __device__ inline float2 cvt<double2, float2>(double2 v) { return ma
 // widened as it is read. Register budget for three workgroups per CU when the LDS allows it (one
 // stage: 33 KiB, 60 -> 66 TFLOP/s at 2048 x 8192 x 512; two stages with one complex64 operand:
 // 50 KiB), two for two complex128 stages (66 KiB).
-template <typename TInA, typename TInB, int NBUF>
+// G3: three real products per complex product (Gauss): P1 = Are Bre, P2 = Aim Bim,
+// P3 = (Are + Aim)(Bre + Bim), Cre = P1 - P2, Cim = P3 - P1 - P2 (a third accumulator, so two
+// workgroups per CU).
+template <typename TInA, typename TInB, int NBUF, bool G3>
 constexpr int gemm_wg_per_cu() {
-  return (NBUF == 1 || !std::is_same<TInA, TInB>::value) ? 3 : 2;
+  return G3 ? 2 : (NBUF == 1 || !std::is_same<TInA, TInB>::value) ? 3 : 2;
 }
-template <typename TInA, typename TInB, typename TA, typename TOut, int BM, int BN, int BK, int NBUF = 1>
-__global__ __launch_bounds__(256, (gemm_wg_per_cu<TInA, TInB, NBUF>())) void gemm_kernel(GemmArgs g) {
+template <typename TInA, typename TInB, typename TA, typename TOut, int BM, int BN, int BK, int NBUF = 1,
+          bool G3 = false>
+__global__ __launch_bounds__(256, (gemm_wg_per_cu<TInA, TInB, NBUF, G3>())) void gemm_kernel(GemmArgs g) {
   typedef typename Acc<TA>::V V;
   typedef typename Acc<TA>::C CA;   // complex in accumulation precision
   // LDS elements: the accumulation type, or the complex64 storage of a complex64 operand when
@@ -103,13 +109,22 @@ __global__ __launch_bounds__(256, (gemm_wg_per_cu<TInA, TInB, NBUF>())) void gem
   const bool bK = (g.rsB == 1);   // B contiguous along k
   const float sA = g.conjA ? -1.f : 1.f, sB = g.conjB ? -1.f : 1.f;
 
-  V accr[TM][TN], acci[TM][TN];
+  // G3: accr = P1, acci = P3, acc2 = P2
+  V accr[TM][TN], acci[TM][TN], acc2[G3 ? TM : 1][G3 ? TN : 1];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j)
 #pragma unroll
       for (int q = 0; q < 4; ++q) { accr[i][j][q] = 0; acci[i][j][q] = 0; }
+  if constexpr (G3) {
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc2[i][j][q] = 0;
+  }
 
   TInA ra[PA];
   TInB rb[PB];
@@ -167,15 +182,31 @@ __global__ __launch_bounds__(256, (gemm_wg_per_cu<TInA, TInB, NBUF>())) void gem
       for (int i = 0; i < TM; ++i) a[i] = cvt<LA, CA>(As[buf][kk + lk][wm + i * 16 + lr]);
 #pragma unroll
       for (int j = 0; j < TN; ++j) bb[j] = cvt<LB, CA>(Bs[buf][kk + lk][wn + j * 16 + lr]);
+      if constexpr (G3) {
+        TA as[TM], bs[TN];
 #pragma unroll
-      for (int i = 0; i < TM; ++i)
+        for (int i = 0; i < TM; ++i) as[i] = a[i].x + a[i].y;
 #pragma unroll
-        for (int j = 0; j < TN; ++j) {
-          accr[i][j] = Acc<TA>::mfma(a[i].x, bb[j].x, accr[i][j]);
-          accr[i][j] = Acc<TA>::mfma(-a[i].y, bb[j].y, accr[i][j]);
-          acci[i][j] = Acc<TA>::mfma(a[i].x, bb[j].y, acci[i][j]);
-          acci[i][j] = Acc<TA>::mfma(a[i].y, bb[j].x, acci[i][j]);
-        }
+        for (int j = 0; j < TN; ++j) bs[j] = bb[j].x + bb[j].y;
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            accr[i][j] = Acc<TA>::mfma(a[i].x, bb[j].x, accr[i][j]);
+            acc2[i][j] = Acc<TA>::mfma(a[i].y, bb[j].y, acc2[i][j]);
+            acci[i][j] = Acc<TA>::mfma(as[i], bs[j], acci[i][j]);
+          }
+      } else {
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            accr[i][j] = Acc<TA>::mfma(a[i].x, bb[j].x, accr[i][j]);
+            accr[i][j] = Acc<TA>::mfma(-a[i].y, bb[j].y, accr[i][j]);
+            acci[i][j] = Acc<TA>::mfma(a[i].x, bb[j].y, acci[i][j]);
+            acci[i][j] = Acc<TA>::mfma(a[i].y, bb[j].x, acci[i][j]);
+          }
+      }
     }
   };
   // a triangular operand leaves only part of the reduction nonzero for this tile
@@ -218,6 +249,11 @@ __global__ __launch_bounds__(256, (gemm_wg_per_cu<TInA, TInB, NBUF>())) void gem
         int col = n0 + wn + j * 16 + lr;
         if (row < g.M && col < g.N) {
           double xr = accr[i][j][q], xi = acci[i][j][q];
+          if constexpr (G3) {
+            const TA p1 = accr[i][j][q], p2 = acc2[i][j][q], p3 = acci[i][j][q];
+            xr = p1 - p2;
+            xi = p3 - p1 - p2;
+          }
           double yr = g.alpha_re * xr - g.alpha_im * xi;
           double yi = g.alpha_re * xi + g.alpha_im * xr;
           long long off = (long long)row * g.rsC + (long long)col * g.csC;
@@ -270,7 +306,8 @@ static hipError_t herm_mirror(const GemmArgs& g, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <typename TInA, typename TInB, typename TA, typename TOut, int BM, int BN, int BK, int NBUF = 1>
+template <typename TInA, typename TInB, typename TA, typename TOut, int BM, int BN, int BK, int NBUF = 1,
+          bool G3 = false>
 static hipError_t launch(const GemmArgs& g0, hipStream_t s) {
   if (g0.M <= 0 || g0.N <= 0 || g0.batch <= 0) return hipSuccess;
   GemmArgs g = g0;
@@ -280,7 +317,7 @@ static hipError_t launch(const GemmArgs& g0, hipStream_t s) {
   const int tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
   const int tiles = g.herm ? tm * (tm + 1) / 2 : tm * tn;
   dim3 grid(tiles, split, g.batch);
-  hipLaunchKernelGGL((gemm_kernel<TInA, TInB, TA, TOut, BM, BN, BK, NBUF>), grid, dim3(256), 0, s, g);
+  hipLaunchKernelGGL((gemm_kernel<TInA, TInB, TA, TOut, BM, BN, BK, NBUF, G3>), grid, dim3(256), 0, s, g);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !g0.herm || split > 1) return e;
   return herm_mirror<TOut>(g0, s);
@@ -333,7 +370,7 @@ hipError_t gemm(Prec p, const GemmArgs& g, hipStream_t s) {
   if (g.tri < 0 || g.tri > 2 || (g.tri && (g.ksplit > 1 || g.herm))) return hipErrorInvalidValue;
   const bool in128 = (p == C128_F64ACC || p == C128_IN_C64_OUT);
   const bool a128 = g.inA ? g.inA == 2 : in128, b128 = g.inB ? g.inB == 2 : in128;
-  if (p == C64_F32ACC && (a128 || b128)) return hipErrorInvalidValue;
+  if ((p == C64_F32ACC || p == C64_F32ACC_3M) && (a128 || b128)) return hipErrorInvalidValue;
   const double inbA = a128 ? 16 : 8, inbB = b128 ? 16 : 8;
   const double outb = (p == C64_F64ACC || p == C128_F64ACC) ? 16 : 8;
   const double mn = (double)g.M * g.N, nb = g.batch;
@@ -353,7 +390,8 @@ hipError_t gemm(Prec p, const GemmArgs& g, hipStream_t s) {
                        outb * mn * ((g.beta_re != 0 || g.beta_im != 0) ? 2 : 1)),
                  shape);
   switch (p) {
-    case C64_F32ACC: return launch<float2, float2, float, float2, 128, 128, 16>(g, s);
+    case C64_F32ACC: return launch<float2, float2, float, float2, 128, 64, 16>(g, s);
+    case C64_F32ACC_3M: return launch<float2, float2, float, float2, 128, 64, 16, 1, true>(g, s);
     case C64_F64ACC:
     case C128_F64ACC: return launch_f64_mixed<double2>(a128, b128, g, s);
     case C128_IN_C64_OUT:

@@ -85,6 +85,8 @@ thread_local std::vector<int> t_kept;   // per pair of the last layer: max kept 
 thread_local bool t_rec_gate = false;
 thread_local std::vector<double> t_gate_eigs;
 thread_local std::vector<int> t_gemms;  // dry run: (source line, M, N, K, batch, precision) of every GEMM
+thread_local int t_gate_skips = 0;     // gate-time cuts decided by the shifted Cholesky (last layer)
+bool g_gate_skip = true;               // mpdo_debug_layer_gate_skip
 
 void add_span(const void* p, size_t bytes) {
   if (t_spans && p && bytes) t_spans->push_back(Span{(uintptr_t)p, (uintptr_t)p + bytes});
@@ -1121,6 +1123,7 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
   };
 
   t_kept.clear();
+  t_gate_skips = 0;
   t_gate_eigs.clear();
   // ---------------- phase 2: QR sweep left to right ----------------
   // Rcur: [B][kp][l] with kp the QR bond so far (1 at the left end)
@@ -1200,32 +1203,45 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
       // 8 x 8 Gram of the channel (16x fewer flops than C C^H over the 2 K2 p2 columns)
       LCK(gram_rows_b(ps.Y, 4 * p1t, ps.p2, 4LL * p1t * ps.p2, Pg, B, s));
       LCK(k_kraus_gram(Pg, dG + (pi - 1) * 16 * K2, K2, p1t, G, B, s, __LINE__));
-      LST(eig_b(nT, B, G, eE, s, true));
-      if (t_rec_gate && !dry) {
-        const size_t o = t_gate_eigs.size();
-        t_gate_eigs.resize(o + nT);
-        LCK(hipMemcpyAsync(t_gate_eigs.data() + o, eE.w, sizeof(double) * nT, hipMemcpyDeviceToHost, s));
-        LCK(hipStreamSynchronize(s));
-      }
-      LCK(K_KEEP(eE.w, nT, nT, len_gate, -1, GATE_EPS, 0, dk, ddisc + (size_t)B * (nsteps + n + i), B, s));
-      // the projector onto the kept directions is applied only when some chain drops one; the
-      // core then lives in the mq-dimensional span of the kept directions (mq = max over chains)
-      ps.has_P = dry;
-      ps.mq = nT;
-      if (!dry) {
-        std::vector<int> kept(B);
-        LCK(hipMemcpyAsync(kept.data(), dk, sizeof(int) * B, hipMemcpyDeviceToHost, s));
-        LCK(hipStreamSynchronize(s));
-        ps.mq = 1;
-        for (int b = 0; b < B; ++b) {
-          ps.has_P = ps.has_P || kept[b] < nT;
-          ps.mq = std::max(ps.mq, std::min(kept[b], nT));
+      // the cut keeps every direction in every chain (then no eigenvector is needed: P = I, nothing
+      // is discarded) exactly when G - D* I is positive definite, D* the largest eigenvalue the
+      // absolute-error rule may drop: one shifted pivoted Cholesky instead of the eigensolver
+      bool keeps_all = false;
+      if (!dry && !t_rec_gate && use_gram_eig() && len_gate >= nT && g_gate_skip)
+        LCK(gram_keeps_all(nT, B, G, nT, (long long)nT * nT, GATE_EPS, eE.ws, s, &keeps_all));
+      if (keeps_all) {
+        ps.has_P = false;
+        ps.mq = nT;
+        t_kept.push_back(nT);
+        ++t_gate_skips;
+      } else {
+        LST(eig_b(nT, B, G, eE, s, true));
+        if (t_rec_gate && !dry) {
+          const size_t o = t_gate_eigs.size();
+          t_gate_eigs.resize(o + nT);
+          LCK(hipMemcpyAsync(t_gate_eigs.data() + o, eE.w, sizeof(double) * nT, hipMemcpyDeviceToHost, s));
+          LCK(hipStreamSynchronize(s));
         }
+        LCK(K_KEEP(eE.w, nT, nT, len_gate, -1, GATE_EPS, 0, dk, ddisc + (size_t)B * (nsteps + n + i), B, s));
+        // the projector onto the kept directions is applied only when some chain drops one; the
+        // core then lives in the mq-dimensional span of the kept directions (mq = max over chains)
+        ps.has_P = dry;
+        ps.mq = nT;
+        if (!dry) {
+          std::vector<int> kept(B);
+          LCK(hipMemcpyAsync(kept.data(), dk, sizeof(int) * B, hipMemcpyDeviceToHost, s));
+          LCK(hipStreamSynchronize(s));
+          ps.mq = 1;
+          for (int b = 0; b < B; ++b) {
+            ps.has_P = ps.has_P || kept[b] < nT;
+            ps.mq = std::max(ps.mq, std::min(kept[b], nT));
+          }
+        }
+        if (!dry) t_kept.push_back(ps.has_P ? ps.mq : nT);
+        if (ps.has_P)
+          LCK(k_mask(eE.V, nT, (long long)nT * nT, nT, ps.mq, dk, ps.Um, ps.mq, (long long)nT * ps.mq, B, s,
+                     __LINE__));
       }
-      if (!dry) t_kept.push_back(ps.has_P ? ps.mq : nT);
-      if (ps.has_P)
-        LCK(k_mask(eE.V, nT, (long long)nT * nT, nT, ps.mq, dk, ps.Um, ps.mq, (long long)nT * ps.mq, B, s,
-                   __LINE__));
     }
     // (2) QR sweep through the pair: X1r = (R (x) I_a1) Q1t = Q1 Ra, [(kcur a1) x p1t]
     LCK(mmb(g_tensor_prec, kcur, a1 * p1t, l, Rcur, l, (long long)kcur * l, Q1t, a1 * p1t,
@@ -1630,6 +1646,14 @@ int mpdo_debug_layer_gate_eigs(int record, double* out, int cap) {
   for (int i = 0; i < n && i < cap; ++i) out[i] = t_gate_eigs[i];
   return n;
 }
+
+int mpdo_debug_layer_gate_skip(int on) {
+  const int old = g_gate_skip ? 1 : 0;
+  if (on >= 0) g_gate_skip = on != 0;
+  return old;
+}
+
+int mpdo_debug_layer_gate_skips(void) { return t_gate_skips; }
 
 int mpdo_debug_layer_kept(int* out, int cap) {
   const int n = (int)t_kept.size();

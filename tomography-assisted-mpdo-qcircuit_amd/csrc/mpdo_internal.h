@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "mpdo.h"
+
 namespace mpdo {
 
 // General strided batched complex GEMM:  C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b]
@@ -37,7 +39,8 @@ enum Prec { C64_F32ACC = 0,   // complex64 in, fp32 MFMA accumulate, complex64 o
             C64_F64ACC = 1,   // complex64 in, fp64 MFMA accumulate, complex128 out
             C128_F64ACC = 2,  // complex128 in, fp64 MFMA accumulate, complex128 out
             C128_IN_C64_OUT = 3,    // complex128 in, fp64 accumulate, complex64 out
-            C64_F64ACC_C64OUT = 4 };  // complex64 in, fp64 accumulate, complex64 out
+            C64_F64ACC_C64OUT = 4,    // complex64 in, fp64 accumulate, complex64 out
+            C64_F32ACC_3M = 5 };      // complex64 in, fp32 MFMA accumulate of 3 real products, complex64 out
 
 hipError_t gemm(Prec p, const GemmArgs& g, hipStream_t s);
 // C[b][i][j] = conj(C[b][j][i]) for j > i (n x n row-major, batch stride bs)
@@ -67,6 +70,16 @@ size_t eigh_gram_workspace_bytes(int n, int batch);
 hipError_t eigh_gram(int n, int batch, void* A, long long lda, long long bsA,
                      double* w, long long ldw, void* V, long long ldv, long long bsV,
                      void* ws, hipStream_t s, int* sweeps_out, double tol_rel = 1e-14);
+
+// Whether the absolute-error cut of decompositions.svd (te = sqrt(cumsum lambda), keep idx + 1
+// values for the first idx with te[-1] - te[idx] <= eps; decompositions.py:121-131) keeps every
+// one of the n directions of each Gram matrix A[b]: *all = true when, for every b, A[b] - D*_b I is
+// positive definite, D*_b = tr A[b] - (sqrt(tr A[b]) - eps)^2 being the largest eigenvalue the
+// rule may drop (decided by a pivoted Cholesky of the shifted matrix completing all n columns,
+// with a small margin towards "no"). A is not modified; `ws`: eigh_gram_workspace_bytes(n, batch).
+// 64 < n <= 1024 (else *all = false). Synchronises once.
+hipError_t gram_keeps_all(int n, int batch, const void* A, long long lda, long long bsA, double eps, void* ws,
+                          hipStream_t st, bool* all);
 
 // Pivoted Cholesky of positive semidefinite Gram matrices A[b] (n x n, leading dim lda, batch
 // stride bsA; destroyed): L[b] (n x n, stored by columns: L[b][k n + i] = L_ik) with A = L L^H,
@@ -99,4 +112,14 @@ hipError_t gather_piv(const float2* M, const int* piv, const int* rank, int n, l
 hipError_t pchol_inverse_b(int n, int batch, const double2* L, const int* piv, const int* rank, int mode, double2* X,
                            void* ws, hipStream_t s);
 
+// Pauli-string expectations of many strings at once through a caller-provided scratch region
+// (api.hip; the arithmetic of mpdo_pauli_expect): expect_check validates the arguments and gives
+// the scratch bytes one string needs; expect_chunk_bytes the bytes of a chunk of `chunk` strings;
+// expect_run evaluates the strings `chunk` at a time in the scratch at `base` (synchronises once
+// per chunk). Failures set the thread's error text.
+constexpr int EXPECT_MAX_CHUNK = 4096;
+mpdo_status expect_check(int n, const mpdo_site* sites, int nstrings, const int* codes, double* out, size_t* per);
+size_t expect_chunk_bytes(int n, const mpdo_site* sites, int chunk);
+mpdo_status expect_run(int n, const mpdo_site* sites, int nstrings, const int* codes, double* out, char* base,
+                       int chunk, hipStream_t s);
 }  // namespace mpdo

@@ -66,4 +66,12 @@ hipError_t phys_op(const float2* X, float2* out, int L, long long rest, const fl
 
 hipError_t trace_prod(const float2* A, const float2* B, int n, double2* out, hipStream_t s);
 
+// Pauli-string expectations of many strings at once (batch entry t = string t, stride bs):
+// E[t][0] = 1; string t's single-site operator (codes[t n + site]: -1 identity, 0 X, 1 Y, 2 Z,
+// 3 |0><0|, 4 |1><1|) on the physical index of X[t][L][2][rest]; res[t] = Re E[t][0]
+hipError_t env_one(float2* E, long long bs, int count, hipStream_t s);
+hipError_t phys_op_codes(const float2* X, float2* out, int L, long long rest, long long bs, int count, const int* codes,
+                         int n, int site, hipStream_t s);
+hipError_t take_re(const float2* E, long long bs, int count, double* res, hipStream_t s);
+
 }  // namespace mpdo

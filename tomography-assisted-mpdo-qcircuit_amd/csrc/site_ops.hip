@@ -671,3 +671,65 @@ hipError_t trace_prod(const float2* A, const float2* B, int n, double2* out, hip
 }
 
 }  // namespace mpdo
+
+namespace mpdo {
+
+// ---------------------------------------------------------------- Pauli strings, many at once
+// E[t][0] = 1 for t < count (the left boundary environment of every string)
+__global__ void env_one_kernel(float2* E, long long bs, int count) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) E[(long long)t * bs] = make_float2(1.f, 0.f);
+}
+
+hipError_t env_one(float2* E, long long bs, int count, hipStream_t s) {
+  hipLaunchKernelGGL(env_one_kernel, dim3((count + 255) / 256), dim3(256), 0, s, E, bs, count);
+  return hipGetLastError();
+}
+
+// String t's operator at this site applied to the physical index of X[t][l][s][rest]:
+// out[t][l][s'][rest] = sum_s P[s'][s] X[t][l][s][rest], P from codes[t * n + site] (0 X, 1 Y,
+// 2 Z, 3 |0><0|, 4 |1><1|; -1 the identity: a copy)
+__global__ void phys_op_codes_kernel(const float2* X, float2* out, int L, long long rest, long long bs, int count,
+                                     const int* codes, int n, int site) {
+  const long long per = (long long)L * rest, tot = per * count;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int t = (int)(e / per);
+    const long long w = e % per, l = w / rest, c = w % rest;
+    const int code = codes[(long long)t * n + site];
+    const float2 x0 = X[t * bs + (l * 2) * rest + c], x1 = X[t * bs + (l * 2 + 1) * rest + c];
+    float2 y0 = x0, y1 = x1;
+    switch (code) {
+      case 0: y0 = x1; y1 = x0; break;                                                  // X
+      case 1: y0 = make_float2(x1.y, -x1.x); y1 = make_float2(-x0.y, x0.x); break;     // Y: (-i x1, i x0)
+      case 2: y1 = make_float2(-x1.x, -x1.y); break;                                    // Z
+      case 3: y1 = make_float2(0.f, 0.f); break;                                        // |0><0|
+      case 4: y0 = make_float2(0.f, 0.f); break;                                        // |1><1|
+      default: break;
+    }
+    out[t * bs + (l * 2) * rest + c] = y0;
+    out[t * bs + (l * 2 + 1) * rest + c] = y1;
+  }
+}
+
+hipError_t phys_op_codes(const float2* X, float2* out, int L, long long rest, long long bs, int count, const int* codes,
+                         int n, int site, hipStream_t s) {
+  const long long tot = (long long)L * rest * count;
+  int blocks = (int)std::min<long long>((tot + 255) / 256, 16384);
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(phys_op_codes_kernel, dim3(blocks), dim3(256), 0, s, X, out, L, rest, bs, count, codes, n, site);
+  return hipGetLastError();
+}
+
+// res[t] = Re E[t][0]
+__global__ void take_re_kernel(const float2* E, long long bs, int count, double* res) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) res[t] = (double)E[(long long)t * bs].x;
+}
+
+hipError_t take_re(const float2* E, long long bs, int count, double* res, hipStream_t s) {
+  hipLaunchKernelGGL(take_re_kernel, dim3((count + 255) / 256), dim3(256), 0, s, E, bs, count, res);
+  return hipGetLastError();
+}
+
+}  // namespace mpdo

@@ -55,6 +55,7 @@ def load() -> ctypes.CDLL:
         'mpdo_trace_rho_rho': (I, [I, SP, SP, P(D), VP]),
         'mpdo_trace_rho_rho_masked': (I, [I, SP, SP, P(I), P(D), VP]),
         'mpdo_contract_state': (I, [I, SP, VP, VP]),
+        'mpdo_eigh': (I, [I, I, VP, P(D), VP, VP]),
         'mpdo_env_right_elems': (ctypes.c_longlong, [I, SP]),
         'mpdo_env_right': (I, [I, SP, VP, VP]),
         'mpdo_sample_step': (I, [SP, I, VP, I, P(I), I, VP, VP, P(D), VP]),
@@ -72,6 +73,9 @@ def load() -> ctypes.CDLL:
         'mpdo_debug_eigh_gram': (I, [I, I, VP, P(D), VP, P(I), VP]),
         'mpdo_debug_layer_dry_run': (I, [I, I, SP, I, I, I, I, I, SP, P(ctypes.c_ulonglong)]),
         'mpdo_debug_prof_enable': (I, [I]),
+        'mpdo_debug_fail_eig': (None, [I]),
+        'mpdo_debug_layer_gate_skip': (I, [I]),
+        'mpdo_debug_layer_gate_skips': (I, []),
         'mpdo_debug_layer_kept': (I, [P(I), I]),
         'mpdo_debug_layer_gate_eigs': (I, [I, P(D), I]),
         'mpdo_debug_layer_gemm_log': (I, [P(I), I]),
@@ -401,6 +405,22 @@ def layer_batched(sites: Sequence[torch.Tensor], kraus1: torch.Tensor, parity: i
                                 _stream(dev))
     check(st, 'mpdo_layer_batched')
     return outs_t, disc
+
+
+def eigh(A: torch.Tensor):
+    """Batched Hermitian eigendecomposition (fp64 Jacobi on the device). A: (batch, n, n)
+    complex128 Hermitian (not modified). Returns (w (batch, n) float64 descending, V (batch, n, n)
+    complex128, column j the eigenvector of w[:, j])."""
+    lib = load()
+    if A.device.type != 'cuda' or A.dtype != torch.complex128 or A.dim() != 3 or A.shape[1] != A.shape[2]:
+        raise ValueError('eigh takes a (batch, n, n) complex128 tensor on the device')
+    b, n, _ = A.shape
+    work = A.contiguous().clone()
+    w = torch.empty(b, n, dtype=torch.float64, device=A.device)
+    V = torch.empty(b, n, n, dtype=torch.complex128, device=A.device)
+    check(lib.mpdo_eigh(n, b, ctypes.c_void_p(work.data_ptr()), ctypes.cast(w.data_ptr(), ctypes.POINTER(ctypes.c_double)),
+                        ctypes.c_void_p(V.data_ptr()), _stream(A.device)), 'mpdo_eigh')
+    return w, V
 
 
 def contract_state(sites: Sequence[torch.Tensor]) -> torch.Tensor:

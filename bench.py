@@ -180,6 +180,10 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-threads', type=int, default=None,
                     help='threads of the CPU oracle sample (default: the cores this process may run on)')
+    ap.add_argument('--math', choices=['fp64acc', 'fp32x3m'], default='fp64acc',
+                    help='contractions whose results are complex64 site data: fp64 MFMA accumulation, or three real '
+                         'products per complex product accumulated in fp32 (3M; Gram matrices, QR/LQ factors and '
+                         'eigensolves stay fp64)')
     ap.add_argument('--fused-only', action='store_true',
                     help='time the fused steps only (no profiled pass, plain path or accuracy check): the '
                          'command of the rocprofv3 counter passes')
@@ -213,7 +217,8 @@ def main():
         _log(f'copy calibration: 3 launches of torch.neg, {CALIB_BYTES} bytes read and {CALIB_BYTES} bytes '
              'written each')
     lib = _lib.load()
-    prev_mode = lib.mpdo_set_math_mode(0)
+    math_mode = {'fp64acc': 0, 'fp32x3m': 2}[args.math]
+    prev_mode = lib.mpdo_set_math_mode(math_mode)
     B = args.copies
     layers = _layers(B, args.warmup + args.steps, dev)
     dims = batched.steady_dims(N_QUBITS, CHI, KAPPA, layers[0].kraus1.shape[2], layers[0].gates2.shape[-1],
@@ -257,7 +262,8 @@ def main():
         eig_log = _lib.prof_eig_log()
         shapes = sorted(_lib.prof_shapes('gemm'), key=lambda r: -r[2])
         _lib.prof_enable(False)
-        _log(f'profiled pass: gate-time split kept counts {_lib.layer_kept()}')
+        _log(f'profiled pass: gate-time split kept counts {_lib.layer_kept()}; '
+             f'{lib.mpdo_debug_layer_gate_skips()} cuts decided by the shifted Cholesky (no eigensolver)')
         eigs = {}
         for (n_e, b_e, sw_e) in eig_log:
             c = eigs.setdefault(n_e, [0, 0])
@@ -299,6 +305,7 @@ def main():
         v_f = np.array(_lib.pauli_expect([t[0].contiguous() for t in outs], codes))
         chain0 = [t[0] for t in saved]
         del outs
+        lib.mpdo_set_math_mode(0)
         torch.cuda.synchronize()
         t = time.perf_counter()
         ref, gate, d_chi, d_kap = batched.per_gate_layer(chain0, L0, 0, CHI, KAPPA)
@@ -337,7 +344,9 @@ def main():
             dims10 = [tuple(int(x) for x in d) for d in batched.steady_dims(10, CHI, KAPPA, 4, 16, 1)]
             ch = [torch.from_numpy(a).to(dev).unsqueeze(0).contiguous() for a in W.steady_state(dims10, seed=7)]
             L10 = _layers(1, 1, dev, n=10, first=1)[0]    # the fixture's layer: pairs (1,2),(3,4),...
+            lib.mpdo_set_math_mode(math_mode)
             o10, _ = _lib.layer_batched(ch, L10.kraus1, L10.parity, L10.gates2, CHI, KAPPA)
+            lib.mpdo_set_math_mode(0)
             c10 = _zz_codes(10)
             v10 = np.array(_lib.pauli_expect([t[0].contiguous() for t in o10], c10))
             ref10 = list(g['z']) + [g['zz'][i, j] for i in range(10) for j in range(i + 1, 10)] + [float(g['trace'])]
@@ -361,7 +370,9 @@ def main():
     line = {'metric': METRIC, 'value': round(value, 6), 'unit': 'circuit-layers/s', 'n_gpus': 1,
             'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': round(T / args.steps * 1e3, 3),
             'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
-            'dtype': 'c64 data, fp64 accumulate/eigensolve', 'data': 'synthetic',
+            'dtype': ('c64 data, fp64 accumulate/eigensolve' if math_mode == 0 else
+                      'c64 data; site contractions 3M fp32-accumulate MFMA, Grams/QR/eigensolves fp64'),
+            'data': 'synthetic',
             'config': {'workload': ('32-qubit depth-30 RY/RZ+CNOT brickwork, depolarizing p=0.002 (1q, 2q), '
                                     'chi=256, kappa=8, 32 angle-perturbed copies; step = one circuit-layer of '
                                     'every copy from the steady state (fused batched layer)'),

@@ -9,8 +9,9 @@
  * *_dims / *_capacity query gives; the entry points return the actual dimensions in the
  * output descriptors. Every entry point is stream-ordered on `stream` (a hipStream_t, NULL =
  * default stream) and returns after the results are complete when it has to read a
- * data-dependent rank back (documented per call). Workspace is a library-internal,
- * grow-only device buffer per device; calls from several host threads must not overlap.
+ * data-dependent rank back (documented per call). Scratch comes from a workspace per (device,
+ * stream): the caller's (mpdo_bind_workspace) or the library's own grow-only buffer; calls on
+ * different streams may run concurrently from several host threads.
  */
 #ifndef MPDO_H
 #define MPDO_H
@@ -39,8 +40,19 @@ typedef struct {
 int mpdo_version(void);
 /* Thread-local text of the last failure. */
 const char* mpdo_last_error(void);
-/* Frees the internal workspace of the current device. */
+/* Frees the library's own workspace buffers of the current device (not bound ones). */
 void mpdo_release_workspace(void);
+/*
+ * Caller-owned workspace (no counterpart in the reference, whose tensors torch allocates). Binds
+ * the device buffer ptr (bytes long, 256-byte aligned) to `stream` on the current device: every
+ * later call on that stream carves its scratch from it and allocates nothing; ptr NULL unbinds.
+ * A call that needs more than the bound buffer returns MPDO_ENOMEM before launching anything and
+ * mpdo_workspace_needed() (thread-local) then gives the bytes it needs. Without a binding the
+ * library keeps a grow-only buffer of its own per (device, stream). Calls on different streams
+ * use different buffers and may overlap in time; calls on one stream must not.
+ */
+mpdo_status mpdo_bind_workspace(void* stream, void* ptr, size_t bytes);
+size_t mpdo_workspace_needed(void);
 /*
  * Accumulation precision of the complex64 tensor contractions whose results are complex64 site
  * data (no counterpart in the reference, which computes in the dtype it is given): 0 = fp64
@@ -161,6 +173,10 @@ mpdo_status mpdo_trace_rho_rho_masked(int n, const mpdo_site* a, const mpdo_site
  * (the kept count of its gate-time split) and at the end.
  */
 int mpdo_layer_dims(int n, const mpdo_site* in, int K1, int parity, int K2, int chi, int kappa, mpdo_site* out);
+/* Workspace bytes mpdo_layer_batched needs for these chains (dims of `in` only; data ignored);
+ * 0 when the layer does not qualify (error text set). */
+size_t mpdo_layer_workspace_bytes(int batch, int n, const mpdo_site* in, int K1, int parity, int K2, int chi,
+                                  int kappa);
 mpdo_status mpdo_layer_batched(int batch, int n, const mpdo_site* in, const float* kraus1, int K1, int parity,
                                const float* gates2, int K2, int chi, int kappa, const mpdo_site* out,
                                double* discarded, void* stream);

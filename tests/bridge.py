@@ -88,3 +88,32 @@ def run_oracle_fused(w, st=None, layers=None):
     if not w.ideal and ops and ops[-1][0] != 'TRUNCATE':
         O.svd_kappa_left2right(st, w.kappa)
     return st
+
+
+def run_hea_oracle(theta, phi, p1, p2, chi, kappa, h=1.0):
+    """The hardware-efficient ansatz of mpdo_amd.autodiff.HEA on the oracle (complex128): per layer
+    RY(theta) RZ(phi) on every qubit, each followed by depolarizing of its own strength p1[d, q, 0/1]
+    (composed into one Kraus set by oracle.compose_compressed), CNOT(q, q+1) for q = 0..n-2 followed by
+    two-qubit depolarizing p2[d, q] (oracle.apply_2q), then bond_truncate(chi) and the kappa cut.
+    Returns (<H> unnormalised, H = -sum Z_i Z_i+1 - h sum X_i; the final state)."""
+    theta, phi, p1, p2 = (np.asarray(x, dtype=np.float64) for x in (theta, phi, p1, p2))
+    depth, n = theta.shape
+    st = O.product_state(n)
+    for d in range(depth):
+        for q in range(n):
+            e1 = np.einsum('nlm,li->nim', O.depolarizing_tensor(float(p1[d, q, 0]), 1), O.gate_matrix('RY', [theta[d, q]]))
+            e2 = np.einsum('nlm,li->nim', O.depolarizing_tensor(float(p1[d, q, 1]), 1), O.gate_matrix('RZ', [phi[d, q]]))
+            st.sites[q] = O.apply_1q(st.sites[q], O.compose_compressed([e1, e2]))
+            st.has_I[q] = True
+        for q in range(n - 1):
+            G = np.einsum('ijklp,klmn->ijmnp', O.depolarizing_tensor(float(p2[d, q]), 2), O.gate_matrix('CNOT'))
+            a, b, _ = O.apply_2q(st.sites[q], st.sites[q + 1], G, True)
+            st.sites[q], st.sites[q + 1] = a, b
+            st.bond[q] = True
+            st.has_I[q + 1] = True
+        if O.check_connectivity(st):
+            O.bond_truncate(st, chi)
+            O.svd_kappa_left2right(st, kappa)
+    e = -sum(O.pauli_expect(st, [2, 2], [i, i + 1]) for i in range(n - 1))
+    e -= h * sum(O.pauli_expect(st, [0], [i]) for i in range(n))
+    return e, st

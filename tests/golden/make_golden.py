@@ -14,6 +14,14 @@ family8     -> family8_chi64_kappa8_oracle.npz: the 8-qubit, depth-6 member (cop
                benchmark's circuit family (RY, RZ + depolarizing p=0.002, CNOT + two-qubit
                depolarizing p=0.002) at chi=64, kappa=8: the 256x256 density matrix, the trace and
                every discarded weight.
+hea16       -> hea16_chi64_fd_oracle.npz: the 16-qubit hardware-efficient ansatz (8 layers of RY, RZ with
+               angles U[-pi, pi) from PCG64(303), CNOT chain, depolarizing p=0.005 after every gate,
+               each p its own parameter), chi=64, kappa=4: <H> of the transverse-field Ising model
+               (h=1) and central finite differences of it (h=1e-4 for angles, 1e-5 for noise
+               strengths) for 8 of the 632 parameters.
+batch16     -> random_brickwork16_oracle.npz: the first 16 circuits of the uneven batch
+               (workloads.random_brickwork_batch, seed 404: n 8..24, depth 4..16, chi 16..64,
+               kappa 2/4, depolarizing or amplitude damping): every <Z_i> and Tr rho~ per circuit.
 layer10     -> layer10_chi256_kappa8_oracle.npz: one circuit-layer (RY, RZ + depolarizing p=0.002
                on every qubit, CNOT + two-qubit depolarizing p=0.002 on pairs (1,2),(3,4),...,
                truncate with chi=256, kappa=8) of the 10-qubit member of the 32-qubit benchmark's
@@ -98,6 +106,52 @@ def family8():
     print('trace', np.trace(rho).real, f'total {time.time() - t:.1f} s')
 
 
+HEA16_FD = [('theta', (0, 3)), ('theta', (5, 8)), ('phi', (2, 10)), ('phi', (7, 1)), ('p1', (1, 4, 0)),
+            ('p1', (6, 12, 1)), ('p2', (0, 7)), ('p2', (7, 14))]
+
+
+def hea16_params():
+    """(theta, phi, p1, p2) of the 16-qubit ansatz: the angles of workloads.hea16_chi64 (the same
+    PCG64(303) draw), every noise strength 0.005."""
+    rng = np.random.Generator(np.random.PCG64(303))
+    angles = rng.uniform(-np.pi, np.pi, size=(8, 16, 2))
+    return angles[..., 0].copy(), angles[..., 1].copy(), np.full((8, 16, 2), 0.005), np.full((8, 15), 0.005)
+
+
+def hea16():
+    theta, phi, p1, p2 = hea16_params()
+    t = time.time()
+    e0, _ = bridge.run_hea_oracle(theta, phi, p1, p2, 64, 4)
+    print(f'E = {e0!r}: {time.time() - t:.1f} s', flush=True)
+    fd = []
+    for name, idx in HEA16_FD:
+        h = 1e-4 if name in ('theta', 'phi') else 1e-5
+        vals = []
+        for sgn in (1, -1):
+            args = dict(theta=theta.copy(), phi=phi.copy(), p1=p1.copy(), p2=p2.copy())
+            args[name][idx] += sgn * h
+            vals.append(bridge.run_hea_oracle(args['theta'], args['phi'], args['p1'], args['p2'], 64, 4)[0])
+        fd.append((vals[0] - vals[1]) / (2 * h))
+        print(name, idx, fd[-1], f'{time.time() - t:.1f} s', flush=True)
+    np.savez_compressed(os.path.join(HERE, 'hea16_chi64_fd_oracle.npz'), energy=e0, fd=np.array(fd),
+                        names=np.array([n for n, _ in HEA16_FD]),
+                        index=np.array([list(i) + [-1] * (3 - len(i)) for _, i in HEA16_FD]))
+
+
+def batch16():
+    ws = W.random_brickwork_batch()[:16]
+    t = time.time()
+    z, tr, off = [], [], [0]
+    for w in ws:
+        st = bridge.run_oracle_fused(w)
+        z.extend(O.pauli_expect(st, [2], [i]) for i in range(w.n))
+        tr.append(O.trace_rho(st))
+        off.append(len(z))
+        print(w.name, w.n, w.depth, w.chi, w.kappa, f'{time.time() - t:.1f} s', flush=True)
+    np.savez_compressed(os.path.join(HERE, 'random_brickwork16_oracle.npz'), z=np.array(z), trace=np.array(tr),
+                        offsets=np.array(off))
+
+
 def layer10_ops():
     """The ops of layer LAYER10_LAYER of the 10-qubit member of the benchmark family (copy 0)."""
     w = W.brickwork32_chi256_family(copies=1, n=10)[0]
@@ -126,4 +180,5 @@ def layer10():
 if __name__ == '__main__':
     which = sys.argv[1:] or ['brickwork8']
     for name in which:
-        {'brickwork8': brickwork8, 'brickwork20': brickwork20, 'family8': family8, 'layer10': layer10}[name]()
+        {'brickwork8': brickwork8, 'brickwork20': brickwork20, 'family8': family8, 'layer10': layer10,
+         'hea16': hea16, 'batch16': batch16}[name]()

@@ -140,10 +140,22 @@ def _layer10_inputs(cuda, copies):
     return chains, layer
 
 
-def test_layer10_chi256_kappa8_golden(cuda):
+@pytest.mark.parametrize('mode', [0, 2])
+def test_layer10_chi256_kappa8_golden(cuda, mode):
     """One chi=256, kappa=8 layer at 10 qubits from the seeded steady state: the fused layer
     (two chains: the fixture's and one from another seed) against the oracle fixture and
-    against the per-gate entry points."""
+    against the per-gate entry points; the site contractions with fp64 accumulation (mode 0) and
+    as 3M products with fp32 accumulation (mode 2)."""
+    from mpdo_amd import _lib
+    lib = _lib.load()
+    old = lib.mpdo_set_math_mode(mode)
+    try:
+        _layer10_check(cuda)
+    finally:
+        lib.mpdo_set_math_mode(old)
+
+
+def _layer10_check(cuda):
     from mpdo_amd import _lib, batched
     g = np.load(os.path.join(HERE, 'golden', 'layer10_chi256_kappa8_oracle.npz'))
     n, B = 10, 2

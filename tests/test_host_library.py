@@ -68,3 +68,21 @@ def test_fused_layer_rejects_thin_pairs():
     dims = [(1, 1, 2), (2, 1, 4), (4, 1, 4), (4, 1, 2), (2, 1, 1)]
     out, why = _lib.layer_dims(dims, 1, 1, 1, 4, 0)
     assert out is None and why
+
+
+def test_layer_workspace_query_matches_dry_run():
+    """mpdo_layer_workspace_bytes (the size a caller binds for mpdo_layer_batched) is the dry run's
+    workspace at the benchmark shape and at a small one; 0 with the reason for an unplannable layer."""
+    import ctypes
+    from mpdo_amd import _lib, batched
+    lib = _lib.load()
+    for n, chi, kappa, B in [(32, 256, 8, 32), (8, 16, 2, 3)]:
+        dims = batched.steady_dims(n, chi, kappa, 4, 16, 0)
+        ins = (_lib.Site * n)()
+        for i, (l, a, r) in enumerate(dims):
+            ins[i] = _lib.Site(None, l, a, r)
+        got = lib.mpdo_layer_workspace_bytes(B, n, ins, 4, 0, 16, chi, kappa)
+        assert got == _lib.layer_dry_run(B, dims, 4, 0, 16, chi, kappa) > 0
+    bad = (_lib.Site * 2)(_lib.Site(None, 1, 1, 2), _lib.Site(None, 3, 1, 1))
+    assert lib.mpdo_layer_workspace_bytes(1, 2, bad, 4, 0, 16, 4, 2) == 0
+    assert b'mismatch' in lib.mpdo_last_error()

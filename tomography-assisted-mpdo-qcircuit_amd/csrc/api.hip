@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <atomic>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -55,24 +56,38 @@ mpdo_status fail(mpdo_status s, const char* fmt, ...) {
     if (s_ != MPDO_OK) return s_;              \
   } while (0)
 
-// ---------------------------------------------------------------- workspace (per device)
+// ---------------------------------------------------------------- workspace (per device and stream)
+// Each call carves its scratch from one device buffer. A caller may bind its own buffer to a stream
+// (mpdo_bind_workspace; the Python host binds one from torch's caching allocator): calls on that
+// stream then use it and never allocate, and a call that needs more returns MPDO_ENOMEM before
+// launching anything, with the size it needs in mpdo_workspace_needed(). Without a binding the
+// library keeps a grow-only buffer of its own for the (device, stream) pair. Calls on different
+// streams use different buffers, so they may run concurrently; calls on one stream are ordered.
 struct DevWS {
   void* p = nullptr;
   size_t cap = 0;
+  bool bound = false;   // the caller's buffer (never freed here)
 };
 std::mutex g_mu;
-DevWS g_ws[64];
+std::map<std::pair<int, hipStream_t>, DevWS> g_ws;
+thread_local size_t t_ws_needed = 0;
 
 mpdo_status ws_reserve(size_t bytes, hipStream_t s, char** base) {
   int dev = 0;
   CK(hipGetDevice(&dev));
-  if (dev < 0 || dev >= 64) return fail(MPDO_EINVAL, "device index %d", dev);
   std::lock_guard<std::mutex> lk(g_mu);
-  DevWS& w = g_ws[dev];
+  DevWS& w = g_ws[{dev, s}];
+  if (w.bound) {
+    if (w.cap < bytes) {
+      t_ws_needed = bytes;
+      return fail(MPDO_ENOMEM, "the workspace bound to this stream holds %zu bytes, the call needs %zu", w.cap, bytes);
+    }
+    *base = (char*)w.p;
+    return MPDO_OK;
+  }
   if (w.cap < bytes) {
     if (w.p) {
       CK(hipStreamSynchronize(s));
-      CK(hipDeviceSynchronize());
       CK(hipFree(w.p));
       w.p = nullptr;
       w.cap = 0;
@@ -81,6 +96,7 @@ mpdo_status ws_reserve(size_t bytes, hipStream_t s, char** base) {
     if (hipMalloc(&w.p, nb) != hipSuccess) {
       (void)hipGetLastError();
       w.p = nullptr;
+      t_ws_needed = bytes;
       return fail(MPDO_ENOMEM, "device workspace of %zu bytes", nb);
     }
     w.cap = nb;
@@ -368,15 +384,44 @@ const char* mpdo_last_error(void) { return g_err.c_str(); }
 
 void mpdo_release_workspace(void) {
   int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return;
+  if (hipGetDevice(&dev) != hipSuccess) return;
   std::lock_guard<std::mutex> lk(g_mu);
-  if (g_ws[dev].p) {
-    (void)hipDeviceSynchronize();
-    (void)hipFree(g_ws[dev].p);
+  for (auto it = g_ws.begin(); it != g_ws.end();) {
+    if (it->first.first == dev && !it->second.bound) {
+      if (it->second.p) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(it->second.p);
+      }
+      it = g_ws.erase(it);
+    } else {
+      ++it;
+    }
   }
-  g_ws[dev].p = nullptr;
-  g_ws[dev].cap = 0;
 }
+
+mpdo_status mpdo_bind_workspace(void* stream, void* ptr, size_t bytes) {
+  int dev = 0;
+  CK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto key = std::make_pair(dev, (hipStream_t)stream);
+  auto it = g_ws.find(key);
+  if (it != g_ws.end() && !it->second.bound && it->second.p) {
+    CK(hipStreamSynchronize((hipStream_t)stream));
+    CK(hipFree(it->second.p));
+  }
+  if (!ptr) {
+    if (it != g_ws.end()) g_ws.erase(it);
+    return MPDO_OK;
+  }
+  if (((uintptr_t)ptr & 255) != 0) return fail(MPDO_EINVAL, "the workspace must be 256-byte aligned");
+  DevWS& w = g_ws[key];
+  w.p = ptr;
+  w.cap = bytes;
+  w.bound = true;
+  return MPDO_OK;
+}
+
+size_t mpdo_workspace_needed(void) { return t_ws_needed; }
 
 // ------------------------------------------------------------------------------ 1q gates
 mpdo_status mpdo_apply_1q(int nsites, const mpdo_site* in, const mpdo_site* out, const float* kraus,

@@ -1661,6 +1661,31 @@ int mpdo_debug_layer_kept(int* out, int cap) {
   return n;
 }
 
+size_t mpdo_layer_workspace_bytes(int batch, int n, const mpdo_site* in, int K1, int parity, int K2, int chi,
+                                  int kappa) {
+  if (batch <= 0 || n <= 0 || !in) {
+    layer_fail(MPDO_EINVAL, "null argument");
+    return 0;
+  }
+  LayerPlan P = plan_layer(n, in, K1, parity, K2, chi, kappa);
+  if (!P.ok) {
+    layer_fail(MPDO_EINVAL, P.why.c_str());
+    return 0;
+  }
+  // disjoint stand-in addresses for the chains (the plan needs their extents, not their data)
+  std::vector<mpdo_site> fin(n), fout(n);
+  uintptr_t addr = uintptr_t(1) << 40;
+  for (int i = 0; i < n; ++i) {
+    fin[i] = mpdo_site{(void*)addr, P.in[i].l, P.in[i].a, P.in[i].r};
+    addr += ((sizeof(float2) * (size_t)batch * P.in[i].l * 2 * P.in[i].a * P.in[i].r) | 4095) + 1;
+    fout[i] = mpdo_site{(void*)addr, P.out[i].l, P.out[i].a, P.out[i].r};
+    addr += ((sizeof(float2) * (size_t)batch * P.out[i].l * 2 * P.out[i].a * P.out[i].r) | 4095) + 1;
+  }
+  t_ws_bytes = 0;
+  if (run_layer(batch, P, fin.data(), nullptr, nullptr, fout.data(), nullptr, nullptr, true) != MPDO_OK) return 0;
+  return t_ws_bytes;
+}
+
 mpdo_status mpdo_debug_layer_dry_run(int batch, int n, const mpdo_site* in, int K1, int parity, int K2, int chi,
                                      int kappa, const mpdo_site* out, unsigned long long* workspace_bytes) {
   if (batch <= 0 || n <= 0 || !in || !out) return layer_fail(MPDO_EINVAL, "null argument");

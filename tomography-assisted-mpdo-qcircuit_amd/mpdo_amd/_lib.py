@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import threading
 from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
@@ -24,6 +25,7 @@ class Site(ctypes.Structure):
 
 
 _lib = None
+_load_lock = threading.Lock()
 
 
 def load() -> ctypes.CDLL:
@@ -31,6 +33,14 @@ def load() -> ctypes.CDLL:
     global _lib
     if _lib is not None:
         return _lib
+    with _load_lock:
+        if _lib is not None:
+            return _lib
+        return _load()
+
+
+def _load() -> ctypes.CDLL:
+    global _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f'{LIB_PATH} is missing: build it with `python -m mpdo_amd._build` '
                           '(hipcc --offload-arch=gfx950); there is no CPU fallback')
@@ -39,6 +49,9 @@ def load() -> ctypes.CDLL:
     SP = P(Site)
     sig = {
         'mpdo_version': (I, []),
+        'mpdo_bind_workspace': (I, [VP, VP, ctypes.c_size_t]),
+        'mpdo_workspace_needed': (ctypes.c_size_t, []),
+        'mpdo_layer_workspace_bytes': (ctypes.c_size_t, [I, I, SP, I, I, I, I, I]),
         'mpdo_last_error': (ctypes.c_char_p, []),
         'mpdo_release_workspace': (None, []),
         'mpdo_set_math_mode': (I, [I]),
@@ -115,6 +128,52 @@ def _stream(device: torch.device) -> ctypes.c_void_p:
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
 
 
+# Workspace: one buffer from torch's caching allocator per (device, stream), bound to the stream
+# (mpdo_bind_workspace) so that the library allocates nothing on the compute path and calls on
+# different streams use different buffers.
+_WS_MIN = 64 << 20
+_ws_bufs = {}
+_ws_lock = threading.Lock()
+
+
+def _bind_workspace(device: torch.device, nbytes: int) -> None:
+    handle = torch.cuda.current_stream(device).cuda_stream
+    key = (device.index if device.index is not None else torch.cuda.current_device(), handle)
+    with _ws_lock:
+        buf = _ws_bufs.get(key)
+        if buf is not None and buf.numel() >= nbytes:
+            return
+        size = max(int(nbytes * 1.125) + (1 << 20), _WS_MIN)
+        buf = torch.empty(size, dtype=torch.uint8, device=device)
+        check(load().mpdo_bind_workspace(ctypes.c_void_p(handle), ctypes.c_void_p(buf.data_ptr()), size),
+              'mpdo_bind_workspace')
+        _ws_bufs[key] = buf
+
+
+def _run(what: str, device: torch.device, call) -> None:
+    """Run one library call on the current stream of `device` with the stream's workspace bound;
+    a call that needs a larger workspace (MPDO_ENOMEM before any launch, with the size in
+    mpdo_workspace_needed) runs again with one that large."""
+    lib = load()
+    _bind_workspace(device, 0)
+    st = call()
+    if st == MPDO_ENOMEM:
+        need = int(lib.mpdo_workspace_needed())
+        if need > 0:
+            _bind_workspace(device, need)
+            st = call()
+    check(st, what)
+
+
+def release_workspaces() -> None:
+    """Unbind and drop every workspace buffer this module holds."""
+    lib = load()
+    for (dev, handle), buf in list(_ws_bufs.items()):
+        with torch.cuda.device(dev):
+            lib.mpdo_bind_workspace(ctypes.c_void_p(handle), None, 0)
+    _ws_bufs.clear()
+
+
 def _require_device(t: torch.Tensor) -> None:
     if t.device.type != 'cuda':
         raise RuntimeError('the MPDO kernels run on a ROCm GPU (gfx950); got a tensor on '
@@ -177,9 +236,9 @@ def apply_1q(sites: Sequence[torch.Tensor], kraus: Sequence[torch.Tensor]) -> Li
             for t, K in zip(sites, Ks)]
     flat = torch.cat(mats).contiguous()
     Karr = (ctypes.c_int * n)(*Ks)
-    st = lib.mpdo_apply_1q(n, _sites(sites), _sites(outs),
-                           ctypes.cast(flat.data_ptr(), ctypes.POINTER(ctypes.c_float)), Karr, _stream(dev))
-    check(st, 'mpdo_apply_1q')
+    _run('mpdo_apply_1q', dev, lambda: lib.mpdo_apply_1q(n, _sites(sites), _sites(outs),
+                                                        ctypes.cast(flat.data_ptr(), ctypes.POINTER(ctypes.c_float)),
+                                                        Karr, _stream(dev)))
     return outs
 
 
@@ -201,10 +260,10 @@ def apply_2q(tmin: torch.Tensor, tmax: torch.Tensor, gate: torch.Tensor, eps_abs
     bmax = _empty(cap * 2 * tmax.shape[2] * K * tmax.shape[3], dev)
     om, oM = Site(bmin.data_ptr(), 0, 0, 0), Site(bmax.data_ptr(), 0, 0, 0)
     disc = ctypes.c_double(0.0)
-    st = lib.mpdo_apply_2q(ctypes.byref(a), ctypes.byref(b),
-                           ctypes.cast(flat.data_ptr(), ctypes.POINTER(ctypes.c_float)), K, float(eps_abs),
-                           ctypes.byref(om), ctypes.byref(oM), ctypes.byref(disc), _stream(dev))
-    check(st, 'mpdo_apply_2q')
+    _run('mpdo_apply_2q', dev, lambda: lib.mpdo_apply_2q(ctypes.byref(a), ctypes.byref(b),
+                                                        ctypes.cast(flat.data_ptr(), ctypes.POINTER(ctypes.c_float)),
+                                                        K, float(eps_abs), ctypes.byref(om), ctypes.byref(oM),
+                                                        ctypes.byref(disc), _stream(dev)))
     return _view(bmin, om).clone(), _view(bmax, oM).clone(), disc.value
 
 
@@ -219,7 +278,7 @@ def qr_left2right(sites: Sequence[torch.Tensor]) -> List[torch.Tensor]:
     outs = (Site * n)()
     for i in range(n):
         outs[i] = Site(bufs[i].data_ptr(), dims[i].l, dims[i].a, dims[i].r)
-    check(lib.mpdo_qr_left2right(n, ins, outs, _stream(dev)), 'mpdo_qr_left2right')
+    _run('mpdo_qr_left2right', dev, lambda: lib.mpdo_qr_left2right(n, ins, outs, _stream(dev)))
     return [_view(bufs[i], outs[i]) for i in range(n)]
 
 
@@ -237,9 +296,9 @@ def svd_sweep(sites: Sequence[torch.Tensor], direction: int, chi: Optional[int],
     for i in range(n):
         outs[i] = Site(bufs[i].data_ptr(), 0, 0, 0)
     disc = (ctypes.c_double * max(n - 1, 1))()
-    st = lib.mpdo_svd_sweep(n, ins, outs, direction, msv, float(err) if err is not None else -1.0,
-                            1 if left_canonical else 0, disc, _stream(dev))
-    check(st, 'mpdo_svd_sweep')
+    _run('mpdo_svd_sweep', dev, lambda: lib.mpdo_svd_sweep(n, ins, outs, direction, msv,
+                                                          float(err) if err is not None else -1.0,
+                                                          1 if left_canonical else 0, disc, _stream(dev)))
     res = [_view(bufs[i], outs[i]) for i in range(n)]
     return res, [disc[i] for i in range(n - 1)]
 
@@ -259,9 +318,9 @@ def svd_kappa(sites: Sequence[torch.Tensor], kappa: Optional[int], err: Optional
         bufs.append(_empty(t.shape[0] * 2 * cap * t.shape[3], dev))
         outs[i] = Site(bufs[i].data_ptr(), 0, 0, 0)
     disc = (ctypes.c_double * n)()
-    st = lib.mpdo_svd_kappa(n, ins, outs, int(kappa) if kappa is not None else 0,
-                            float(err) if err is not None else -1.0, disc, _stream(dev))
-    check(st, 'mpdo_svd_kappa')
+    _run('mpdo_svd_kappa', dev, lambda: lib.mpdo_svd_kappa(n, ins, outs, int(kappa) if kappa is not None else 0,
+                                                          float(err) if err is not None else -1.0, disc,
+                                                          _stream(dev)))
     return [_view(bufs[i], outs[i]) for i in range(n)], [disc[i] for i in range(n)]
 
 
@@ -276,7 +335,9 @@ def pauli_expect(sites: Sequence[torch.Tensor], codes: Sequence[Sequence[int]]) 
         for i, v in enumerate(c):
             flat[t * n + i] = int(v)
     out = (ctypes.c_double * ns)()
-    check(lib.mpdo_pauli_expect(n, _sites(sites), ns, flat, out, _stream(sites[0].device)), 'mpdo_pauli_expect')
+    dev = sites[0].device
+    arr = _sites(sites)
+    _run('mpdo_pauli_expect', dev, lambda: lib.mpdo_pauli_expect(n, arr, ns, flat, out, _stream(dev)))
     return [out[i] for i in range(ns)]
 
 
@@ -287,8 +348,8 @@ def reduced_dm(sites: Sequence[torch.Tensor], keep: Sequence[bool]) -> torch.Ten
     dev = sites[0].device
     out = torch.empty(2 ** k, 2 ** k, dtype=torch.complex64, device=dev)
     kf = (ctypes.c_int * n)(*[1 if x else 0 for x in keep])
-    check(lib.mpdo_reduced_dm(n, _sites(sites), kf, ctypes.c_void_p(out.data_ptr()), _stream(dev)),
-          'mpdo_reduced_dm')
+    arr = _sites(sites)
+    _run('mpdo_reduced_dm', dev, lambda: lib.mpdo_reduced_dm(n, arr, kf, ctypes.c_void_p(out.data_ptr()), _stream(dev)))
     return out
 
 
@@ -299,12 +360,14 @@ def trace_rho_rho(a: Sequence[torch.Tensor], b: Sequence[torch.Tensor], keep: Op
         raise ValueError('Density matrices must have the same number of nodes.')
     out = ctypes.c_double(0.0)
     if keep is None:
-        check(lib.mpdo_trace_rho_rho(len(a), _sites(a), _sites(b), ctypes.byref(out), _stream(a[0].device)),
-              'mpdo_trace_rho_rho')
+        sa, sb = _sites(a), _sites(b)
+        _run('mpdo_trace_rho_rho', a[0].device,
+             lambda: lib.mpdo_trace_rho_rho(len(a), sa, sb, ctypes.byref(out), _stream(a[0].device)))
     else:
         kf = (ctypes.c_int * len(a))(*[1 if k else 0 for k in keep])
-        check(lib.mpdo_trace_rho_rho_masked(len(a), _sites(a), _sites(b), kf, ctypes.byref(out),
-                                            _stream(a[0].device)), 'mpdo_trace_rho_rho_masked')
+        sa, sb = _sites(a), _sites(b)
+        _run('mpdo_trace_rho_rho_masked', a[0].device,
+             lambda: lib.mpdo_trace_rho_rho_masked(len(a), sa, sb, kf, ctypes.byref(out), _stream(a[0].device)))
     return out.value
 
 
@@ -316,7 +379,7 @@ def env_right(sites: Sequence[torch.Tensor]) -> List[torch.Tensor]:
     dev = sites[0].device
     total = lib.mpdo_env_right_elems(n, arr)
     buf = torch.empty(total, dtype=torch.complex128, device=dev)
-    check(lib.mpdo_env_right(n, arr, ctypes.c_void_p(buf.data_ptr()), _stream(dev)), 'mpdo_env_right')
+    _run('mpdo_env_right', dev, lambda: lib.mpdo_env_right(n, arr, ctypes.c_void_p(buf.data_ptr()), _stream(dev)))
     out, off = [], 0
     for t in sites:
         m = t.shape[0]
@@ -347,12 +410,14 @@ def sample_step(site: torch.Tensor, prev: torch.Tensor, sel: Optional[Sequence[i
         out = torch.empty(G, r, r, dtype=torch.complex128, device=dev)
     probs = np.zeros((G, 2), dtype=np.float64)
     csel = None if sel is None else (ctypes.c_int * G)(*[int(x) for x in sel])
-    check(lib.mpdo_sample_step(ctypes.byref(site_desc(site)), G, ctypes.c_void_p(prev.data_ptr()), prev.shape[0],
-                               csel, 1 if sampled else 0,
-                               ctypes.c_void_p(R_next.contiguous().data_ptr()) if sampled else None,
-                               ctypes.c_void_p(out.data_ptr()),
-                               probs.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if sampled else None,
-                               _stream(dev)), 'mpdo_sample_step')
+    sd = site_desc(site)
+    Rn = R_next.contiguous() if sampled else None
+    _run('mpdo_sample_step', dev,
+         lambda: lib.mpdo_sample_step(ctypes.byref(sd), G, ctypes.c_void_p(prev.data_ptr()), prev.shape[0], csel,
+                                      1 if sampled else 0, ctypes.c_void_p(Rn.data_ptr()) if sampled else None,
+                                      ctypes.c_void_p(out.data_ptr()),
+                                      probs.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if sampled else None,
+                                      _stream(dev)))
     return out, (probs if sampled else None)
 
 
@@ -399,6 +464,8 @@ def layer_batched(sites: Sequence[torch.Tensor], kraus1: torch.Tensor, parity: i
     k1 = kraus1.detach().to(torch.complex64).cpu().contiguous().view(torch.float32).reshape(-1).contiguous()
     g2 = gates2.detach().to(torch.complex64).cpu().contiguous().view(torch.float32).reshape(-1).contiguous()
     disc = np.zeros((B, 3 * n), dtype=np.float64)
+    need = int(lib.mpdo_layer_workspace_bytes(B, n, ins, K1, parity, K2, int(chi or 0), int(kappa or 0)))
+    _bind_workspace(dev, need)
     st = lib.mpdo_layer_batched(B, n, ins, ctypes.cast(k1.data_ptr(), ctypes.POINTER(ctypes.c_float)), K1, parity,
                                 ctypes.cast(g2.data_ptr(), ctypes.POINTER(ctypes.c_float)), K2, int(chi or 0),
                                 int(kappa or 0), outs, disc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
@@ -418,8 +485,9 @@ def eigh(A: torch.Tensor):
     work = A.contiguous().clone()
     w = torch.empty(b, n, dtype=torch.float64, device=A.device)
     V = torch.empty(b, n, n, dtype=torch.complex128, device=A.device)
-    check(lib.mpdo_eigh(n, b, ctypes.c_void_p(work.data_ptr()), ctypes.cast(w.data_ptr(), ctypes.POINTER(ctypes.c_double)),
-                        ctypes.c_void_p(V.data_ptr()), _stream(A.device)), 'mpdo_eigh')
+    _run('mpdo_eigh', A.device, lambda: lib.mpdo_eigh(n, b, ctypes.c_void_p(work.data_ptr()),
+                                                     ctypes.cast(w.data_ptr(), ctypes.POINTER(ctypes.c_double)),
+                                                     ctypes.c_void_p(V.data_ptr()), _stream(A.device)))
     return w, V
 
 
@@ -427,8 +495,9 @@ def contract_state(sites: Sequence[torch.Tensor]) -> torch.Tensor:
     lib = load()
     n = len(sites)
     out = torch.empty(2 ** n, dtype=torch.complex64, device=sites[0].device)
-    check(lib.mpdo_contract_state(n, _sites(sites), ctypes.c_void_p(out.data_ptr()), _stream(sites[0].device)),
-          'mpdo_contract_state')
+    arr = _sites(sites)
+    _run('mpdo_contract_state', sites[0].device,
+         lambda: lib.mpdo_contract_state(n, arr, ctypes.c_void_p(out.data_ptr()), _stream(sites[0].device)))
     return out
 
 
@@ -480,10 +549,9 @@ def debug_eigh(A: torch.Tensor, gram: bool = False):
     V = torch.empty(b, n, n, dtype=torch.complex128, device=A.device)
     sw = ctypes.c_int(0)
     fn = lib.mpdo_debug_eigh_gram if gram else lib.mpdo_debug_eigh
-    check(fn(n, b, ctypes.c_void_p(A.data_ptr()),
-                              ctypes.cast(w.data_ptr(), ctypes.POINTER(ctypes.c_double)),
-                              ctypes.c_void_p(V.data_ptr()), ctypes.byref(sw), _stream(A.device)),
-          'mpdo_debug_eigh')
+    _run('mpdo_debug_eigh', A.device,
+         lambda: fn(n, b, ctypes.c_void_p(A.data_ptr()), ctypes.cast(w.data_ptr(), ctypes.POINTER(ctypes.c_double)),
+                    ctypes.c_void_p(V.data_ptr()), ctypes.byref(sw), _stream(A.device)))
     return w, V, sw.value
 
 

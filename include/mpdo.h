@@ -137,6 +137,20 @@ mpdo_status mpdo_pauli_expect(int n, const mpdo_site* sites, int nstrings, const
                               void* stream);
 
 /*
+ * mpdo_pauli_expect without any device allocation. No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same arguments, results and synchronisation as
+ * mpdo_pauli_expect; the strings' environments live in a fixed 32 MiB device array of the library
+ * (one per GPU, held until the process ends) instead of the grow-only workspace. Returns
+ * MPDO_ENOMEM, before launching anything, when one string needs more than that at these site
+ * shapes (two l*2*a*r complex64 intermediates of the largest site plus two bond^2 environments:
+ * n = 32, chi = 256, a = 8 fits). Every call on one GPU shares that array: calls that overlap in
+ * time on one GPU (from several host threads or streams) race and must not be made.
+ */
+mpdo_status mpdo_pauli_expect_noalloc(int n, const mpdo_site* sites, int nstrings, const int* codes, double* out,
+                                      void* stream);
+
+/*
  * Reduced density matrix of the qubits with keep[i] != 0 (others traced), rows = ket bits of
  * the kept qubits in increasing index order, lowest index most significant, unnormalised.
  * Replaces Circuit.py:226-262 (_create_dmNodes/_contract_dm) + dmOperations.py:18-37.

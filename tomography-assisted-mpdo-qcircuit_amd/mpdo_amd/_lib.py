@@ -64,6 +64,7 @@ def _load() -> ctypes.CDLL:
         'mpdo_svd_sweep': (I, [I, SP, SP, I, I, D, I, P(D), VP]),
         'mpdo_svd_kappa': (I, [I, SP, SP, I, D, P(D), VP]),
         'mpdo_pauli_expect': (I, [I, SP, I, P(I), P(D), VP]),
+        'mpdo_pauli_expect_noalloc': (I, [I, SP, I, P(I), P(D), VP]),
         'mpdo_reduced_dm': (I, [I, SP, P(I), VP, VP]),
         'mpdo_trace_rho_rho': (I, [I, SP, SP, P(D), VP]),
         'mpdo_trace_rho_rho_masked': (I, [I, SP, SP, P(I), P(D), VP]),
@@ -324,7 +325,9 @@ def svd_kappa(sites: Sequence[torch.Tensor], kappa: Optional[int], err: Optional
     return [_view(bufs[i], outs[i]) for i in range(n)], [disc[i] for i in range(n)]
 
 
-def pauli_expect(sites: Sequence[torch.Tensor], codes: Sequence[Sequence[int]]) -> List[float]:
+def pauli_expect(sites: Sequence[torch.Tensor], codes: Sequence[Sequence[int]], noalloc: bool = False) -> List[float]:
+    """Re Tr(O rho~) per code string (dmOperations.py:174-201). noalloc: through
+    mpdo_pauli_expect_noalloc (the library's fixed device scratch; no workspace)."""
     lib = load()
     n = len(sites)
     ns = len(codes)
@@ -337,7 +340,10 @@ def pauli_expect(sites: Sequence[torch.Tensor], codes: Sequence[Sequence[int]]) 
     out = (ctypes.c_double * ns)()
     dev = sites[0].device
     arr = _sites(sites)
-    _run('mpdo_pauli_expect', dev, lambda: lib.mpdo_pauli_expect(n, arr, ns, flat, out, _stream(dev)))
+    if noalloc:
+        check(lib.mpdo_pauli_expect_noalloc(n, arr, ns, flat, out, _stream(dev)), 'mpdo_pauli_expect_noalloc')
+    else:
+        _run('mpdo_pauli_expect', dev, lambda: lib.mpdo_pauli_expect(n, arr, ns, flat, out, _stream(dev)))
     return [out[i] for i in range(ns)]
 
 

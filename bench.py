@@ -171,6 +171,57 @@ def cpu_baseline(Ti, Tj, k1i, k1j, g2, threads):
                        'rate (the 15 chi and QR steps between pairs are not timed)')}
 
 
+def _uneven_batch(count, streams, dev):
+    """The first `count` circuits of the 256-circuit uneven batch (n 8..24, depth 4..16, chi 16..128,
+    kappa 2/4, depolarizing or amplitude damping) from |0..0> through scheduler.evolve_many on
+    `streams` streams, and through one stream: aggregate circuit-layers/s."""
+    import torch
+    import mpdo_amd as M
+    from mpdo_amd import scheduler
+    from mpdo_amd import workloads as W
+    ws = W.random_brickwork_batch()[:count]
+    out = {'circuits': count, 'workload': 'workloads.random_brickwork_batch()[:%d] (seed 404)' % count}
+    for s in (1, streams):
+        cs = []
+        for w in ws:
+            c = M.TensorCircuit(device=str(dev), **w.circuit_kwargs())
+            W.apply_ops(c, w.ops)
+            cs.append(c)
+        states = [M.Tools.create_ket0Series(w.n, device=str(dev)) for w in ws]
+        torch.cuda.synchronize()
+        st = {}
+        scheduler.evolve_many(cs, states, streams=s, stats=st)
+        out[f'streams_{s}'] = {'circuit_layers_per_s': round(st['circuit_layers'] / st['seconds'], 3),
+                               'seconds': round(st['seconds'], 3), 'circuit_layers': st['circuit_layers'],
+                               'fused_layers': st['fused_layers'], 'per_gate_layers': st['per_gate_layers']}
+    return out
+
+
+def _differentiation(dev):
+    """One forward + reverse sweep of the 16-qubit, 8-layer ansatz (chi=64, kappa=4, 256 angles and
+    376 per-gate depolarizing strengths, per-layer checkpointing): seconds and peak device memory."""
+    import numpy as np
+    import torch
+    from mpdo_amd import autodiff as AD
+    rng = np.random.Generator(np.random.PCG64(303))
+    angles = rng.uniform(-np.pi, np.pi, size=(8, 16, 2))
+    m = AD.HEA(16, 8, 64, 4, angles[..., 0], angles[..., 1], np.full((8, 16, 2), 0.005), np.full((8, 15), 0.005),
+               device=str(dev))
+    torch.cuda.reset_peak_memory_stats(dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    E = AD.tfim_energy(m())
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    E.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    return {'workload': '16-qubit 8-layer RY/RZ + CNOT-chain ansatz, per-gate depolarizing p=0.005, chi=64, kappa=4, '
+                        'loss <H> (TFIM h=1)', 'parameters': 632, 'energy': E.item(),
+            'forward_s': round(t1 - t0, 3), 'backward_s': round(t2 - t1, 3),
+            'peak_device_gib': round(torch.cuda.max_memory_allocated(dev) / 2 ** 30, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split('\n\n')[0])
     ap.add_argument('--gpus', type=int, default=1)
@@ -184,6 +235,12 @@ def main():
                     help='contractions whose results are complex64 site data: fp64 MFMA accumulation, or three real '
                          'products per complex product accumulated in fp32 (3M; Gram matrices, QR/LQ factors and '
                          'eigensolves stay fp64)')
+    ap.add_argument('--uneven', type=int, default=64,
+                    help='circuits of the 256-circuit uneven batch (workloads.random_brickwork_batch) to time through '
+                         'scheduler.evolve_many after the main measurement (0: skip)')
+    ap.add_argument('--uneven-streams', type=int, default=8)
+    ap.add_argument('--no-differentiation', action='store_true',
+                    help='skip timing one forward + backward of the 16-qubit differentiable ansatz')
     ap.add_argument('--fused-only', action='store_true',
                     help='time the fused steps only (no profiled pass, plain path or accuracy check): the '
                          'command of the rocprofv3 counter passes')
@@ -367,6 +424,14 @@ def main():
             _log(f'cpu baseline (oracle sample, {threads} threads)')
             cpu = cpu_baseline(*cpu_in, threads)
     lib.mpdo_set_math_mode(prev_mode)
+    uneven = None
+    if not args.fused_only and args.uneven > 0:
+        uneven = _uneven_batch(args.uneven, args.uneven_streams, dev)
+        _log(f'uneven batch: {uneven}')
+    diff = None
+    if not args.fused_only and not args.no_differentiation:
+        diff = _differentiation(dev)
+        _log(f'differentiation: {diff}')
     line = {'metric': METRIC, 'value': round(value, 6), 'unit': 'circuit-layers/s', 'n_gpus': 1,
             'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': round(T / args.steps * 1e3, 3),
             'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
@@ -380,7 +445,7 @@ def main():
                        'parallelism': 'none (one GPU)'},
             'roofline': roofline, 'roofline_block_jacobi': others, 'plain': plain,
             'batched_vs_plain': plain['batched_vs_plain'] if plain else None, 'accuracy': accuracy,
-            'cpu_baseline': cpu}
+            'uneven_batch': uneven, 'differentiation': diff, 'cpu_baseline': cpu}
     print(json.dumps(line), flush=True)
 
 

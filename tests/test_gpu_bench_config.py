@@ -172,9 +172,11 @@ def _layer10_check(cuda):
     assert np.abs(vals[n:-1] - zz).max() <= 1e-5 * tr
     kinds = [str(k) for k in g['disc_kind']]
     ref_chi = [w for k, w in zip(kinds, g['disc_weight']) if k == 'chi']
-    assert np.allclose([disc[0, i] for i in reversed(range(n - 1))], ref_chi, rtol=1e-4, atol=1e-10)
+    # the chi weights of this layer are 1e-5 .. 1e-8 of Tr: complex64 site data resolves a weight w
+    # to about 2 sqrt(w) x 6e-8 per discarded value, so below ~1e-4 Tr the bar is 1e-8 Tr absolute
+    assert np.allclose([disc[0, i] for i in reversed(range(n - 1))], ref_chi, rtol=1e-4, atol=1e-8 * tr)
     ref_kap = sum(w for k, w in zip(kinds, g['disc_weight']) if k == 'kappa')
-    assert abs(disc[0, n:2 * n].sum() - ref_kap) <= 1e-4 * ref_kap + 1e-10
+    assert abs(disc[0, n:2 * n].sum() - ref_kap) <= 1e-4 * ref_kap + 1e-8 * tr
     ref_gate = {int(i): w for k, i, w in zip(kinds, g['disc_index'], g['disc_weight']) if k == 'gate'}
     for p, w in ref_gate.items():
         assert abs(disc[0, 2 * n + p] - w) <= 1e-3 * abs(w) + 1e-12, (p, disc[0, 2 * n + p], w)

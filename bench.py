@@ -23,9 +23,13 @@ After the timed steps, on the input of the first timed layer (kept aside):
   * accuracy: copy 0 of the batched layer against the per-gate path in fp64 accumulation on the
     same input (all 496 <Z_i Z_j>, 32 <Z_i>, Tr rho~, chi/kappa discarded weights), and one
     chi=256, kappa=8 layer at 10 qubits against its complex128 oracle fixture.
-Then the CPU oracle on a bounded sample of the same workload.
+Then the uneven batch (64 circuits of mixed size through the multi-stream scheduler, one stream
+and eight), one forward + reverse sweep of the 16-qubit differentiable ansatz, and the CPU oracle
+on a bounded sample of the same workload.
 
-Prints one JSON line.
+Prints one JSON line. A ~1 us spin kernel (torch.cuda._sleep) brackets the timed steps and the
+profiled pass, so that `bench.py --kernel-trace-summary TRACE.csv` can cut a rocprofv3 kernel trace
+of this run into those two windows.
 """
 from __future__ import annotations
 
@@ -54,6 +58,76 @@ CALIB_BYTES = 1 << 30
 
 def _log(msg):
     print(f'# {msg}', file=sys.stderr, flush=True)
+
+
+def _trace_marker():
+    """A spin kernel of about a microsecond: brackets a window of a rocprofv3 kernel trace."""
+    import torch
+    torch.cuda._sleep(1000)
+
+
+MARKER = 'spin_kernel'
+
+
+def kernel_trace_summary(path, top=24):
+    """Markdown summary of the two marked windows (timed steps, profiled pass) of a rocprofv3
+    kernel trace (CSV: Kernel_Name, Start_Timestamp, End_Timestamp in ns) of a default run."""
+    import csv
+    rows = []
+    with open(path, newline='') as f:
+        rd = csv.DictReader(f)
+        cols = {c.lower(): c for c in rd.fieldnames}
+        kn, ks, ke = cols['kernel_name'], cols['start_timestamp'], cols['end_timestamp']
+        for r in rd:
+            rows.append((int(r[ks]), int(r[ke]), r[kn]))
+    rows.sort()
+    marks = [i for i, r in enumerate(rows) if MARKER in r[2]]
+    if len(marks) != 4:
+        raise SystemExit(f'{path}: expected 4 {MARKER} launches (two windows), found {len(marks)}')
+    out = [f'Kernel trace `{os.path.basename(path)}`: {len(rows)} kernels; windows between the '
+           f'`{MARKER}` markers.', '']
+
+    def short(name):
+        name = name.replace('void ', '').replace('(anonymous namespace)::', '')
+        name = name.replace('HIP_vector_type<double, 2u>', 'double2').replace('HIP_vector_type<float, 2u>', 'float2')
+        depth, cut = 0, len(name)
+        for i, ch in enumerate(name):   # drop the argument list, keep the template arguments
+            depth += ch == '<'
+            depth -= ch == '>'
+            if ch == '(' and depth == 0:
+                cut = i
+                break
+        return name[:cut]
+
+    for title, (a, b) in (('timed steps (no timing hooks)', (marks[0], marks[1])),
+                          ('profiled pass (timing hooks on)', (marks[2], marks[3]))):
+        win = rows[a + 1:b]
+        if not win:
+            continue
+        agg = {}
+        for t0, t1, nm in win:
+            v = agg.setdefault(short(nm), [0, 0])
+            v[0] += 1
+            v[1] += t1 - t0
+        tot = sum(v[1] for v in agg.values())
+        span = win[-1][1] - win[0][0]
+        g = [v for k, v in agg.items() if 'gemm_kernel' in k]
+        gl, gt = sum(v[0] for v in g), sum(v[1] for v in g)
+        out += [f'### {title}', '',
+                f'{len(win)} kernels, {tot / 1e6:.1f} ms inside kernels, {span / 1e6:.1f} ms from the first start '
+                f'to the last end. All `mpdo::gemm_kernel` instantiations: {gl} launches, {gt / 1e6:.1f} ms, '
+                f'average {gt / max(gl, 1) / 1e6:.4f} ms per launch.', '',
+                '| kernel | launches | total ms | avg ms | % of kernel time |', '|---|---|---|---|---|']
+        items = sorted(agg.items(), key=lambda kv: -kv[1][1])
+        for k, (c, t) in items[:top]:
+            out.append(f'| `{k}` | {c} | {t / 1e6:.1f} | {t / c / 1e6:.4f} | {100 * t / tot:.1f} |')
+        rest = items[top:]
+        if rest:
+            c = sum(v[0] for _, v in rest)
+            t = sum(v[1] for _, v in rest)
+            out.append(f'| {len(rest)} other kernels | {c} | {t / 1e6:.1f} | {t / c / 1e6:.4f} | {100 * t / tot:.1f} |')
+        out.append('')
+    return '\n'.join(out)
 
 
 def _traffic_per_launch():
@@ -199,7 +273,7 @@ def _uneven_batch(count, streams, dev):
 
 def _differentiation(dev):
     """One forward + reverse sweep of the 16-qubit, 8-layer ansatz (chi=64, kappa=4, 256 angles and
-    376 per-gate depolarizing strengths, per-layer checkpointing): seconds and peak device memory."""
+    376 per-gate depolarizing strengths, per-layer checkpointing): seconds and the peak device memory it adds."""
     import numpy as np
     import torch
     from mpdo_amd import autodiff as AD
@@ -207,8 +281,9 @@ def _differentiation(dev):
     angles = rng.uniform(-np.pi, np.pi, size=(8, 16, 2))
     m = AD.HEA(16, 8, 64, 4, angles[..., 0], angles[..., 1], np.full((8, 16, 2), 0.005), np.full((8, 15), 0.005),
                device=str(dev))
-    torch.cuda.reset_peak_memory_stats(dev)
     torch.cuda.synchronize()
+    torch.cuda.reset_peak_memory_stats(dev)
+    base = torch.cuda.memory_allocated(dev)
     t0 = time.perf_counter()
     E = AD.tfim_energy(m())
     torch.cuda.synchronize()
@@ -219,7 +294,7 @@ def _differentiation(dev):
     return {'workload': '16-qubit 8-layer RY/RZ + CNOT-chain ansatz, per-gate depolarizing p=0.005, chi=64, kappa=4, '
                         'loss <H> (TFIM h=1)', 'parameters': 632, 'energy': E.item(),
             'forward_s': round(t1 - t0, 3), 'backward_s': round(t2 - t1, 3),
-            'peak_device_gib': round(torch.cuda.max_memory_allocated(dev) / 2 ** 30, 2)}
+            'peak_device_gib': round((torch.cuda.max_memory_allocated(dev) - base) / 2 ** 30, 2)}
 
 
 def main():
@@ -248,7 +323,13 @@ def main():
                     help='before the warmup, run torch.neg over 1 GiB of float32 three times (1 GiB read and '
                          '1 GiB written per launch): the reference launch that scales the HBM counters of a '
                          'rocprofv3 --pmc run of this benchmark')
+    ap.add_argument('--kernel-trace-summary', metavar='CSV',
+                    help='print a markdown summary of the timed steps and the profiled pass of a rocprofv3 kernel '
+                         'trace (CSV) of this benchmark, then exit (no GPU work)')
     args = ap.parse_args()
+    if args.kernel_trace_summary:
+        print(kernel_trace_summary(args.kernel_trace_summary))
+        return
     if args.gpus != 1:
         print(f'bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). '
               'Chains are independent, so more GPUs would each take a share of the copies; run with --gpus 1.',
@@ -296,6 +377,7 @@ def main():
         _log(f'warmup step {k}: {time.perf_counter() - t:.2f} s')
 
     saved = None if args.fused_only else [t.clone() for t in sites]   # input of the first timed layer
+    _trace_marker()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -303,6 +385,7 @@ def main():
         _log(f'timed step {k} issued')
     torch.cuda.synchronize()
     T = time.perf_counter() - t0
+    _trace_marker()
     value = B * args.steps / T
     _log(f'timed: {T:.2f} s, {value:.3f} circuit-layers/s')
     del sites
@@ -311,9 +394,11 @@ def main():
     if not args.fused_only:
         L0 = layers[args.warmup]
         # ---- profiled pass: the first timed layer again, on its kept input
+        _trace_marker()
         _lib.prof_enable(True)
         outs, disc = _lib.layer_batched(saved, L0.kraus1, L0.parity, L0.gates2, CHI, KAPPA)
         torch.cuda.synchronize()
+        _trace_marker()
         prof = _lib.prof_read()
         execd = _lib.prof_read_executed()
         eig_log = _lib.prof_eig_log()
@@ -424,6 +509,8 @@ def main():
             _log(f'cpu baseline (oracle sample, {threads} threads)')
             cpu = cpu_baseline(*cpu_in, threads)
     lib.mpdo_set_math_mode(prev_mode)
+    _lib.release_workspaces()    # the fused layer's workspace (3.6 GiB per chain) is not needed below
+    torch.cuda.empty_cache()
     uneven = None
     if not args.fused_only and args.uneven > 0:
         uneven = _uneven_batch(args.uneven, args.uneven_streams, dev)

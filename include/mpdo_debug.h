@@ -77,10 +77,13 @@ int mpdo_debug_layer_gemm_log(int* out, int cap);
  * values, returns the number of pairs. */
 int mpdo_debug_layer_kept(int* out, int cap);
 
-/* Recording of the gate-time cut spectra of chain 0 in the fused layers that follow on this
- * thread (record = 1 on, 0 off: each pair then costs one extra readback); copies the last layer's
- * recorded eigenvalues (per pair its 2 p1 values, descending) into out and returns their count. */
+/* Recording of the spectra of chain 0 in the fused layers that follow on this thread (record = 1
+ * on, 0 off: each gate-time cut and chi step then costs one extra readback, and every gate-time cut
+ * runs its eigensolver). gate_eigs copies the last layer's gate-time cut spectra into out, per pair
+ * its order n followed by its n eigenvalues (descending), and returns the number of doubles;
+ * chi_eigs does the same for the chi steps at the pair bonds. */
 int mpdo_debug_layer_gate_eigs(int record, double* out, int cap);
+int mpdo_debug_layer_chi_eigs(double* out, int cap);
 
 /* The fused layer's gate-time cut decides "every direction kept in every chain" by one pivoted
  * Cholesky of G - D* I (D* the largest eigenvalue the absolute-error rule may drop) and then needs
@@ -88,6 +91,36 @@ int mpdo_debug_layer_gate_eigs(int record, double* out, int cap);
  * previous setting. mpdo_debug_layer_gate_skips: how many pairs of the last fused layer on this
  * thread were decided that way. */
 int mpdo_debug_layer_gate_skip(int on);
+/* Where the shifted Cholesky does not settle the cut, the fused layer first decides it from the
+ * bottom of the spectrum (subspace iteration with the inverse Cholesky factor, Rayleigh-Ritz and a
+ * positive-definiteness certificate; the full eigensolver only where that declines); on = 0 always
+ * runs the eigensolver, on < 0 leaves the switch. Returns the previous setting.
+ * mpdo_debug_layer_gate_tails: how many pairs of the last fused layer on this thread were decided
+ * that way. */
+int mpdo_debug_layer_gate_tail(int on);
+int mpdo_debug_layer_gate_tails(void);
+/* The chi step at a pair bond keeps chi of about 4 chi directions: the fused layer takes its top chi
+ * eigenpairs by subspace iteration on 2 chi columns, Rayleigh-Ritz and a positive-definiteness
+ * certificate (the full eigensolver only where that declines); on = 0 always runs the eigensolver,
+ * on < 0 leaves the switch. Returns the previous setting. mpdo_debug_layer_chi_tops: how many
+ * pair-bond chi steps of the last fused layer on this thread were decided that way. */
+int mpdo_debug_layer_chi_top(int on);
+int mpdo_debug_layer_chi_tops(void);
+/* That top-k solver on `batch` Hermitian positive semidefinite complex128 Gram matrices G (device,
+ * n x n each, contiguous; not modified): m subspace columns, q steps. info (host, 2 ints): accepted,
+ * the step that declined. When accepted, w (device, n doubles per matrix) holds the m Ritz values
+ * (descending), the trace's remainder, then zeros, and V (device, complex128 n x n per matrix) the
+ * m Ritz vectors in its first m columns. */
+mpdo_status mpdo_debug_gram_top_eig(int n, int batch, const void* G, int k, int m, int q, double* w, void* V, int* info,
+                                    void* stream);
+/* That bottom-of-the-spectrum decision on `batch` Hermitian positive semidefinite complex128 Gram
+ * matrices G (device, n x n each, contiguous): m subspace columns, q iteration steps, absolute
+ * error eps. info (host, 4 ints): accepted, some chain drops a direction, the largest kept count,
+ * the step that declined (0 when accepted). When accepted, kept and disc (host, batch each) hold the
+ * rule's kept counts and dropped eigenvalue sums, and where a direction is dropped P (device,
+ * complex64 n x n each) holds I - V_d V_d^H. */
+mpdo_status mpdo_debug_gram_tail_cut(int n, int batch, const void* G, double eps, int m, int q, int* kept,
+                                     double* disc, void* P, int* info, void* stream);
 int mpdo_debug_layer_gate_skips(void);
 
 /* The next `count` eigensolver calls of the per-gate entry points (mpdo_apply_2q, the sweeps,

@@ -298,6 +298,8 @@ def test_gate_cut_keep_all_test_matches_eigensolver(cuda):
                   for b in range(B)]
         sites = [torch.stack([chains[b][i] for b in range(B)]).contiguous() for i in range(n)]
         old = lib.mpdo_debug_layer_gate_skip(1)
+        old_tail = lib.mpdo_debug_layer_gate_tail(0)   # the other cuts by the eigensolver in both runs
+        old_top = lib.mpdo_debug_layer_chi_top(0)
         try:
             outs_a, disc_a = _lib.layer_batched(sites, L.kraus1, L.parity, L.gates2, 256, 8)
             skips = lib.mpdo_debug_layer_gate_skips()
@@ -308,6 +310,8 @@ def test_gate_cut_keep_all_test_matches_eigensolver(cuda):
             kept_b = _lib.layer_kept()
         finally:
             lib.mpdo_debug_layer_gate_skip(old)
+            lib.mpdo_debug_layer_gate_tail(old_tail)
+            lib.mpdo_debug_layer_chi_top(old_top)
         print(f'decay {decay}: {skips} of {len(kept_a)} gate cuts decided without the eigensolver; kept {kept_b}')
         if decay == 1.0:
             assert skips > 0
@@ -315,3 +319,98 @@ def test_gate_cut_keep_all_test_matches_eigensolver(cuda):
         for a, b in zip(outs_a, outs_b):
             assert torch.equal(a, b)
         assert np.array_equal(disc_a, disc_b)
+
+
+def test_gate_cut_from_the_bottom_matches_eigensolver(cuda):
+    """Where the shifted Cholesky does not settle the gate-time cut, the fused layer decides it from
+    the bottom of the spectrum (subspace iteration, Rayleigh-Ritz, certificate) and projects with
+    P = I - V_d V_d^H: on decaying-weight chains (cuts that drop a few directions) the kept counts are
+    those of the eigensolver, the dropped weights agree to 1e-6 relative, and every <Z_i>, <Z_i Z_j>
+    and Tr rho~ to 1e-6 of Tr rho~ (the expectations are read from complex64 data, resolved to
+    ~1.2e-7), chi and kappa weights to 1e-6 relative (1e-12 Tr absolute)."""
+    from mpdo_amd import _lib
+    from mpdo_amd import workloads as W
+    lib = _lib.load()
+    n, B = 10, 2
+    _, L = _layer10_inputs(cuda, B)
+    import importlib.util
+    spec = importlib.util.spec_from_file_location('make_golden', os.path.join(HERE, 'golden', 'make_golden.py'))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    codes, pairs = _zz_codes(n)
+    tails_seen = 0
+    for decay in (0.995, 0.99, 0.985):
+        chains = [[torch.from_numpy(t).to(cuda) for t in W.steady_state(mg.LAYER10_DIMS, seed=21 + b, decay=decay)]
+                  for b in range(B)]
+        sites = [torch.stack([chains[b][i] for b in range(B)]).contiguous() for i in range(n)]
+        old = lib.mpdo_debug_layer_gate_tail(1)
+        old_top = lib.mpdo_debug_layer_chi_top(0)   # the chi steps by the eigensolver in both runs
+        try:
+            outs_a, disc_a = _lib.layer_batched(sites, L.kraus1, L.parity, L.gates2, 256, 8)
+            tails = lib.mpdo_debug_layer_gate_tails()
+            kept_a = _lib.layer_kept()
+            lib.mpdo_debug_layer_gate_tail(0)
+            outs_b, disc_b = _lib.layer_batched(sites, L.kraus1, L.parity, L.gates2, 256, 8)
+            assert lib.mpdo_debug_layer_gate_tails() == 0
+            kept_b = _lib.layer_kept()
+        finally:
+            lib.mpdo_debug_layer_gate_tail(old)
+            lib.mpdo_debug_layer_chi_top(old_top)
+        print(f'decay {decay}: {tails} of {len(kept_a)} gate cuts decided from the bottom; kept {kept_a} / {kept_b}')
+        tails_seen += tails
+        assert kept_a == kept_b
+        assert [tuple(a.shape) for a in outs_a] == [tuple(b.shape) for b in outs_b]
+        for b in range(B):
+            va = np.array(_lib.pauli_expect([o[b].contiguous() for o in outs_a], codes))
+            vb = np.array(_lib.pauli_expect([o[b].contiguous() for o in outs_b], codes))
+            assert np.abs(va - vb).max() <= 1e-6 * abs(vb[-1]), (decay, b, np.abs(va - vb).max() / abs(vb[-1]))
+            tr = abs(vb[-1])
+            gate_a, gate_b = disc_a[b, 2 * n:], disc_b[b, 2 * n:]
+            assert np.allclose(gate_a, gate_b, rtol=1e-6, atol=1e-15 * tr), (decay, b, gate_a, gate_b)
+            assert np.allclose(disc_a[b, :2 * n], disc_b[b, :2 * n], rtol=1e-6, atol=1e-12 * tr), (decay, b)
+    assert tails_seen > 0
+
+
+def test_chi_step_by_top_eigenpairs_matches_eigensolver(cuda):
+    """The chi step at a pair bond by its top chi eigenpairs (subspace iteration on 2 chi columns,
+    Rayleigh-Ritz, certificate) against the same layer with the full eigensolver: same shapes and
+    kept counts, every <Z_i>, <Z_i Z_j> and Tr rho~ within 1e-6 of Tr rho~, discarded weights within
+    1e-6 relative (1e-12 Tr absolute), on the fixture chain and a flatter one."""
+    from mpdo_amd import _lib
+    from mpdo_amd import workloads as W
+    lib = _lib.load()
+    n, B = 10, 2
+    chains0, L = _layer10_inputs(cuda, B)
+    import importlib.util
+    spec = importlib.util.spec_from_file_location('make_golden', os.path.join(HERE, 'golden', 'make_golden.py'))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    codes, _ = _zz_codes(n)
+    tops_seen = 0
+    for name, chains in (('fixture', chains0),
+                         ('decay 0.99', [[torch.from_numpy(t).to(cuda)
+                                          for t in W.steady_state(mg.LAYER10_DIMS, seed=31 + b, decay=0.99)]
+                                         for b in range(B)])):
+        sites = [torch.stack([chains[b][i] for b in range(B)]).contiguous() for i in range(n)]
+        old = lib.mpdo_debug_layer_chi_top(1)
+        try:
+            outs_a, disc_a = _lib.layer_batched(sites, L.kraus1, L.parity, L.gates2, 256, 8)
+            tops = lib.mpdo_debug_layer_chi_tops()
+            kept_a = _lib.layer_kept()
+            lib.mpdo_debug_layer_chi_top(0)
+            outs_b, disc_b = _lib.layer_batched(sites, L.kraus1, L.parity, L.gates2, 256, 8)
+            assert lib.mpdo_debug_layer_chi_tops() == 0
+            kept_b = _lib.layer_kept()
+        finally:
+            lib.mpdo_debug_layer_chi_top(old)
+        print(f'{name}: {tops} pair-bond chi steps by the top solver; kept {kept_a}')
+        tops_seen += tops
+        assert kept_a == kept_b
+        assert [tuple(a.shape) for a in outs_a] == [tuple(b.shape) for b in outs_b]
+        for b in range(B):
+            va = np.array(_lib.pauli_expect([o[b].contiguous() for o in outs_a], codes))
+            vb = np.array(_lib.pauli_expect([o[b].contiguous() for o in outs_b], codes))
+            tr = abs(vb[-1])
+            assert np.abs(va - vb).max() <= 1e-6 * tr, (name, b, np.abs(va - vb).max() / tr)
+            assert np.allclose(disc_a[b], disc_b[b], rtol=1e-6, atol=1e-12 * tr), (name, b)
+    assert tops_seen > 0

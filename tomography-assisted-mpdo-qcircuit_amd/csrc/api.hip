@@ -1118,6 +1118,52 @@ mpdo_status mpdo_debug_eigh(int n, int batch, void* A, double* w, void* V, int* 
   return MPDO_OK;
 }
 
+mpdo_status mpdo_debug_gram_tail_cut(int n, int batch, const void* G, double eps, int m, int q, int* kept,
+                                     double* disc, void* P, int* info, void* stream) {
+  if (n <= 0 || batch <= 0 || !G || !kept || !disc || !P || !info)
+    return fail(MPDO_EINVAL, "n=%d batch=%d or a null pointer", n, batch);
+  hipStream_t s = S(stream);
+  Carver c0;
+  c0.get<char>(gram_tail_workspace_bytes(n, m, batch));
+  const size_t o_kept = c0.off;
+  c0.get<int>(batch);
+  const size_t o_disc = c0.off;
+  c0.get<double>(batch);
+  char* base;
+  ST(ws_reserve(c0.off, s, &base));
+  int* dkept = (int*)(base + o_kept);
+  double* ddisc = (double*)(base + o_disc);
+  TailResult r;
+  CK(gram_tail_cut(n, batch, (const double2*)G, eps, m, q, base, s, dkept, ddisc, (float2*)P, n, (long long)n * n,
+                   &r));
+  if (r.ok) {
+    CK(hipMemcpyAsync(kept, dkept, sizeof(int) * batch, hipMemcpyDeviceToHost, s));
+    CK(hipMemcpyAsync(disc, ddisc, sizeof(double) * batch, hipMemcpyDeviceToHost, s));
+  }
+  CK(hipStreamSynchronize(s));
+  info[0] = r.ok;
+  info[1] = r.any_drop;
+  info[2] = r.max_kept;
+  info[3] = r.reason;
+  return MPDO_OK;
+}
+
+mpdo_status mpdo_debug_gram_top_eig(int n, int batch, const void* G, int k, int m, int q, double* w, void* V, int* info,
+                                    void* stream) {
+  if (n <= 0 || batch <= 0 || !G || !w || !V || !info) return fail(MPDO_EINVAL, "n=%d batch=%d or a null pointer", n, batch);
+  hipStream_t s = S(stream);
+  Carver c0;
+  c0.get<char>(gram_top_workspace_bytes(n, m, batch));
+  char* base;
+  ST(ws_reserve(c0.off, s, &base));
+  TopResult r;
+  CK(gram_top_eig(n, batch, (const double2*)G, k, m, q, base, s, w, n, (double2*)V, n, (long long)n * n, &r));
+  CK(hipStreamSynchronize(s));
+  info[0] = r.ok;
+  info[1] = r.reason;
+  return MPDO_OK;
+}
+
 mpdo_status mpdo_debug_eigh_gram(int n, int batch, void* A, double* w, void* V, int* sweeps, void* stream) {
   if (n <= 0 || batch <= 0) return fail(MPDO_EINVAL, "n=%d batch=%d", n, batch);
   hipStream_t s = S(stream);

@@ -81,12 +81,31 @@ thread_local std::string t_oob;
 thread_local bool t_dry = false;
 thread_local size_t t_ws_bytes = 0;   // workspace size of the last planned layer
 thread_local std::vector<int> t_kept;   // per pair of the last layer: max kept count of its gate-time split
-// gate-time cut spectra of chain 0 (per pair: nT eigenvalues, descending), recorded on request
+// spectra of chain 0 recorded on request: per gate-time cut and per chi step, its order n followed
+// by its n eigenvalues (descending)
 thread_local bool t_rec_gate = false;
 thread_local std::vector<double> t_gate_eigs;
+thread_local std::vector<double> t_chi_eigs;
+mpdo_status record_eigs(std::vector<double>& dst, const double* w, int n, hipStream_t s) {
+  const size_t o = dst.size();
+  dst.resize(o + 1 + n);
+  dst[o] = n;
+  LCK(hipMemcpyAsync(dst.data() + o + 1, w, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  LCK(hipStreamSynchronize(s));
+  return MPDO_OK;
+}
 thread_local std::vector<int> t_gemms;  // dry run: (source line, M, N, K, batch, precision) of every GEMM
 thread_local int t_gate_skips = 0;     // gate-time cuts decided by the shifted Cholesky (last layer)
+thread_local int t_gate_tails = 0;     // gate-time cuts decided from the bottom of the spectrum (last layer)
 bool g_gate_skip = true;               // mpdo_debug_layer_gate_skip
+bool g_gate_tail = true;               // mpdo_debug_layer_gate_tail
+// the bottom-of-the-spectrum cut: subspace columns and iteration steps
+constexpr int TAIL_M = 128, TAIL_Q = 5;
+// the chi step at a pair bond by its top k = chi eigenpairs: 2k subspace columns, iteration steps
+constexpr int TOP_Q = 4;
+thread_local int t_chi_tops = 0;       // pair-bond chi steps decided that way (last layer)
+bool g_chi_top = true;                 // mpdo_debug_layer_chi_top
+int top_m(int n, int k) { return (k >= 16 && 2 * k + 64 <= n && n > 64 && n <= 1024) ? 2 * k : 0; }
 
 void add_span(const void* p, size_t bytes) {
   if (t_spans && p && bytes) t_spans->push_back(Span{(uintptr_t)p, (uintptr_t)p + bytes});
@@ -462,10 +481,19 @@ struct EigB {
   double* w;
   void* ws;
 };
-void carve_eig(Carver& c, int n, int batch, EigB& e) {
+// workspace of the eigensolvers at order n: the full one, the bottom-of-the-spectrum cut and the
+// top-k solver for k up to kmax
+size_t eig_ws_bytes(int n, int batch, int kmax) {
+  size_t b = eigh_gram_workspace_bytes(n, batch);
+  if (n > 64 && 2 * TAIL_M <= n) b = std::max(b, gram_tail_workspace_bytes(n, TAIL_M, batch));
+  const int k = std::min(kmax, (n - 64) / 2);
+  if (top_m(n, k)) b = std::max(b, gram_top_workspace_bytes(n, top_m(n, k), batch));
+  return b;
+}
+void carve_eig(Carver& c, int n, int batch, int kmax, EigB& e) {
   e.V = c.get<double2>((size_t)n * n * batch);
   e.w = c.get<double>((size_t)n * batch);
-  e.ws = c.get<char>(eigh_gram_workspace_bytes(n, batch));
+  e.ws = c.get<char>(eig_ws_bytes(n, batch, kmax));
 }
 mpdo_status eig_b_(int line, int n, int batch, double2* G, EigB& e, hipStream_t s, bool gram = false) {
   int sweeps = 0;
@@ -915,6 +943,8 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     mxPn = std::max(mxPn, (size_t)P.in[ps.i + 1].r * std::max(kr, P.in[ps.i + 1].r));
   }
   const int mxEig = std::max({mxEigQR, mxEigC, mxEigChi, mxEigK, mxEigR});
+  int mxKp = 0;   // the largest kept count of a pair-bond chi step
+  for (auto& ps : pairs) mxKp = std::max(mxKp, P.chi_out[ps.i + 1].l);
   size_t mxBond = 1;
   for (int i = 0; i < n; ++i) {
     mxBond = std::max(mxBond, (size_t)std::max(P.in[i].l, P.in[i].r));
@@ -1008,7 +1038,7 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     Vf = c.get<float2>(mxV * B);
     Vt = c.get<float2>(mxVt * B);
     GR = c.get<double2>((size_t)mxEig * mxEig * B);
-    carve_eig(c, mxEig, B, eE);
+    carve_eig(c, mxEig, B, mxKp, eE);
     carve_qr(c, (int)mxQR_rows, (int)mxQR_cols, B, qw);
     dk = c.get<int>((size_t)B * 4);
     ddisc = c.get<double>((size_t)B * (nsteps + 2 * n));
@@ -1124,7 +1154,10 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
 
   t_kept.clear();
   t_gate_skips = 0;
+  t_gate_tails = 0;
+  t_chi_tops = 0;
   t_gate_eigs.clear();
+  t_chi_eigs.clear();
   // ---------------- phase 2: QR sweep left to right ----------------
   // Rcur: [B][kp][l] with kp the QR bond so far (1 at the left end)
   int kcur = 1;
@@ -1209,19 +1242,32 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
       bool keeps_all = false;
       if (!dry && !t_rec_gate && use_gram_eig() && len_gate >= nT && g_gate_skip)
         LCK(gram_keeps_all(nT, B, G, nT, (long long)nT * nT, GATE_EPS, eE.ws, s, &keeps_all));
+      // otherwise the few directions the rule drops come from the bottom of the spectrum (subspace
+      // iteration, Rayleigh-Ritz and a certificate; P = I - V_d V_d^H over all nT columns), or,
+      // where that declines, from the full eigensolver
+      TailResult tail;
+      if (!keeps_all && !dry && !t_rec_gate && use_gram_eig() && len_gate >= nT && g_gate_tail && nT > 64 &&
+          2 * TAIL_M <= nT) {
+        if (!fits(eE.ws, (long long)gram_tail_workspace_bytes(nT, TAIL_M, B), 1, "gate-time cut workspace", __LINE__))
+          return layer_fail(MPDO_EINVAL, t_oob.c_str());
+        if (!fits(ps.Um, (long long)nT * nT * B, sizeof(float2), "gate-time projector", __LINE__))
+          return layer_fail(MPDO_EINVAL, t_oob.c_str());
+        LCK(gram_tail_cut(nT, B, G, GATE_EPS, TAIL_M, TAIL_Q, eE.ws, s, dk, ddisc + (size_t)B * (nsteps + n + i), ps.Um,
+                          nT, (long long)nT * nT, &tail));
+      }
       if (keeps_all) {
         ps.has_P = false;
         ps.mq = nT;
         t_kept.push_back(nT);
         ++t_gate_skips;
+      } else if (tail.ok) {
+        ps.has_P = tail.any_drop;
+        ps.mq = nT;
+        t_kept.push_back(tail.any_drop ? tail.max_kept : nT);
+        ++t_gate_tails;
       } else {
         LST(eig_b(nT, B, G, eE, s, true));
-        if (t_rec_gate && !dry) {
-          const size_t o = t_gate_eigs.size();
-          t_gate_eigs.resize(o + nT);
-          LCK(hipMemcpyAsync(t_gate_eigs.data() + o, eE.w, sizeof(double) * nT, hipMemcpyDeviceToHost, s));
-          LCK(hipStreamSynchronize(s));
-        }
+        if (t_rec_gate && !dry) LST(record_eigs(t_gate_eigs, eE.w, nT, s));
         LCK(K_KEEP(eE.w, nT, nT, len_gate, -1, GATE_EPS, 0, dk, ddisc + (size_t)B * (nsteps + n + i), B, s));
         // the projector onto the kept directions is applied only when some chain drops one; the
         // core then lives in the mq-dimensional span of the kept directions (mq = max over chains)
@@ -1526,7 +1572,18 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
       h.herm = 1;
       LCK(K_GEMM(C128_F64ACC, h, s));
     }
-    LST(eig_b(nC, B, G, eE, s, true));
+    // the step keeps kp of about 4 kp directions: its top kp eigenpairs by subspace iteration and
+    // a certificate, or, where that declines, the full eigensolver
+    TopResult top;
+    const int mt = top_m(nC, kp);
+    if (mt && !dry && !t_rec_gate && use_gram_eig() && g_chi_top && kp <= P.step_len[i]) {
+      if (!fits(eE.ws, (long long)gram_top_workspace_bytes(nC, mt, B), 1, "chi step workspace", __LINE__))
+        return layer_fail(MPDO_EINVAL, t_oob.c_str());
+      LCK(gram_top_eig(nC, B, G, kp, mt, TOP_Q, eE.ws, s, eE.w, nC, eE.V, nC, (long long)nC * nC, &top));
+      if (top.ok) ++t_chi_tops;
+    }
+    if (!top.ok) LST(eig_b(nC, B, G, eE, s, true));
+    if (t_rec_gate && !dry) LST(record_eigs(t_chi_eigs, eE.w, nC, s));
     LCK(K_KEEP(eE.w, nC, nC, P.step_len[i], kp, -1.0, 1, dk, ddisc + (size_t)B * i, B, s));
     // new min site: (Q1 (x) I) U sqrt(S) -> (kprev, a1, s1, kp) -> (kprev, s1, a1, kp)
     LCK(K_FACTOR(eE.V, nC, (long long)nC * nC, eE.w, nC, nC, nC, kp, 0.25, SPLIT_FLOOR, 0, FL, kp, (long long)nC * kp, B, s));
@@ -1647,6 +1704,12 @@ int mpdo_debug_layer_gate_eigs(int record, double* out, int cap) {
   return n;
 }
 
+int mpdo_debug_layer_chi_eigs(double* out, int cap) {
+  const int n = (int)t_chi_eigs.size();
+  for (int i = 0; i < n && i < cap; ++i) out[i] = t_chi_eigs[i];
+  return n;
+}
+
 int mpdo_debug_layer_gate_skip(int on) {
   const int old = g_gate_skip ? 1 : 0;
   if (on >= 0) g_gate_skip = on != 0;
@@ -1654,6 +1717,22 @@ int mpdo_debug_layer_gate_skip(int on) {
 }
 
 int mpdo_debug_layer_gate_skips(void) { return t_gate_skips; }
+
+int mpdo_debug_layer_gate_tail(int on) {
+  const int old = g_gate_tail ? 1 : 0;
+  if (on >= 0) g_gate_tail = on != 0;
+  return old;
+}
+
+int mpdo_debug_layer_gate_tails(void) { return t_gate_tails; }
+
+int mpdo_debug_layer_chi_top(int on) {
+  const int old = g_chi_top ? 1 : 0;
+  if (on >= 0) g_chi_top = on != 0;
+  return old;
+}
+
+int mpdo_debug_layer_chi_tops(void) { return t_chi_tops; }
 
 int mpdo_debug_layer_kept(int* out, int cap) {
   const int n = (int)t_kept.size();

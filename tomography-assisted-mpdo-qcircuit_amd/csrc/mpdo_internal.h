@@ -81,6 +81,38 @@ hipError_t eigh_gram(int n, int batch, void* A, long long lda, long long bsA,
 hipError_t gram_keeps_all(int n, int batch, const void* A, long long lda, long long bsA, double eps, void* ws,
                           hipStream_t st, bool* all);
 
+// The same cut decided from the bottom of the spectrum (tailcut.hip): subspace iteration with the
+// inverse of a pivoted Cholesky factor of G[b] (n x n complex128, contiguous batch entries; not
+// modified) on m columns for q steps, Rayleigh-Ritz, the rule on the Ritz values and a
+// positive-definiteness certificate for the values it rests on. On success (res->ok) kept[b] and
+// disc[b] (device) hold the rule's kept count and dropped eigenvalue sum, and where some chain
+// drops a direction (res->any_drop) P[b] (complex64, leading dim ldP, batch stride bsP) = I - V_d
+// V_d^H, V_d the dropped eigenvectors. Otherwise nothing is written and the caller runs the full
+// eigensolver (res->reason says which step declined). 64 < n <= 1024, 16 <= m <= n / 2.
+// Synchronises the stream four times. `ws`: gram_tail_workspace_bytes(n, m, batch).
+struct TailResult {
+  bool ok = false, any_drop = false;
+  int max_kept = 0, reason = 0;
+};
+size_t gram_tail_workspace_bytes(int n, int m, int batch);
+hipError_t gram_tail_cut(int n, int batch, const double2* G, double eps, int m, int q, void* ws, hipStream_t st,
+                         int* kept, double* disc, float2* P, long long ldP, long long bsP, TailResult* res);
+
+// The k largest eigenpairs of Gram matrices G[b] (n x n complex128, contiguous; not modified) by
+// subspace iteration on m columns (q steps), Rayleigh-Ritz and a positive-definiteness certificate
+// that the k leading Ritz values are the k largest eigenvalues (tailcut.hip). On success (res->ok)
+// w[b] (leading dim ldw) holds the m Ritz values, descending, then the trace's remainder, then
+// zeros (n values), and V[b] (leading dim ldv, batch stride bsV) the m Ritz vectors in its first m
+// columns; otherwise nothing is written (res->reason says which step declined). 64 < n <= 1024,
+// k + 16 <= m <= n - 64. Synchronises the stream three times. `ws`: gram_top_workspace_bytes.
+struct TopResult {
+  bool ok = false;
+  int reason = 0;
+};
+size_t gram_top_workspace_bytes(int n, int m, int batch);
+hipError_t gram_top_eig(int n, int batch, const double2* G, int k, int m, int q, void* ws, hipStream_t st, double* w,
+                        long long ldw, double2* V, long long ldv, long long bsV, TopResult* res);
+
 // Pivoted Cholesky of positive semidefinite Gram matrices A[b] (n x n, leading dim lda, batch
 // stride bsA; destroyed): L[b] (n x n, stored by columns: L[b][k n + i] = L_ik) with A = L L^H,
 // piv[b][k] the row pivoted at column k, rank[b] the columns produced before the largest remaining

@@ -90,8 +90,15 @@ def _load() -> ctypes.CDLL:
         'mpdo_debug_fail_eig': (None, [I]),
         'mpdo_debug_layer_gate_skip': (I, [I]),
         'mpdo_debug_layer_gate_skips': (I, []),
+        'mpdo_debug_layer_gate_tail': (I, [I]),
+        'mpdo_debug_layer_gate_tails': (I, []),
+        'mpdo_debug_layer_chi_top': (I, [I]),
+        'mpdo_debug_layer_chi_tops': (I, []),
+        'mpdo_debug_gram_top_eig': (I, [I, I, VP, I, I, I, VP, VP, P(I), VP]),
+        'mpdo_debug_gram_tail_cut': (I, [I, I, VP, D, I, I, P(I), P(D), VP, P(I), VP]),
         'mpdo_debug_layer_kept': (I, [P(I), I]),
         'mpdo_debug_layer_gate_eigs': (I, [I, P(D), I]),
+        'mpdo_debug_layer_chi_eigs': (I, [P(D), I]),
         'mpdo_debug_layer_gemm_log': (I, [P(I), I]),
         'mpdo_debug_gemm_tiles': (I, [I]),
         'mpdo_debug_bj_threads': (I, [I]),
@@ -169,10 +176,21 @@ def _run(what: str, device: torch.device, call) -> None:
 def release_workspaces() -> None:
     """Unbind and drop every workspace buffer this module holds."""
     lib = load()
-    for (dev, handle), buf in list(_ws_bufs.items()):
-        with torch.cuda.device(dev):
-            lib.mpdo_bind_workspace(ctypes.c_void_p(handle), None, 0)
-    _ws_bufs.clear()
+    with _ws_lock:
+        for (dev, handle), buf in list(_ws_bufs.items()):
+            with torch.cuda.device(dev):
+                lib.mpdo_bind_workspace(ctypes.c_void_p(handle), None, 0)
+        _ws_bufs.clear()
+
+
+def release_stream_workspace(device: torch.device) -> None:
+    """Unbind and drop the workspace of the current stream of `device` (after the stream's work
+    that used it has finished): its memory returns to torch's caching allocator."""
+    handle = torch.cuda.current_stream(device).cuda_stream
+    key = (device.index if device.index is not None else torch.cuda.current_device(), handle)
+    with _ws_lock:
+        if _ws_bufs.pop(key, None) is not None:
+            load().mpdo_bind_workspace(ctypes.c_void_p(handle), None, 0)
 
 
 def _require_device(t: torch.Tensor) -> None:

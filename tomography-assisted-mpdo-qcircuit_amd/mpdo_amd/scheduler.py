@@ -22,7 +22,7 @@ from typing import Dict, List, Optional, Sequence
 
 import torch
 
-from . import batched
+from . import _lib, batched
 from .circuit import TensorCircuit
 from .node import SiteNode
 
@@ -77,6 +77,14 @@ def evolve_many(circuits: Sequence[TensorCircuit], states: Sequence[List[SiteNod
     done = {'fused_layers': 0, 'per_gate_layers': 0}
 
     def worker(stream: torch.cuda.Stream):
+        try:
+            run(stream)
+        finally:   # the stream's workspace goes back to torch's allocator
+            with torch.cuda.device(dev), torch.cuda.stream(stream):
+                stream.synchronize()
+                _lib.release_stream_workspace(dev)
+
+    def run(stream: torch.cuda.Stream):
         while True:
             with lock:
                 if nxt[0] >= len(groups) or errors:

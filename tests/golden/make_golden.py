@@ -1,7 +1,9 @@
 """Writes the oracle's outputs for the golden fixtures of the GPU parity tests. Run from the
 repository root (in the build container; the GPU tests only read the .npz files):
 
-    python tests/golden/make_golden.py [brickwork8 | brickwork20 | layer10]
+    python tests/golden/make_golden.py [brickwork8 | brickwork20 | family8 | layer10 | hea16 | batch16]
+
+(hea16 and batch16 spread their oracle runs over GOLDEN_JOBS single-threaded processes, default 5.)
 
 brickwork8  -> brickwork8_chi32_oracle.npz: 8 qubits, 4 CNOT brickwork layers, dep1 p=1e-3 /
                dep2 p=1e-2, chi=32, kappa=4: the full 256x256 density matrix, the trace and the
@@ -18,7 +20,7 @@ hea16       -> hea16_chi64_fd_oracle.npz: the 16-qubit hardware-efficient ansatz
                angles U[-pi, pi) from PCG64(303), CNOT chain, depolarizing p=0.005 after every gate,
                each p its own parameter), chi=64, kappa=4: <H> of the transverse-field Ising model
                (h=1) and central finite differences of it (h=1e-4 for angles, 1e-5 for noise
-               strengths) for 8 of the 632 parameters.
+               strengths) for 2 of the 632 parameters (an angle and a two-qubit noise strength).
 batch16     -> random_brickwork16_oracle.npz: the first 16 circuits of the uneven batch
                (workloads.random_brickwork_batch, seed 404: n 8..24, depth 4..16, chi 16..64,
                kappa 2/4, depolarizing or amplitude damping): every <Z_i> and Tr rho~ per circuit.
@@ -106,8 +108,7 @@ def family8():
     print('trace', np.trace(rho).real, f'total {time.time() - t:.1f} s')
 
 
-HEA16_FD = [('theta', (0, 3)), ('theta', (5, 8)), ('phi', (2, 10)), ('phi', (7, 1)), ('p1', (1, 4, 0)),
-            ('p1', (6, 12, 1)), ('p2', (0, 7)), ('p2', (7, 14))]
+HEA16_FD = [('theta', (0, 3)), ('p2', (0, 7))]
 
 
 def hea16_params():
@@ -118,38 +119,62 @@ def hea16_params():
     return angles[..., 0].copy(), angles[..., 1].copy(), np.full((8, 16, 2), 0.005), np.full((8, 15), 0.005)
 
 
-def hea16():
+def _hea16_energy(task):
+    """<H> of the 16-qubit ansatz with parameter `name`[idx] moved by sgn x h (name None: as drawn)."""
+    name, idx, sgn = task
     theta, phi, p1, p2 = hea16_params()
-    t = time.time()
-    e0, _ = bridge.run_hea_oracle(theta, phi, p1, p2, 64, 4)
-    print(f'E = {e0!r}: {time.time() - t:.1f} s', flush=True)
-    fd = []
-    for name, idx in HEA16_FD:
-        h = 1e-4 if name in ('theta', 'phi') else 1e-5
-        vals = []
-        for sgn in (1, -1):
-            args = dict(theta=theta.copy(), phi=phi.copy(), p1=p1.copy(), p2=p2.copy())
-            args[name][idx] += sgn * h
-            vals.append(bridge.run_hea_oracle(args['theta'], args['phi'], args['p1'], args['p2'], 64, 4)[0])
-        fd.append((vals[0] - vals[1]) / (2 * h))
-        print(name, idx, fd[-1], f'{time.time() - t:.1f} s', flush=True)
-    np.savez_compressed(os.path.join(HERE, 'hea16_chi64_fd_oracle.npz'), energy=e0, fd=np.array(fd),
-                        names=np.array([n for n, _ in HEA16_FD]),
-                        index=np.array([list(i) + [-1] * (3 - len(i)) for _, i in HEA16_FD]))
+    args = dict(theta=theta, phi=phi, p1=p1, p2=p2)
+    if name is not None:
+        args[name][idx] += sgn * (1e-4 if name in ('theta', 'phi') else 1e-5)
+    return bridge.run_hea_oracle(args['theta'], args['phi'], args['p1'], args['p2'], 64, 4)[0]
 
 
-def batch16():
-    ws = W.random_brickwork_batch()[:16]
+def _batch16_circuit(k):
+    w = W.random_brickwork_batch()[k]
+    st = bridge.run_oracle_fused(w)
+    return [O.pauli_expect(st, [2], [i]) for i in range(w.n)], O.trace_rho(st)
+
+
+def _work(job):
+    kind, arg = job
     t = time.time()
-    z, tr, off = [], [], [0]
-    for w in ws:
-        st = bridge.run_oracle_fused(w)
-        z.extend(O.pauli_expect(st, [2], [i]) for i in range(w.n))
-        tr.append(O.trace_rho(st))
-        off.append(len(z))
-        print(w.name, w.n, w.depth, w.chi, w.kappa, f'{time.time() - t:.1f} s', flush=True)
-    np.savez_compressed(os.path.join(HERE, 'random_brickwork16_oracle.npz'), z=np.array(z), trace=np.array(tr),
-                        offsets=np.array(off))
+    out = _hea16_energy(arg) if kind == 'hea16' else _batch16_circuit(arg)
+    print(kind, arg, f'{time.time() - t:.1f} s', flush=True)
+    return job, out
+
+
+def hea16_batch16(names, jobs):
+    """The hea16 and batch16 fixtures, their oracle runs spread over `jobs` single-threaded worker
+    processes (each run is a long sequence of mid-sized LAPACK calls)."""
+    import multiprocessing as mp
+    for v in ('OMP_NUM_THREADS', 'OPENBLAS_NUM_THREADS', 'MKL_NUM_THREADS'):
+        os.environ[v] = '1'
+    tasks = []
+    if 'hea16' in names:
+        tasks += [('hea16', (None, None, 0))] + [('hea16', (nm, idx, sg)) for nm, idx in HEA16_FD for sg in (1, -1)]
+    if 'batch16' in names:
+        tasks += [('batch16', k) for k in range(16)]
+    with mp.get_context('spawn').Pool(jobs) as pool:
+        res = dict(pool.imap_unordered(_work, tasks))
+    if 'hea16' in names:
+        e0 = res[('hea16', (None, None, 0))]
+        fd = []
+        for nm, idx in HEA16_FD:
+            h = 1e-4 if nm in ('theta', 'phi') else 1e-5
+            fd.append((res[('hea16', (nm, idx, 1))] - res[('hea16', (nm, idx, -1))]) / (2 * h))
+        print('E', repr(e0), 'fd', fd, flush=True)
+        np.savez_compressed(os.path.join(HERE, 'hea16_chi64_fd_oracle.npz'), energy=e0, fd=np.array(fd),
+                            names=np.array([n for n, _ in HEA16_FD]),
+                            index=np.array([list(i) + [-1] * (3 - len(i)) for _, i in HEA16_FD]))
+    if 'batch16' in names:
+        z, tr, off = [], [], [0]
+        for k in range(16):
+            zk, tk = res[('batch16', k)]
+            z.extend(zk)
+            tr.append(tk)
+            off.append(len(z))
+        np.savez_compressed(os.path.join(HERE, 'random_brickwork16_oracle.npz'), z=np.array(z), trace=np.array(tr),
+                            offsets=np.array(off))
 
 
 def layer10_ops():
@@ -179,6 +204,10 @@ def layer10():
 
 if __name__ == '__main__':
     which = sys.argv[1:] or ['brickwork8']
+    jobs = int(os.environ.get('GOLDEN_JOBS', '5'))
+    pooled = [n for n in which if n in ('hea16', 'batch16')]
     for name in which:
-        {'brickwork8': brickwork8, 'brickwork20': brickwork20, 'family8': family8, 'layer10': layer10,
-         'hea16': hea16, 'batch16': batch16}[name]()
+        if name not in pooled:
+            {'brickwork8': brickwork8, 'brickwork20': brickwork20, 'family8': family8, 'layer10': layer10}[name]()
+    if pooled:
+        hea16_batch16(pooled, jobs)

@@ -5,7 +5,7 @@ oracle (complex128):
 * the eigensolver entry point mpdo_eigh against numpy;
 * a 5-qubit, 2-layer ansatz with chi and kappa truncating: every one of its 48 parameters;
 * the 16-qubit, 8-layer ansatz at chi = 64, kappa = 4 (the differentiation configuration): <H> and
-  8 of its 632 gradient components against the committed fixture (tests/golden/make_golden.py
+  2 of its 632 gradient components against the committed fixture (tests/golden/make_golden.py
   hea16), all 632 from one reverse sweep with per-layer checkpointing.
 
 Finite differences: h = 1e-4 for angles (at h = 1e-3 one component of the 5-qubit case carries a
@@ -82,7 +82,7 @@ def test_hea_small_all_parameters(cuda):
 @pytest.mark.timeout(900)
 def test_hea16_chi64_gradient_fixture(cuda):
     """The 16-qubit differentiation configuration: one forward and one reverse sweep (checkpointed
-    per layer) give <H> and its 632 partial derivatives; <H> and 8 of them against the oracle's
+    per layer) give <H> and its 632 partial derivatives; <H> and 2 of them against the oracle's
     central differences."""
     import importlib.util
     from mpdo_amd import autodiff as AD

@@ -308,7 +308,7 @@ size_t pchol_workspace_bytes(int n, int batch) {
 }
 
 hipError_t pchol_b(int n, int batch, void* A, long long lda, long long bsA, double tol_rel, double2* L, int* piv,
-                   int* rank, void* ws, hipStream_t st) {
+                   int* rank, void* ws, hipStream_t st, bool early_exit) {
   if (n <= 0 || batch <= 0) return hipSuccess;
   if (n > PC_ROWS * PC_THREADS) return hipErrorInvalidValue;
   hipError_t err;
@@ -338,8 +338,9 @@ hipError_t pchol_b(int n, int batch, void* A, long long lda, long long bsA, doub
     g.B = L + (long long)j0 * n; g.rsB = n; g.csB = 1; g.bsB = (long long)n * n; g.conjB = 1;
     g.C = A; g.rsC = lda; g.csC = 1; g.bsC = bsA;
     g.alpha_re = -1.0; g.beta_re = 1.0;
+    g.herm = 1;   // the updated matrix stays Hermitian: lower tiles, then the mirror
     if ((err = gemm(C128_F64ACC, g, st)) != hipSuccess) return err;
-    if ((j0 + nb) % 128 == 0) {   // every 128 columns: stop once every matrix has stopped
+    if (early_exit && (j0 + nb) % 128 == 0) {   // every 128 columns: stop once every matrix has stopped
       if ((err = hipMemcpyAsync(hs.data(), pc.stop, sizeof(int) * batch, hipMemcpyDeviceToHost, st)) != hipSuccess)
         return err;
       if ((err = hipStreamSynchronize(st)) != hipSuccess) return err;
@@ -347,6 +348,171 @@ hipError_t pchol_b(int n, int batch, void* A, long long lda, long long bsA, doub
       for (int b = 0; b < batch; ++b) all = all && hs[b];
       if (all) break;
     }
+  }
+  return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------------------
+// Unpivoted blocked Cholesky for well-conditioned Hermitian matrices (the Gram matrices of nearly
+// orthonormal blocks, and positive-definiteness tests): per 64-column panel one workgroup per
+// matrix factors the diagonal block in LDS and inverts its triangle, one GEMM solves the rows
+// below (A_ip Lpp^{-H}, triangular operand), one Hermitian GEMM updates the trailing matrix. No
+// host synchronisation. A matrix whose pivot falls to tol_rel x max_i A_ii stops there (rank).
+// Output in pchol_b's layout with the identity pivot order, for pchol_inverse_b.
+// ---------------------------------------------------------------------------------------
+namespace {
+
+struct ChState {
+  int* rank;
+  int* stop;
+  double* thr;
+  double2* Linv;   // [batch][64][64] inverse of the current diagonal block (row-major)
+};
+
+__global__ __launch_bounds__(256) void chol_init_kernel(const double2* A, int n, long long lda, long long bsA,
+                                                        double tol_rel, double2* L, int* piv, ChState st) {
+  __shared__ double red[256];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  A += (long long)b * bsA;
+  double m = 0;
+  for (int i = tid; i < n; i += 256) {
+    m = fmax(m, A[(long long)i * lda + i].x);
+    piv[(long long)b * n + i] = i;
+  }
+  double2* Lb = L + (long long)b * n * n;
+  for (long long e = tid; e < (long long)n * n; e += 256) Lb[e] = make_double2(0.0, 0.0);
+  red[tid] = m;
+  __syncthreads();
+  for (int s2 = 128; s2 > 0; s2 >>= 1) {
+    if (tid < s2) red[tid] = fmax(red[tid], red[tid + s2]);
+    __syncthreads();
+  }
+  if (tid == 0) {
+    st.thr[b] = tol_rel * red[0];
+    st.rank[b] = 0;
+    st.stop[b] = 0;
+  }
+}
+
+// Diagonal block j0 .. j0 + nb - 1 of matrix b (already updated by the earlier panels): Cholesky in
+// LDS, written to L (column k at L[k n + i]); its triangle's inverse to st.Linv.
+__global__ __launch_bounds__(256) void chol_diag_kernel(const double2* A, int n, long long lda, long long bsA, int j0,
+                                                        double2* L, ChState st) {
+  __shared__ double2 S[64][65];
+  __shared__ double2 X[64][65];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  if (st.stop[b]) return;
+  const int nb = min(64, n - j0);
+  A += (long long)b * bsA + (long long)j0 * lda + j0;
+  for (int e = tid; e < nb * nb; e += 256) {
+    const int i = e / nb, k = e % nb;
+    S[i][k] = A[(long long)i * lda + k];
+  }
+  const double thr = st.thr[b];
+  int fail = -1;
+  for (int j = 0; j < nb; ++j) {
+    __syncthreads();
+    const double d = S[j][j].x;
+    if (!(d > thr)) { fail = j; break; }   // uniform: every thread read the same value
+    const double piv = sqrt(d);
+    __syncthreads();
+    for (int i = j + 1 + tid; i < nb; i += 256) S[i][j] = make_double2(S[i][j].x / piv, S[i][j].y / piv);
+    if (tid == 0) S[j][j] = make_double2(piv, 0.0);
+    __syncthreads();
+    // S[i][k] -= S[i][j] conj(S[k][j]) for j < k <= i < nb
+    const int m = nb - j - 1;
+    for (int e = tid; e < m * m; e += 256) {
+      const int i = j + 1 + e / m, k = j + 1 + e % m;
+      if (k <= i) {
+        const double2 x = S[i][j], y = S[k][j];
+        S[i][k].x -= x.x * y.x + x.y * y.y;
+        S[i][k].y -= x.y * y.x - x.x * y.y;
+      }
+    }
+  }
+  __syncthreads();
+  double2* Lb = L + (long long)b * n * n;
+  double2* Li = st.Linv + (long long)b * 64 * 64;
+  if (fail >= 0) {
+    for (int e = tid; e < 64 * 64; e += 256) Li[e] = make_double2(0.0, 0.0);
+    if (tid == 0) {
+      st.rank[b] = j0 + fail;
+      st.stop[b] = 1;
+    }
+    return;
+  }
+  for (int e = tid; e < nb * nb; e += 256) {
+    const int i = e / nb, k = e % nb;
+    if (k <= i) Lb[(long long)(j0 + k) * n + j0 + i] = S[i][k];
+  }
+  // inverse of the lower triangle: thread t < 64 forms column t by forward substitution (in LDS)
+  if (tid < 64) {
+    const int t = tid;
+    for (int i = 0; i < 64; ++i) {
+      double2 v = make_double2(0.0, 0.0);
+      if (t < nb && i < nb && i >= t) {
+        double2 acc = make_double2(i == t ? 1.0 : 0.0, 0.0);
+        for (int q = t; q < i; ++q) {   // acc -= S_iq X_qt
+          const double2 a = S[i][q], x = X[q][t];
+          acc.x -= a.x * x.x - a.y * x.y;
+          acc.y -= a.x * x.y + a.y * x.x;
+        }
+        const double2 a = S[i][i];
+        v = make_double2(acc.x / a.x, acc.y / a.x);   // diagonal entries are real
+      }
+      X[i][t] = v;
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < 64 * 64; e += 256) Li[e] = X[e / 64][e % 64];
+  if (tid == 0) st.rank[b] = j0 + nb;
+}
+
+}  // namespace
+
+size_t chol_workspace_bytes(int batch) {
+  return 2 * a256(sizeof(int) * batch) + a256(sizeof(double) * batch) + a256((size_t)64 * 64 * sizeof(double2) * batch);
+}
+
+hipError_t chol_b(int n, int batch, void* A, long long lda, long long bsA, double tol_rel, double2* L, int* piv,
+                  int* rank, void* ws, hipStream_t st) {
+  if (n <= 0 || batch <= 0) return hipSuccess;
+  hipError_t err;
+  char* p = (char*)ws;
+  ChState cs;
+  cs.rank = rank;
+  cs.stop = (int*)p; p += a256(sizeof(int) * batch);
+  p += a256(sizeof(int) * batch);
+  cs.thr = (double*)p; p += a256(sizeof(double) * batch);
+  cs.Linv = (double2*)p;
+  prof::Scope ps(prof::EIG_SMALL, st, 0.0, 16.0 * n * n * batch);
+  hipLaunchKernelGGL(chol_init_kernel, dim3(batch), dim3(256), 0, st, (const double2*)A, n, lda, bsA, tol_rel, L, piv,
+                     cs);
+  if ((err = hipGetLastError()) != hipSuccess) return err;
+  double2* Ab = (double2*)A;
+  for (int j0 = 0; j0 < n; j0 += 64) {
+    hipLaunchKernelGGL(chol_diag_kernel, dim3(batch), dim3(256), 0, st, (const double2*)A, n, lda, bsA, j0, L, cs);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+    const int nb = std::min(64, n - j0), rest = n - j0 - nb;
+    if (rest <= 0) break;
+    // L[j0+nb:, j0:j0+nb] = A[j0+nb:, j0:j0+nb] Linv^H (Linv^H upper triangular)
+    GemmArgs g{};
+    g.M = rest; g.N = nb; g.K = nb; g.batch = batch;
+    g.A = Ab + (long long)(j0 + nb) * lda + j0; g.rsA = lda; g.csA = 1; g.bsA = bsA;
+    g.B = cs.Linv; g.rsB = 1; g.csB = 64; g.bsB = 64 * 64; g.conjB = 1;
+    g.C = L + (long long)j0 * n + j0 + nb; g.rsC = 1; g.csC = n; g.bsC = (long long)n * n;
+    g.alpha_re = 1.0;
+    g.tri = 1;
+    if ((err = gemm(C128_F64ACC, g, st)) != hipSuccess) return err;
+    // A[j0+nb:, j0+nb:] -= L[j0+nb:, j0:j0+nb] L[j0+nb:, j0:j0+nb]^H
+    GemmArgs h{};
+    h.M = rest; h.N = rest; h.K = nb; h.batch = batch;
+    h.A = L + (long long)j0 * n + j0 + nb; h.rsA = 1; h.csA = n; h.bsA = (long long)n * n;
+    h.B = L + (long long)j0 * n + j0 + nb; h.rsB = n; h.csB = 1; h.bsB = (long long)n * n; h.conjB = 1;
+    h.C = Ab + (long long)(j0 + nb) * lda + j0 + nb; h.rsC = lda; h.csC = 1; h.bsC = bsA;
+    h.alpha_re = -1.0; h.beta_re = 1.0;
+    h.herm = 1;
+    if ((err = gemm(C128_F64ACC, h, st)) != hipSuccess) return err;
   }
   return hipSuccess;
 }

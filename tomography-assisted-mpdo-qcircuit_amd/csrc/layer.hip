@@ -99,13 +99,14 @@ thread_local int t_gate_skips = 0;     // gate-time cuts decided by the shifted 
 thread_local int t_gate_tails = 0;     // gate-time cuts decided from the bottom of the spectrum (last layer)
 bool g_gate_skip = true;               // mpdo_debug_layer_gate_skip
 bool g_gate_tail = true;               // mpdo_debug_layer_gate_tail
-// the bottom-of-the-spectrum cut: subspace columns and iteration steps
-constexpr int TAIL_M = 128, TAIL_Q = 5;
+// the bottom-of-the-spectrum cut: subspace columns and iteration steps; below order PARTIAL_MIN the
+// full eigensolver costs less than the partial solvers' launches and readbacks
+constexpr int TAIL_M = 128, TAIL_Q = 5, PARTIAL_MIN = 512;
 // the chi step at a pair bond by its top k = chi eigenpairs: 2k subspace columns, iteration steps
 constexpr int TOP_Q = 4;
 thread_local int t_chi_tops = 0;       // pair-bond chi steps decided that way (last layer)
 bool g_chi_top = true;                 // mpdo_debug_layer_chi_top
-int top_m(int n, int k) { return (k >= 16 && 2 * k + 64 <= n && n > 64 && n <= 1024) ? 2 * k : 0; }
+int top_m(int n, int k) { return (k >= 16 && 2 * k + 64 <= n && n >= PARTIAL_MIN && n <= 1024) ? 2 * k : 0; }
 
 void add_span(const void* p, size_t bytes) {
   if (t_spans && p && bytes) t_spans->push_back(Span{(uintptr_t)p, (uintptr_t)p + bytes});
@@ -485,7 +486,7 @@ struct EigB {
 // top-k solver for k up to kmax
 size_t eig_ws_bytes(int n, int batch, int kmax) {
   size_t b = eigh_gram_workspace_bytes(n, batch);
-  if (n > 64 && 2 * TAIL_M <= n) b = std::max(b, gram_tail_workspace_bytes(n, TAIL_M, batch));
+  if (n >= PARTIAL_MIN && 2 * TAIL_M <= n) b = std::max(b, gram_tail_workspace_bytes(n, TAIL_M, batch));
   const int k = std::min(kmax, (n - 64) / 2);
   if (top_m(n, k)) b = std::max(b, gram_top_workspace_bytes(n, top_m(n, k), batch));
   return b;
@@ -1246,7 +1247,7 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
       // iteration, Rayleigh-Ritz and a certificate; P = I - V_d V_d^H over all nT columns), or,
       // where that declines, from the full eigensolver
       TailResult tail;
-      if (!keeps_all && !dry && !t_rec_gate && use_gram_eig() && len_gate >= nT && g_gate_tail && nT > 64 &&
+      if (!keeps_all && !dry && !t_rec_gate && use_gram_eig() && len_gate >= nT && g_gate_tail && nT >= PARTIAL_MIN &&
           2 * TAIL_M <= nT) {
         if (!fits(eE.ws, (long long)gram_tail_workspace_bytes(nT, TAIL_M, B), 1, "gate-time cut workspace", __LINE__))
           return layer_fail(MPDO_EINVAL, t_oob.c_str());

@@ -118,8 +118,17 @@ hipError_t gram_top_eig(int n, int batch, const double2* G, int k, int m, int q,
 // piv[b][k] the row pivoted at column k, rank[b] the columns produced before the largest remaining
 // diagonal fell to tol_rel * max_i A_ii (later columns zero). `ws`: pchol_workspace_bytes.
 size_t pchol_workspace_bytes(int n, int batch);
+// early_exit: every 128 columns read back whether every matrix has stopped (saves the rest of the
+// work for low-rank inputs at the price of a synchronisation).
 hipError_t pchol_b(int n, int batch, void* A, long long lda, long long bsA, double tol_rel, double2* L, int* piv,
-                   int* rank, void* ws, hipStream_t s);
+                   int* rank, void* ws, hipStream_t s, bool early_exit = true);
+// Unpivoted blocked Cholesky of well-conditioned Hermitian matrices, in pchol_b's output layout
+// with the identity pivot order (piv[b][k] = k): rank[b] = n when every pivot stays above
+// tol_rel * max_i A_ii, else the column where it did not (later columns meaningless). A is
+// destroyed. No host synchronisation. `ws`: chol_workspace_bytes(batch).
+size_t chol_workspace_bytes(int batch);
+hipError_t chol_b(int n, int batch, void* A, long long lda, long long bsA, double tol_rel, double2* L, int* piv,
+                  int* rank, void* ws, hipStream_t s);
 // After pchol_b (same ws, rank): w[b][rank[b] + j], j < n - rank[b], receives the diagonal left in
 // matrix b's Schur complement, sorted descending (the Gram mass below the pivot threshold); n <= 1024.
 hipError_t pchol_tail_eigs(int n, int batch, const void* ws, const int* rank, double* w, long long ldw, hipStream_t s);

@@ -17,7 +17,7 @@
 //     values are the bottom k eigenvalues to within the residual norm: for sigma in the gap after
 //     theta_k, G + tr V_k V_k^H - sigma I positive definite (a rank-k positive update raises
 //     eigenvalue j at most to eigenvalue j + k of G, so lambda_{k+1}(G) > sigma), checked by a
-//     pivoted Cholesky completing all n columns; the decision must clear D* by the residual and
+//     Cholesky factorisation completing all n columns; the decision must clear D* by the residual and
 //     rounding margins on both sides.
 // A chain that fails any step (rank loss, no gap, a decision too close to D*, the certificate)
 // makes the caller run the full eigensolver for the batch. On success the projector onto the kept
@@ -198,7 +198,7 @@ struct TailWS {
   double2 *Gs, *L, *X, *Z, *Q, *T, *Y, *Vr, *GV, *C, *Lc, *Xc, *H, *U;
   double *tr, *sig, *wH, *rn, *disc;
   int *piv, *rank, *pivc, *rankc, *dcnt, *kcnt, *status, *kept;
-  void *pws, *pinv, *ews;
+  void *pws, *pinv, *ews, *cws;
 };
 
 size_t carve_tail(int n, int m, int batch, char* base, TailWS* t) {
@@ -241,6 +241,7 @@ size_t carve_tail(int n, int m, int batch, char* base, TailWS* t) {
   w.pws = take(pchol_workspace_bytes(n, batch));
   w.pinv = take(pchol_inverse_workspace_bytes(n, batch));
   w.ews = take(eigh_workspace_bytes(m, batch));
+  w.cws = take(chol_workspace_bytes(batch));
   if (t) *t = w;
   return off;
 }
@@ -265,7 +266,7 @@ hipError_t cholqr(int n, int m, int batch, const double2* Z, double2* Q, const W
   g.conjA = 1;
   g.herm = 1;
   if ((err = gemm(C128_F64ACC, g, st)) != hipSuccess) return err;
-  if ((err = pchol_b(m, batch, w.C, m, mm, 1e-14, w.Lc, w.pivc, w.rankc, w.pws, st)) != hipSuccess) return err;
+  if ((err = chol_b(m, batch, w.C, m, mm, 1e-14, w.Lc, w.pivc, w.rankc, w.cws, st)) != hipSuccess) return err;
   if ((err = pchol_inverse_b(m, batch, w.Lc, w.pivc, w.rankc, 0, w.Xc, w.pinv, st)) != hipSuccess) return err;
   return gemm(C128_F64ACC, mat(n, m, m, batch, Z, m, 1, nm, w.Xc, m, 1, mm, Q, m, nm), st);
 }
@@ -296,7 +297,7 @@ hipError_t gram_tail_cut(int n, int batch, const double2* G, double eps, int m, 
   hipLaunchKernelGGL(tail_shift_kernel, gnn, dim3(256), 0, st, G, n, 1.0, (const double*)w.tr, 1e-12,
                      (const double*)nullptr, w.Gs);
   if ((err = hipGetLastError()) != hipSuccess) return err;
-  if ((err = pchol_b(n, batch, w.Gs, n, nn, 1e-14, w.L, w.piv, w.rank, w.pws, st)) != hipSuccess) return err;
+  if ((err = pchol_b(n, batch, w.Gs, n, nn, 1e-14, w.L, w.piv, w.rank, w.pws, st, false)) != hipSuccess) return err;
   std::vector<int> hr;
   if ((err = fetch(hr, (const int*)w.rank, batch, st)) != hipSuccess) return err;
   for (int b = 0; b < batch; ++b)
@@ -370,7 +371,7 @@ hipError_t gram_tail_cut(int n, int batch, const double2* G, double eps, int m, 
     g.beta_re = 1.0;
     if ((err = gemm(C128_F64ACC, g, st)) != hipSuccess) return err;
   }
-  if ((err = pchol_b(n, batch, w.Gs, n, nn, 1e-14, w.L, w.piv, w.rank, w.pws, st)) != hipSuccess) return err;
+  if ((err = chol_b(n, batch, w.Gs, n, nn, 1e-14, w.L, w.piv, w.rank, w.cws, st)) != hipSuccess) return err;
   if ((err = fetch(hr, (const int*)w.rank, batch, st)) != hipSuccess) return err;
   for (int b = 0; b < batch; ++b)
     if (hr[b] != n) { res->reason = 5; return hipSuccess; }
@@ -405,7 +406,7 @@ hipError_t gram_tail_cut(int n, int batch, const double2* G, double eps, int m, 
 // projection, then a certificate that the k leading Ritz values are the k largest eigenvalues to
 // within the residual norm: for sigma in the gap below theta_{k-1}, sigma I - G + tr V_k V_k^H
 // positive definite (a rank-k positive update lowers eigenvalue j of -G at most to eigenvalue j + k,
-// so lambda_{k+1}(G) < sigma), checked by a pivoted Cholesky completing all n columns.
+// so lambda_{k+1}(G) < sigma), checked by a Cholesky factorisation completing all n columns.
 // =======================================================================================
 namespace {
 
@@ -481,7 +482,7 @@ struct TopWS {
   double2 *Gs, *L, *Z, *Q, *Y, *Vr, *GV, *C, *Lc, *Xc, *H, *U;
   double *tr, *sig, *wH, *rn;
   int *piv, *rank, *pivc, *rankc, *status;
-  void *pws, *pinv, *ews;
+  void *pws, *pinv, *ews, *cws;
 };
 
 size_t carve_top(int n, int m, int batch, char* base, TopWS* t) {
@@ -518,6 +519,7 @@ size_t carve_top(int n, int m, int batch, char* base, TopWS* t) {
   w.pws = take(pchol_workspace_bytes(n, batch));
   w.pinv = take(pchol_inverse_workspace_bytes(m, batch));
   w.ews = take(eigh_gram_workspace_bytes(m, batch));
+  w.cws = take(chol_workspace_bytes(batch));
   if (t) *t = w;
   return off;
 }
@@ -590,7 +592,7 @@ hipError_t gram_top_eig(int n, int batch, const double2* G, int k, int m, int q,
     g.beta_re = 1.0;
     if ((err = gemm(C128_F64ACC, g, st)) != hipSuccess) return err;
   }
-  if ((err = pchol_b(n, batch, t.Gs, n, nn, 1e-14, t.L, t.piv, t.rank, t.pws, st)) != hipSuccess) return err;
+  if ((err = chol_b(n, batch, t.Gs, n, nn, 1e-14, t.L, t.piv, t.rank, t.cws, st)) != hipSuccess) return err;
   if ((err = fetch(hr, (const int*)t.rank, batch, st)) != hipSuccess) return err;
   for (int b = 0; b < batch; ++b)
     if (hr[b] != n) { res->reason = 5; return hipSuccess; }

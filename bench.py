@@ -113,10 +113,14 @@ def kernel_trace_summary(path, top=24):
         span = win[-1][1] - win[0][0]
         g = [v for k, v in agg.items() if 'gemm_kernel' in k]
         gl, gt = sum(v[0] for v in g), sum(v[1] for v in g)
+        g3 = [v for k, v in agg.items() if 'gemm_kernel' in k and k.rstrip().endswith('true>')]
+        g3l, g3t = sum(v[0] for v in g3), sum(v[1] for v in g3)
         out += [f'### {title}', '',
                 f'{len(win)} kernels, {tot / 1e6:.1f} ms inside kernels, {span / 1e6:.1f} ms from the first start '
                 f'to the last end. All `mpdo::gemm_kernel` instantiations: {gl} launches, {gt / 1e6:.1f} ms, '
-                f'average {gt / max(gl, 1) / 1e6:.4f} ms per launch.', '',
+                f'average {gt / max(gl, 1) / 1e6:.4f} ms per launch; the fp64-MFMA ones {gl - g3l} launches, '
+                f'average {(gt - g3t) / max(gl - g3l, 1) / 1e6:.4f} ms; the 3M ones {g3l} launches, average '
+                f'{g3t / max(g3l, 1) / 1e6:.4f} ms.', '',
                 '| kernel | launches | total ms | avg ms | % of kernel time |', '|---|---|---|---|---|']
         items = sorted(agg.items(), key=lambda kv: -kv[1][1])
         for k, (c, t) in items[:top]:
@@ -130,13 +134,52 @@ def kernel_trace_summary(path, top=24):
     return '\n'.join(out)
 
 
-def _traffic_per_launch():
-    """The measured HBM bytes per GEMM launch recorded in TRAFFIC_FILE (line
-    `traffic_bytes_per_launch = N`), or None when that measurement is absent."""
+def counter_summary(path):
+    """Markdown summary of a rocprofv3 --pmc run of this benchmark (CSV: Dispatch_Id, Kernel_Name,
+    Counter_Name, Counter_Value): for each window between consecutive pairs of marker launches (with
+    --copy-calibration the first window holds the calibration copies), the mean value of every
+    counter per dispatch of each kernel, and over all mpdo::gemm_kernel dispatches."""
+    import csv
+    disp = {}
+    with open(path, newline='') as f:
+        rd = csv.DictReader(f)
+        cols = {c.lower(): c for c in rd.fieldnames}
+        kd, kn, cn, cv = cols['dispatch_id'], cols['kernel_name'], cols['counter_name'], cols['counter_value']
+        for r in rd:
+            d = disp.setdefault(int(r[kd]), {'name': r[kn], 'c': {}})
+            d['c'][r[cn]] = d['c'].get(r[cn], 0.0) + float(r[cv])
+    order = sorted(disp)
+    marks = [i for i, k in enumerate(order) if MARKER in disp[k]['name']]
+    out = [f'Counter run `{os.path.basename(path)}`: {len(order)} dispatches, {len(marks)} markers.', '']
+    for w in range(len(marks) // 2):
+        win = [disp[k] for k in order[marks[2 * w] + 1:marks[2 * w + 1]]]
+        agg = {}
+        for d in win:
+            name = d['name'].replace('void ', '').replace('(anonymous namespace)::', '').split('(')[0]
+            if 'gemm_kernel' in name:   # the 3M instantiations end in `true>`
+                name = ('mpdo::gemm_kernel, 3M on fp32 MFMA' if name.rstrip().endswith('true>')
+                        else 'mpdo::gemm_kernel, fp64 MFMA')
+            a = agg.setdefault(name, [0, {}])
+            a[0] += 1
+            for c, v in d['c'].items():
+                a[1][c] = a[1].get(c, 0.0) + v
+        counters = sorted({c for a in agg.values() for c in a[1]})
+        out += [f'### window {w + 1}: {len(win)} dispatches', '',
+                '| kernel | dispatches | ' + ' | '.join(f'mean {c} per dispatch' for c in counters) + ' |',
+                '|---|---|' + '---|' * len(counters)]
+        for name, (n, tot) in sorted(agg.items(), key=lambda kv: -kv[1][0])[:12]:
+            out.append(f'| `{name[:90]}` | {n} | ' + ' | '.join(f'{tot.get(c, 0.0) / n:.1f}' for c in counters) + ' |')
+        out.append('')
+    return '\n'.join(out)
+
+
+def _traffic_per_launch(group='fp64'):
+    """The measured HBM bytes per GEMM launch of one group (fp64 MFMA, or 3m) recorded in
+    TRAFFIC_FILE (line `traffic_bytes_per_launch_<group> = N`), or None when absent."""
     import re
     try:
         with open(TRAFFIC_FILE) as f:
-            m = re.search(r'traffic_bytes_per_launch = ([0-9.e+]+)', f.read())
+            m = re.search(r'traffic_bytes_per_launch_' + group + r' = ([0-9.e+]+)', f.read())
     except OSError:
         return None
     return float(m.group(1)) if m else None
@@ -306,7 +349,7 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-threads', type=int, default=None,
                     help='threads of the CPU oracle sample (default: the cores this process may run on)')
-    ap.add_argument('--math', choices=['fp64acc', 'fp32x3m'], default='fp64acc',
+    ap.add_argument('--math', choices=['fp64acc', 'fp32x3m'], default='fp32x3m',
                     help='contractions whose results are complex64 site data: fp64 MFMA accumulation, or three real '
                          'products per complex product accumulated in fp32 (3M; Gram matrices, QR/LQ factors and '
                          'eigensolves stay fp64)')
@@ -323,12 +366,18 @@ def main():
                     help='before the warmup, run torch.neg over 1 GiB of float32 three times (1 GiB read and '
                          '1 GiB written per launch): the reference launch that scales the HBM counters of a '
                          'rocprofv3 --pmc run of this benchmark')
+    ap.add_argument('--counter-summary', metavar='CSV',
+                    help='print a markdown summary of a rocprofv3 --pmc run (counter_collection CSV) of this '
+                         'benchmark, window by window between the markers, then exit (no GPU work)')
     ap.add_argument('--kernel-trace-summary', metavar='CSV',
                     help='print a markdown summary of the timed steps and the profiled pass of a rocprofv3 kernel '
                          'trace (CSV) of this benchmark, then exit (no GPU work)')
     args = ap.parse_args()
     if args.kernel_trace_summary:
         print(kernel_trace_summary(args.kernel_trace_summary))
+        return
+    if args.counter_summary:
+        print(counter_summary(args.counter_summary))
         return
     if args.gpus != 1:
         print(f'bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). '
@@ -348,9 +397,11 @@ def main():
     if args.copy_calibration:
         src = torch.ones(CALIB_BYTES // 4, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
+        _trace_marker()
         for _ in range(3):
             torch.neg(src, out=dst)
         torch.cuda.synchronize()
+        _trace_marker()
         del src, dst
         _log(f'copy calibration: 3 launches of torch.neg, {CALIB_BYTES} bytes read and {CALIB_BYTES} bytes '
              'written each')
@@ -389,6 +440,22 @@ def main():
     value = B * args.steps / T
     _log(f'timed: {T:.2f} s, {value:.3f} circuit-layers/s')
     del sites
+    fp64 = None
+    if math_mode != 0 and not args.fused_only:
+        # the same timed layers with every site contraction accumulated in fp64 (math mode 0)
+        lib.mpdo_set_math_mode(0)
+        s64 = [t.clone() for t in saved]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            s64 = step(args.warmup + k, s64)
+        torch.cuda.synchronize()
+        T64 = time.perf_counter() - t0
+        del s64
+        lib.mpdo_set_math_mode(math_mode)
+        fp64 = {'value': round(B * args.steps / T64, 6), 'ms_per_step': round(T64 / args.steps * 1e3, 3),
+                'math': 'every site contraction with fp64 MFMA accumulation (mpdo_set_math_mode(0)), same layers and input'}
+        _log(f'fp64 accumulation: {T64:.2f} s, {fp64["value"]:.3f} circuit-layers/s')
 
     roofline, others, plain, accuracy, cpu = None, None, None, None, None
     if not args.fused_only:
@@ -417,14 +484,35 @@ def main():
              '; '.join(f'{sh}: {c}, {t:.0f}, {f / (t * 1e-3) / 1e12:.1f}' for sh, c, t, f in shapes[:14]))
         total_ms = max(sum(v[1] for v in prof.values()), 1e-9)
         n_l, ms, fl, by = prof['gemm']
-        achieved = fl / (ms * 1e-3) / 1e12
-        roofline = {'bound': 'mfma', 'achieved': round(achieved, 3), 'peak': PEAK_TFLOPS['fp64'], 'unit': 'TFLOP/s',
-                    'frac': round(achieved / PEAK_TFLOPS['fp64'], 4), 'traffic': _traffic_per_launch(),
-                    'kernel': 'mpdo::gemm_kernel (complex GEMM, fp64 MFMA accumulate)', 'launches': n_l,
-                    'avg_launch_ms': round(ms / max(n_l, 1), 4), 'flops_per_launch': fl / max(n_l, 1),
-                    'operand_bytes_per_launch': by / max(n_l, 1),
-                    'share_of_kernel_time': round(ms / total_ms, 3),
-                    'classes_ms': {k: round(v[1], 1) for k, v in prof.items()}}
+        # the GEMM launches by arithmetic: fp64 MFMA (complex products as four real ones) and the 3M
+        # products on fp32 MFMA (precision mode 5; three real products, 6 M N K executed flops of the
+        # 8 M N K a complex product counts)
+        groups = {}
+        for sh, c, t, f in shapes:
+            g3 = sh[4] % 10 == 5
+            a = groups.setdefault('3m' if g3 else 'fp64', [0, 0.0, 0.0])
+            a[0] += c
+            a[1] += t
+            a[2] += f * (0.75 if g3 else 1.0)
+        gemm_groups = {}
+        for key, (c, t, f) in groups.items():
+            peak = PEAK_TFLOPS['fp32' if key == '3m' else 'fp64']
+            a = f / (t * 1e-3) / 1e12
+            gemm_groups[key] = {'kernel': ('mpdo::gemm_kernel, 3M products on fp32 MFMA (site contractions)' if key == '3m'
+                                           else 'mpdo::gemm_kernel, fp64 MFMA (Grams, factors, fp64-mode contractions)'),
+                                'bound': 'mfma', 'achieved': round(a, 3), 'peak': peak, 'unit': 'TFLOP/s',
+                                'frac': round(a / peak, 4), 'launches': c, 'avg_launch_ms': round(t / max(c, 1), 4),
+                                'executed_flops_per_launch': f / max(c, 1), 'share_of_kernel_time': round(t / total_ms, 3),
+                                'traffic': _traffic_per_launch(key)}
+        dom = max(gemm_groups, key=lambda k: gemm_groups[k]['share_of_kernel_time'])
+        roofline = dict(gemm_groups[dom])
+        roofline.update({'traffic_scope': 'HBM bytes per launch of this group over the timed steps (FETCH_SIZE and '
+                                          'WRITE_SIZE counter passes scaled by the copy calibration)',
+                         'gemm_all_launches': {'launches': n_l, 'avg_launch_ms': round(ms / max(n_l, 1), 4),
+                                               'share_of_kernel_time': round(ms / total_ms, 3),
+                                               'operand_bytes_per_launch': by / max(n_l, 1)},
+                         'gemm_groups': gemm_groups,
+                         'classes_ms': {k: round(v[1], 1) for k, v in prof.items()}})
         names = {'eig_update': 'bj_update (two-sided 64x64 block-pair rotation of A, fp64 MFMA)',
                  'eig_rotate': 'bj_rotate_v_direct (eigenvector tile update V W, fp64 MFMA)',
                  'eig_solve': 'bj_solve (64x64 subproblem Jacobi sweep in LDS, fp64 vector ALU)'}
@@ -531,7 +619,8 @@ def main():
                        'n_qubits': N_QUBITS, 'chi': CHI, 'kappa': KAPPA, 'copies': B, 'global_batch': B,
                        'parallelism': 'none (one GPU)'},
             'roofline': roofline, 'roofline_block_jacobi': others, 'plain': plain,
-            'batched_vs_plain': plain['batched_vs_plain'] if plain else None, 'accuracy': accuracy,
+            'batched_vs_plain': plain['batched_vs_plain'] if plain else None, 'fp64_accumulate': fp64,
+            'accuracy': accuracy,
             'uneven_batch': uneven, 'differentiation': diff, 'cpu_baseline': cpu}
     print(json.dumps(line), flush=True)
 

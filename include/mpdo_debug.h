@@ -67,6 +67,11 @@ int mpdo_debug_gemm_tiles(int v);
 /* Threads per workgroup of the block-Jacobi subproblem kernel (256, 512 or 1024); returns the
  * previous choice (any other v only reads it). */
 int mpdo_debug_bj_threads(int v);
+/* Block-Jacobi rounds with at most v subproblems (block pairs x matrices) solve them with the
+ * rotation product accumulated in LDS next to the subproblem (one workgroup per CU, one pass);
+ * larger rounds log the rotations and replay them (two workgroups per CU). Default 256; returns
+ * the previous value (v < 0 only reads it). */
+int mpdo_debug_bj_fused_max(int v);
 
 /* The GEMMs the last dry run of the fused layer would launch, in order: out [6 n] = source line,
  * M, N, K, batch, precision mode; returns n (fills at most cap ints). */
@@ -85,8 +90,8 @@ int mpdo_debug_layer_kept(int* out, int cap);
 int mpdo_debug_layer_gate_eigs(int record, double* out, int cap);
 int mpdo_debug_layer_chi_eigs(double* out, int cap);
 
-/* The fused layer's gate-time cut decides "every direction kept in every chain" by one pivoted
- * Cholesky of G - D* I (D* the largest eigenvalue the absolute-error rule may drop) and then needs
+/* The fused layer's gate-time cut decides "every direction kept in every chain" by one Cholesky
+ * factorisation of G - D* I (D* the largest eigenvalue the absolute-error rule may drop) and then needs
  * no eigensolver; on = 0 always runs the eigensolver, on < 0 leaves the switch. Returns the
  * previous setting. mpdo_debug_layer_gate_skips: how many pairs of the last fused layer on this
  * thread were decided that way. */

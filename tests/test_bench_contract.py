@@ -18,15 +18,21 @@ def test_refuses_several_gpus():
 
 
 def test_traffic_is_the_profile_figure():
+    """The roofline's HBM traffic per GEMM launch, group by group, is the figure the committed
+    counter profile states, and that figure is its table's raw counters times the calibration
+    scales (FETCH_SIZE x 2, WRITE_SIZE x 1, KiB)."""
     import bench
     with open(bench.TRAFFIC_FILE) as f:
         text = f.read()
-    want = float(re.search(r'traffic_bytes_per_launch = ([0-9.e+]+)', text).group(1))
-    assert bench._traffic_per_launch() == want
-    # FETCH_SIZE and WRITE_SIZE rows of the table add up to it
-    rows = re.findall(r'\| (FETCH_SIZE|WRITE_SIZE) \| [0-9.]+ \| [0-9.]+ \| ([0-9.e+]+) \|', text)
-    assert len(rows) == 2
-    assert abs(sum(float(v) for _, v in rows) - want) <= 1e-4 * want
+    rows = {g: (float(fe), float(wr), float(by)) for g, fe, wr, by in re.findall(
+        r'\| (fp64 MFMA|3M on fp32 MFMA) \| [0-9]+ \| ([0-9.]+) \| ([0-9.]+) \| ([0-9.e+]+) \|', text)}
+    assert set(rows) == {'fp64 MFMA', '3M on fp32 MFMA'}
+    for key, row in (('fp64', 'fp64 MFMA'), ('3m', '3M on fp32 MFMA')):
+        want = float(re.search(r'traffic_bytes_per_launch_' + key + r' = ([0-9.e+]+)', text).group(1))
+        assert bench._traffic_per_launch(key) == want
+        fe, wr, by = rows[row]
+        assert abs(by - want) <= 1e-4 * want
+        assert abs((2.0 * fe + wr) * 1024 - want) <= 1e-4 * want
 
 
 def test_timed_workload_family():

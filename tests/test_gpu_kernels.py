@@ -100,10 +100,13 @@ def test_gemm_mixed_operands(cuda, prec, inA, inB, M, N, K, tA, tB):
     assert err < (1e-13 if prec in (1, 2) else 1e-6), (prec, inA, inB, M, N, K, err)
 
 
+@pytest.mark.parametrize('prec', [4, 0, 5])
 @pytest.mark.parametrize('M,N,K,tA', [(16, 4096, 256, True), (32, 640, 40, False), (5, 513, 17, True),
                                       (31, 1000, 300, False)])
-def test_gemm_few_rows(cuda, M, N, K, tA):
-    """At most 32 output rows and a wide output: the 32 x 128 tile shape; vs numpy."""
+def test_gemm_few_rows(cuda, prec, M, N, K, tA):
+    """At most 32 output rows and a wide output: the 32 x 128 tile shape, with fp64 accumulation
+    (mode 4) and fp32 accumulation of four (mode 0) and three (mode 5, 3M) real products; vs numpy
+    (fp32 bars as for the general test: 3e-6 and 6e-6 x sqrt(K) of the largest element)."""
     from mpdo_amd import _lib
     rng = np.random.default_rng(M * 11 + N + K)
     batch = 3
@@ -111,12 +114,13 @@ def test_gemm_few_rows(cuda, M, N, K, tA):
     B = _crand(rng, batch, K, N).astype(np.complex64)
     C = torch.zeros(batch, M, N, dtype=torch.complex64, device=cuda)
     sa = (1, M, M * K) if tA else (K, 1, M * K)
-    _lib.debug_gemm(4, torch.from_numpy(A).to(cuda), torch.from_numpy(B).to(cuda), C, M, N, K, sa, (N, 1, K * N),
+    _lib.debug_gemm(prec, torch.from_numpy(A).to(cuda), torch.from_numpy(B).to(cuda), C, M, N, K, sa, (N, 1, K * N),
                     (N, 1, M * N), batch=batch)
     opA = np.swapaxes(A, 1, 2) if tA else A
     ref = opA.astype(np.complex128) @ B.astype(np.complex128)
     got = C.cpu().numpy()
-    assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-6
+    tol = {4: 1e-6, 0: 3e-6 * np.sqrt(K), 5: 6e-6 * np.sqrt(K)}[prec]
+    assert np.abs(got - ref).max() / np.abs(ref).max() < tol
 
 
 @pytest.mark.parametrize('tri', [1, 2])
@@ -190,7 +194,8 @@ def test_eigh(cuda, n):
 def test_eigh_block_threads(cuda, n):
     """The blocked solver's subproblem kernel strides its rounds, its rotation log and the replay
     of that log into W over any workgroup size: 256, 512 and 1024 threads give the same
-    eigenpairs to the bit (each element sees the same operations in the same order)."""
+    eigenpairs to the bit (each element sees the same operations in the same order), and so does
+    the variant that accumulates W in LDS during the sweep."""
     from mpdo_amd import _lib
     lib = _lib.load()
     rng = np.random.default_rng(n)
@@ -198,14 +203,21 @@ def test_eigh_block_threads(cuda, n):
     H = X @ np.swapaxes(X.conj(), 1, 2)
     outs = []
     old = lib.mpdo_debug_bj_threads(0)
+    old_fused = lib.mpdo_debug_bj_fused_max(0)   # the logged sweep and its replay
     try:
         for t in (256, 512, 1024):
             lib.mpdo_debug_bj_threads(t)
             w, V, _ = _lib.debug_eigh(torch.from_numpy(H.copy()).to(cuda))
             outs.append((w.cpu().numpy(), V.cpu().numpy()))
+        # the rotation product accumulated in LDS next to the subproblem: the same operations on
+        # every element in the same order
+        lib.mpdo_debug_bj_fused_max(1 << 20)
+        w, V, _ = _lib.debug_eigh(torch.from_numpy(H.copy()).to(cuda))
+        fused = (w.cpu().numpy(), V.cpu().numpy())
     finally:
         lib.mpdo_debug_bj_threads(old)
-    for w, V in outs[:2]:
+        lib.mpdo_debug_bj_fused_max(old_fused)
+    for w, V in outs[:2] + [fused]:
         assert np.array_equal(w, outs[2][0]) and np.array_equal(V, outs[2][1])
     for b in range(2):
         ref = np.sort(np.linalg.eigvalsh(H[b]))[::-1]

@@ -445,25 +445,24 @@ __global__ __launch_bounds__(256) void chol_diag_kernel(const double2* A, int n,
     const int i = e / nb, k = e % nb;
     if (k <= i) Lb[(long long)(j0 + k) * n + j0 + i] = S[i][k];
   }
-  // inverse of the lower triangle: thread t < 64 forms column t by forward substitution (in LDS)
-  if (tid < 64) {
-    const int t = tid;
-    for (int i = 0; i < 64; ++i) {
-      double2 v = make_double2(0.0, 0.0);
-      if (t < nb && i < nb && i >= t) {
-        double2 acc = make_double2(i == t ? 1.0 : 0.0, 0.0);
-        for (int q = t; q < i; ++q) {   // acc -= S_iq X_qt
-          const double2 a = S[i][q], x = X[q][t];
-          acc.x -= a.x * x.x - a.y * x.y;
-          acc.y -= a.x * x.y + a.y * x.x;
-        }
-        const double2 a = S[i][i];
-        v = make_double2(acc.x / a.x, acc.y / a.x);   // diagonal entries are real
-      }
-      X[i][t] = v;
-    }
-  }
+  // inverse of the lower triangle, row by row: X starts as the identity (the right-hand sides);
+  // row i is final once divided by S_ii, then its multiples leave the rows below it (one rank-1
+  // update per row, spread over the workgroup)
+  for (int e = tid; e < 64 * 64; e += 256) X[e / 64][e % 64] = make_double2((e / 64 == e % 64) ? 1.0 : 0.0, 0.0);
   __syncthreads();
+  for (int i = 0; i < nb; ++i) {
+    const double dii = S[i][i].x;   // real and positive
+    if (tid <= i) X[i][tid] = make_double2(X[i][tid].x / dii, X[i][tid].y / dii);
+    __syncthreads();
+    const int rows = nb - i - 1, cols = i + 1;
+    for (int e = tid; e < rows * cols; e += 256) {   // X[r][t] -= S[r][i] X[i][t], r > i, t <= i
+      const int r = i + 1 + e / cols, t = e % cols;
+      const double2 a = S[r][i], x = X[i][t];
+      X[r][t].x -= a.x * x.x - a.y * x.y;
+      X[r][t].y -= a.x * x.y + a.y * x.x;
+    }
+    __syncthreads();
+  }
   for (int e = tid; e < 64 * 64; e += 256) Li[e] = X[e / 64][e % 64];
   if (tid == 0) st.rank[b] = j0 + nb;
 }

@@ -270,6 +270,8 @@ __global__ __launch_bounds__(256) void eigh_small_kernel(int n, double2* Ag, lon
 // ---------------------------------------------------------------------------------------
 // threads per bj_solve workgroup (mpdo_debug_bj_threads)
 int g_bj_threads = 1024;
+// rounds with at most this many subproblems solve them with W in LDS (bj_solve_w): one per CU
+int g_bj_fused_max = 256;
 
 struct BJ {
   int n, N, batch;
@@ -369,6 +371,58 @@ __global__ __launch_bounds__(1024) void bj_solve(BJ s, int r) {
   for (int rr = 0; rr < NS - 1; ++rr) replay_round(A, rl + rr * (NS / 2), NS, rr);
   double2* Wb = s.W + ((long long)b * (P / 2) + pi) * NS * NS;
   for (int e = tid; e < NS * NS; e += blockDim.x) Wb[e] = A[e / NS][e % NS];
+  if (tid == 0) {
+    if (cnt > 0) atomicOr(&s.flag[b], 1);
+    s.ident[b * (P / 2) + pi] = cnt == 0;
+  }
+}
+
+// The same subproblem sweep with W accumulated in LDS alongside A, round by round (two 64x64
+// arrays, 133 KB of LDS: one workgroup per CU), for launches with no more subproblems than CUs:
+// one pass of 63 rounds instead of the logged sweep and its replay.
+__global__ __launch_bounds__(1024) void bj_solve_w(BJ s, int r) {
+  __shared__ double2 A[NS][LD];
+  __shared__ double2 W[NS][LD];
+  __shared__ Rot rot[NS / 2];
+  __shared__ int cnt, need;
+  const int P = s.N / 32, pi = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+  int I, J;
+  block_pair(P, r, pi, I, J);
+  const double2* Ab = s.A + (long long)b * s.N * s.N;
+  for (int q = tid; q < NS * NS; q += blockDim.x) {
+    const int sb = q >> 10, e = q & 1023, sr = sb >> 1, sc = sb & 1;
+    const int R = sr ? J : I, C = sc ? J : I, u = e >> 5, v = e & 31;
+    if (R >= C) {
+      A[sr * 32 + u][sc * 32 + v] = Ab[(long long)(R * 32 + u) * s.N + C * 32 + v];
+    } else {
+      const double2 t = Ab[(long long)(C * 32 + u) * s.N + R * 32 + v];
+      A[sr * 32 + v][sc * 32 + u] = make_double2(t.x, -t.y);
+    }
+  }
+  if (tid == 0) {
+    cnt = 0;
+    need = 0;
+  }
+  __syncthreads();
+  const double fl = s.floor_abs[b];
+  for (int e = tid; e < NS * NS; e += blockDim.x) {
+    const int i = e / NS, j = e % NS;
+    if (j > i) {
+      const double2 c = A[i][j];
+      const double ac = sqrt(c.x * c.x + c.y * c.y);
+      if (ac > fl && ac > REL_TOL * sqrt(fabs(A[i][i].x * A[j][j].x))) need = 1;
+    }
+  }
+  __syncthreads();
+  if (need == 0) {
+    if (tid == 0) s.ident[b * (P / 2) + pi] = 1;
+    return;
+  }
+  for (int e = tid; e < NS * NS; e += blockDim.x) W[e / NS][e % NS] = make_double2(e / NS == e % NS ? 1.0 : 0.0, 0);
+  __syncthreads();
+  for (int rr = 0; rr < NS - 1; ++rr) jacobi_round_blk(A, W, rot, NS, rr, fl, &cnt);
+  double2* Wb = s.W + ((long long)b * (P / 2) + pi) * NS * NS;
+  for (int e = tid; e < NS * NS; e += blockDim.x) Wb[e] = W[e / NS][e % NS];
   if (tid == 0) {
     if (cnt > 0) atomicOr(&s.flag[b], 1);
     s.ident[b * (P / 2) + pi] = cnt == 0;
@@ -668,7 +722,10 @@ hipError_t eigh(int n, int batch, void* A, long long lda, long long bsA, double*
     for (int r = 0; r < P - 1; ++r) {
       {
         prof::Scope ps(prof::EIG_SOLVE, st, solve_flops, solve_bytes);
-        hipLaunchKernelGGL(bj_solve, dim3(P / 2, batch), dim3(g_bj_threads), 0, st, s, r);
+        if ((long long)(P / 2) * batch <= g_bj_fused_max)
+          hipLaunchKernelGGL(bj_solve_w, dim3(P / 2, batch), dim3(1024), 0, st, s, r);
+        else
+          hipLaunchKernelGGL(bj_solve, dim3(P / 2, batch), dim3(g_bj_threads), 0, st, s, r);
       }
       {   // A <- J^H A J block pair by block pair (Hermitian: lower triangle + mirror)
         prof::Scope ps(prof::EIG_UPDATE, st, tiles_sym * 2 * tile_flops, tiles_sym * tile_bytes);
@@ -725,6 +782,12 @@ hipError_t eigh(int n, int batch, void* A, long long lda, long long bsA, double*
 
 }  // namespace mpdo
 
+extern "C" int mpdo_debug_bj_fused_max(int v) {
+  const int old = mpdo::g_bj_fused_max;
+  if (v >= 0) mpdo::g_bj_fused_max = v;
+  return old;
+}
+
 extern "C" int mpdo_debug_bj_threads(int v) {
   const int old = mpdo::g_bj_threads;
   if (v == 256 || v == 512 || v == 1024) mpdo::g_bj_threads = v;
@@ -749,14 +812,13 @@ __global__ void pad_eigs_kernel(const double* wm, int r, double* w, long long ld
   for (int j = threadIdx.x; j < n; j += blockDim.x) w[(long long)b * ldw + j] = j < r ? wm[(long long)b * r + j] : 0.0;
 }
 
-// Gs[b] = G[b] - shift_b I with shift_b = D*_b + margin: D*_b = tr G - (sqrt(tr G) - eps)^2 is
-// the largest eigenvalue the absolute-error cut may drop (te[-1] - sqrt(te[-1]^2 - lambda) <= eps),
-// margin = 1e-9 D* + 1e-13 tr G keeps rounding on the side of running the eigensolver
-__global__ void shift_gram_kernel(const double2* G, long long lda, long long bsA, int n, double eps, double2* Gs) {
+// shift[b] = D*_b + margin: D*_b = tr G - (sqrt(tr G) - eps)^2 is the largest eigenvalue the
+// absolute-error cut may drop (te[-1] - sqrt(te[-1]^2 - lambda) <= eps), margin = 1e-9 D* +
+// 1e-13 tr G keeps rounding on the side of running the eigensolver
+__global__ void gate_shift_kernel(const double2* G, long long lda, long long bsA, int n, double eps, double* shift) {
   __shared__ double red[256];
   const int b = blockIdx.x;
   const double2* Gb = G + (long long)b * bsA;
-  double2* Sb = Gs + (long long)b * n * n;
   double t = 0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) t += Gb[(long long)i * lda + i].x;
   red[threadIdx.x] = t;
@@ -765,14 +827,26 @@ __global__ void shift_gram_kernel(const double2* G, long long lda, long long bsA
     if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
     __syncthreads();
   }
-  const double tr = red[0];
-  const double rt = sqrt(fmax(tr, 0.0));
-  const double dstar = rt > eps ? tr - (rt - eps) * (rt - eps) : tr;
-  const double shift = dstar * (1.0 + 1e-9) + 1e-13 * tr;
-  for (long long e = threadIdx.x; e < (long long)n * n; e += blockDim.x) {
-    const int i = (int)(e / n), j = (int)(e % n);
-    double2 v = Gb[(long long)i * lda + j];
-    if (i == j) v.x -= shift;
+  if (threadIdx.x == 0) {
+    const double tr = red[0];
+    const double rt = sqrt(fmax(tr, 0.0));
+    const double dstar = rt > eps ? tr - (rt - eps) * (rt - eps) : tr;
+    shift[b] = dstar * (1.0 + 1e-9) + 1e-13 * tr;
+  }
+}
+
+// Gs[b] = G[b] - shift[b] I (n x n, contiguous)
+__global__ void shift_gram_kernel(const double2* G, long long lda, long long bsA, int n, const double* shift,
+                                  double2* Gs) {
+  const int b = blockIdx.y;
+  const double2* Gb = G + (long long)b * bsA;
+  double2* Sb = Gs + (long long)b * n * n;
+  const double sh = shift[b];
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < (long long)n * n;
+       e += (long long)gridDim.x * blockDim.x) {
+    const long long i = e / n, j = e - i * n;
+    double2 v = Gb[i * lda + j];
+    if (i == j) v.x -= sh;
     Sb[e] = v;
   }
 }
@@ -789,16 +863,21 @@ hipError_t gram_keeps_all(int n, int batch, const void* A, long long lda, long l
   p += mat;                                     // (eigh_gram's L)
   double2* Gs = (double2*)p; p += mat;          // (M)
   double2* L = (double2*)p; p += mat;           // (U)
-  p += mat;                                     // (Y)
-  p += align256((size_t)n * sizeof(double) * batch);
+  void* cws = p; p += mat;                      // (Y): the Cholesky's state, n > 64 so it fits
+  double* shift = (double*)p; p += align256((size_t)n * sizeof(double) * batch);
   int* piv = (int*)p; p += align256((size_t)n * sizeof(int) * batch);
   int* rank = (int*)p; p += align256(sizeof(int) * batch);
-  void* pws = p;
-  hipLaunchKernelGGL(shift_gram_kernel, dim3(batch), dim3(256), 0, st, (const double2*)A, lda, bsA, n, eps, Gs);
+  hipLaunchKernelGGL(gate_shift_kernel, dim3(batch), dim3(256), 0, st, (const double2*)A, lda, bsA, n, eps, shift);
+  {
+    const long long nn = (long long)n * n;
+    const int blocks = (int)std::min<long long>((nn + 255) / 256, 2048);
+    hipLaunchKernelGGL(shift_gram_kernel, dim3(blocks, batch), dim3(256), 0, st, (const double2*)A, lda, bsA, n,
+                       (const double*)shift, Gs);
+  }
   if ((err = hipGetLastError()) != hipSuccess) return err;
-  // positive definite <=> the pivoted Cholesky completes all n columns with pivots above 1e-14 of
-  // the largest diagonal
-  if ((err = pchol_b(n, batch, Gs, n, (long long)n * n, 1e-14, L, piv, rank, pws, st)) != hipSuccess) return err;
+  // positive definite <=> the (unpivoted) Cholesky completes all n columns with pivots above 1e-14
+  // of the largest diagonal
+  if ((err = chol_b(n, batch, Gs, n, (long long)n * n, 1e-14, L, piv, rank, cws, st)) != hipSuccess) return err;
   std::vector<int> hr(batch);
   if ((err = hipMemcpyAsync(hr.data(), rank, sizeof(int) * batch, hipMemcpyDeviceToHost, st)) != hipSuccess)
     return err;

@@ -390,8 +390,13 @@ hipError_t gemm(Prec p, const GemmArgs& g, hipStream_t s) {
                        outb * mn * ((g.beta_re != 0 || g.beta_im != 0) ? 2 : 1)),
                  shape);
   switch (p) {
-    case C64_F32ACC: return launch<float2, float2, float, float2, 128, 64, 16>(g, s);
-    case C64_F32ACC_3M: return launch<float2, float2, float, float2, 128, 64, 16, 1, true>(g, s);
+    case C64_F32ACC:
+      if (g.M <= 32 && g.N >= 512 && !g.herm && !g.tri) return launch<float2, float2, float, float2, 32, 128, 16>(g, s);
+      return launch<float2, float2, float, float2, 128, 64, 16>(g, s);
+    case C64_F32ACC_3M:   // at most 32 rows: 32 x 128 tiles, as for the fp64 modes
+      if (g.M <= 32 && g.N >= 512 && !g.herm && !g.tri)
+        return launch<float2, float2, float, float2, 32, 128, 16, 1, true>(g, s);
+      return launch<float2, float2, float, float2, 128, 64, 16, 1, true>(g, s);
     case C64_F64ACC:
     case C128_F64ACC: return launch_f64_mixed<double2>(a128, b128, g, s);
     case C128_IN_C64_OUT:

@@ -1239,7 +1239,7 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
       LCK(k_kraus_gram(Pg, dG + (pi - 1) * 16 * K2, K2, p1t, G, B, s, __LINE__));
       // the cut keeps every direction in every chain (then no eigenvector is needed: P = I, nothing
       // is discarded) exactly when G - D* I is positive definite, D* the largest eigenvalue the
-      // absolute-error rule may drop: one shifted pivoted Cholesky instead of the eigensolver
+      // absolute-error rule may drop: one Cholesky factorisation of the shifted matrix instead of the eigensolver
       bool keeps_all = false;
       if (!dry && !t_rec_gate && use_gram_eig() && len_gate >= nT && g_gate_skip)
         LCK(gram_keeps_all(nT, B, G, nT, (long long)nT * nT, GATE_EPS, eE.ws, s, &keeps_all));

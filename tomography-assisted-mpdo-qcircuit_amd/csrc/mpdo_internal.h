@@ -75,7 +75,7 @@ hipError_t eigh_gram(int n, int batch, void* A, long long lda, long long bsA,
 // values for the first idx with te[-1] - te[idx] <= eps; decompositions.py:121-131) keeps every
 // one of the n directions of each Gram matrix A[b]: *all = true when, for every b, A[b] - D*_b I is
 // positive definite, D*_b = tr A[b] - (sqrt(tr A[b]) - eps)^2 being the largest eigenvalue the
-// rule may drop (decided by a pivoted Cholesky of the shifted matrix completing all n columns,
+// rule may drop (decided by a Cholesky factorisation of the shifted matrix completing all n columns,
 // with a small margin towards "no"). A is not modified; `ws`: eigh_gram_workspace_bytes(n, batch).
 // 64 < n <= 1024 (else *all = false). Synchronises once.
 hipError_t gram_keeps_all(int n, int batch, const void* A, long long lda, long long bsA, double eps, void* ws,

@@ -63,6 +63,26 @@ size_t mpdo_workspace_needed(void);
  * factors are always fp64. Returns the previous mode.
  */
 int mpdo_set_math_mode(int mode);
+/*
+ * Tile autotuner of the fp64-accumulating complex GEMMs (no counterpart in the reference, which
+ * calls the vendor's GEMMs). A key is a GEMM's (M, N, K, batch, precision mode, Hermitian output,
+ * triangular operand, A stored complex128, B stored complex128, reduction slices, A contiguous
+ * along k, B contiguous along k); the table maps keys to tile shapes (0 = 64x64x16 one LDS stage,
+ * 1 = 64x64x32, 2 = 128x64x16, 3 = 64x128x16, 4 = 128x128x16, 5 = 64x64x16 two LDS stages,
+ * 6 = 32x128x16). Keys absent from the table use the built-in rules.
+ * record(1) starts recording the first launch of every new key (clearing the previous record),
+ * record(0) stops; returns the previous state. tune_run replays every recorded GEMM with each tile
+ * shape that fits it, `reps` timed launches on `stream` after one untimed, keeps the fastest in the
+ * table and returns the number of keys tuned (-1 on a HIP error). The replays overwrite the
+ * recorded GEMMs' outputs, whose buffers must still be allocated: run it right after the recorded
+ * work and discard that work's results. get writes (12 key ints, tile) per entry into out (at most
+ * cap entries) and returns the number of entries; set replaces the table with `count` such entries
+ * and returns how many it accepted.
+ */
+int mpdo_gemm_tune_record(int on);
+int mpdo_gemm_tune_run(int reps, void* stream);
+int mpdo_gemm_tune_get(int* out, int cap);
+int mpdo_gemm_tune_set(const int* in, int count);
 
 /*
  * Single-qubit gate / Kraus absorption, batched over sites.

@@ -86,3 +86,27 @@ def test_layer_workspace_query_matches_dry_run():
     bad = (_lib.Site * 2)(_lib.Site(None, 1, 1, 2), _lib.Site(None, 3, 1, 1))
     assert lib.mpdo_layer_workspace_bytes(1, 2, bad, 4, 0, 16, 4, 2) == 0
     assert b'mismatch' in lib.mpdo_last_error()
+
+
+def test_gemm_tile_table_round_trip():
+    """The tile autotuner's table is host state: set, get and the JSON file round-trip without a
+    GPU; out-of-range tiles are refused."""
+    from mpdo_amd import _lib, autotune
+    saved = autotune.table()
+    try:
+        entries = [{'key': [256, 16384, 256, 32, 2, 0, 0, 1, 1, 1, 1, 0], 'tile': 2},
+                   {'key': [512, 512, 1024, 32, 2, 1, 0, 1, 1, 1, 0, 1], 'tile': 4},
+                   {'key': [1, 2, 3, 4, 2, 0, 0, 0, 0, 1, 1, 1], 'tile': 9}]
+        assert autotune.set_table(entries) == 2
+        got = sorted(autotune.table(), key=lambda e: e['key'])
+        assert got == sorted(entries[:2], key=lambda e: e['key'])
+        import tempfile
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, 't.json')
+            autotune.save(path, got)
+            autotune.set_table([])
+            assert autotune.table() == []
+            assert autotune.load(path) == 2
+            assert sorted(autotune.table(), key=lambda e: e['key']) == got
+    finally:
+        autotune.set_table(saved)

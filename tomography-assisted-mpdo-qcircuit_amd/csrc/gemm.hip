@@ -16,7 +16,11 @@
 #include "prof.h"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <shared_mutex>
 #include <type_traits>
+#include <vector>
 
 namespace mpdo {
 
@@ -324,13 +328,15 @@ static hipError_t launch(const GemmArgs& g0, hipStream_t s) {
 }
 
 // Tile shape of the fp64-accumulating GEMMs (BM x BN x BK): 0 = 64x64x16, 1 = 64x64x32,
-// 2 = 128x64x16, 3 = 64x128x16, 4 = 128x128x16, 5 = 64x64x16 with two LDS stages.
+// 2 = 128x64x16, 3 = 64x128x16, 4 = 128x128x16, 5 = 64x64x16 with two LDS stages, 6 = 32x128x16.
 int g_tile_variant = 0;
 
+// variant < 0: the rules below (and g_tile_variant); else that tile shape (the autotuner's choice)
 template <typename TInA, typename TInB, typename TOut>
-static hipError_t launch_f64(const GemmArgs& g, hipStream_t s) {
+static hipError_t launch_f64(const GemmArgs& g, hipStream_t s, int variant = -1) {
+  const int v = variant >= 0 ? variant : g_tile_variant;
   if constexpr (std::is_same<TInA, TInB>::value) {
-    switch (g_tile_variant) {
+    switch (v) {
       case 1: return launch<TInA, TInB, double, TOut, 64, 64, 32>(g, s);
       case 2: return launch<TInA, TInB, double, TOut, 128, 64, 16>(g, s);
       case 3: return launch<TInA, TInB, double, TOut, 64, 128, 16>(g, s);
@@ -338,24 +344,85 @@ static hipError_t launch_f64(const GemmArgs& g, hipStream_t s) {
       default: break;
     }
   }
+  if (v == 6 && !g.herm) return launch<TInA, TInB, double, TOut, 32, 128, 16>(g, s);
   // at most 32 rows: 32 x 128 tiles (a 64-row tile would leave most matrix-core work idle)
-  if (g.M <= 32 && g.N >= 512 && !g.herm && !g.tri && g_tile_variant == 0)
+  if (variant < 0 && g.M <= 32 && g.N >= 512 && !g.herm && !g.tri && g_tile_variant == 0)
     return launch<TInA, TInB, double, TOut, 32, 128, 16>(g, s);
   // a complex128 operand doubles the staging traffic per stage: two LDS stages hide it
   // (43.9 -> 58.7 TFLOP/s at 8192 x 512 x 512); complex64 operands gain nothing from them
   constexpr bool wide = std::is_same<TInA, double2>::value || std::is_same<TInB, double2>::value;
-  if (wide || g_tile_variant == 5) return launch<TInA, TInB, double, TOut, 64, 64, 16, 2>(g, s);
+  if (v == 0 && variant >= 0) return launch<TInA, TInB, double, TOut, 64, 64, 16>(g, s);
+  if (wide || v == 5) return launch<TInA, TInB, double, TOut, 64, 64, 16, 2>(g, s);
   return launch<TInA, TInB, double, TOut, 64, 64, 16>(g, s);
 }
 
 // operand types of the fp64-accumulating paths: complex128 when the precision mode says so,
 // unless the operand's own flag (GemmArgs::inA / inB) overrides it
 template <typename TOut>
-static hipError_t launch_f64_mixed(bool a128, bool b128, const GemmArgs& g, hipStream_t s) {
-  if (a128 && b128) return launch_f64<double2, double2, TOut>(g, s);
-  if (a128) return launch_f64<double2, float2, TOut>(g, s);
-  if (b128) return launch_f64<float2, double2, TOut>(g, s);
-  return launch_f64<float2, float2, TOut>(g, s);
+static hipError_t launch_f64_mixed(bool a128, bool b128, const GemmArgs& g, hipStream_t s, int variant = -1) {
+  if (a128 && b128) return launch_f64<double2, double2, TOut>(g, s, variant);
+  if (a128) return launch_f64<double2, float2, TOut>(g, s, variant);
+  if (b128) return launch_f64<float2, double2, TOut>(g, s, variant);
+  return launch_f64<float2, float2, TOut>(g, s, variant);
+}
+
+// ---------------------------------------------------------------------------------------
+// Tile autotuner for the fp64-accumulating GEMMs. A key is a GEMM's shape, modes and operand
+// layout; the table maps keys to tile variants. While recording, gemm() keeps the arguments of
+// the first launch of every new key; tune_run() then replays each recorded GEMM with every tile
+// variant that fits it (timed with HIP events on the caller's stream, `reps` launches after one
+// untimed) and keeps the fastest. The replays overwrite the recorded GEMMs' outputs: tune right
+// after the work that was recorded, while its buffers are still allocated, and discard its
+// results. Keys absent from the table use the built-in rules.
+// ---------------------------------------------------------------------------------------
+constexpr int TUNE_FIELDS = 12;
+
+struct TuneRec {
+  GemmArgs g;
+  Prec p;
+  int key[TUNE_FIELDS];
+};
+
+std::shared_mutex g_tune_mu;
+std::map<std::vector<int>, int> g_tune_table;
+bool g_tune_recording = false;
+std::vector<TuneRec> g_tune_log;
+
+void tune_key(Prec p, const GemmArgs& g, bool a128, bool b128, int* k) {
+  const int f[TUNE_FIELDS] = {g.M, g.N, g.K, g.batch, (int)p, g.herm, g.tri, a128, b128,
+                              g.ksplit > 1 ? g.ksplit : 1, g.csA == 1, g.rsB == 1};
+  for (int i = 0; i < TUNE_FIELDS; ++i) k[i] = f[i];
+}
+
+bool is_f64_mode(Prec p) { return p == C64_F64ACC || p == C128_F64ACC || p == C128_IN_C64_OUT || p == C64_F64ACC_C64OUT; }
+
+// the tuned variant of a key (-1: none), recording the key's first launch while recording is on
+int tune_lookup(Prec p, const GemmArgs& g, bool a128, bool b128) {
+  int k[TUNE_FIELDS];
+  tune_key(p, g, a128, b128, k);
+  std::vector<int> key(k, k + TUNE_FIELDS);
+  {
+    std::shared_lock<std::shared_mutex> lk(g_tune_mu);
+    if (!g_tune_recording) {
+      auto it = g_tune_table.find(key);
+      return it == g_tune_table.end() ? -1 : it->second;
+    }
+  }
+  std::unique_lock<std::shared_mutex> lk(g_tune_mu);
+  bool seen = false;
+  for (const auto& r : g_tune_log) seen = seen || std::equal(r.key, r.key + TUNE_FIELDS, k);
+  if (!seen) {
+    TuneRec r{g, p, {}};
+    std::copy(k, k + TUNE_FIELDS, r.key);
+    g_tune_log.push_back(r);
+  }
+  auto it = g_tune_table.find(key);
+  return it == g_tune_table.end() ? -1 : it->second;
+}
+
+hipError_t launch_variant(Prec p, const GemmArgs& g, hipStream_t s, bool a128, bool b128, int variant) {
+  if (p == C64_F64ACC || p == C128_F64ACC) return launch_f64_mixed<double2>(a128, b128, g, s, variant);
+  return launch_f64_mixed<float2>(a128, b128, g, s, variant);
 }
 
 hipError_t gemm(Prec p, const GemmArgs& g, hipStream_t s) {
@@ -389,6 +456,10 @@ hipError_t gemm(Prec p, const GemmArgs& g, hipStream_t s) {
                  nb * (inbA * (double)g.M * kt + inbB * kt * g.N +
                        outb * mn * ((g.beta_re != 0 || g.beta_im != 0) ? 2 : 1)),
                  shape);
+  if (is_f64_mode(p)) {
+    const int v = tune_lookup(p, g, a128, b128);
+    if (v >= 0) return launch_variant(p, g, s, a128, b128, v);
+  }
   switch (p) {
     case C64_F32ACC:
       if (g.M <= 32 && g.N >= 512 && !g.herm && !g.tri) return launch<float2, float2, float, float2, 32, 128, 16>(g, s);
@@ -419,6 +490,104 @@ extern "C" int mpdo_debug_gemm_tiles(int v) {
   if (v >= 0 && v <= 5) mpdo::g_tile_variant = v;
   return old;
 }
+
+namespace mpdo {
+namespace {
+// the tile variants a recorded GEMM can take: square tiles only for Hermitian outputs, the wider
+// and taller fp64 tiles only when both operands are stored alike
+std::vector<int> tune_variants(const TuneRec& r) {
+  const bool same = r.key[7] == r.key[8];
+  std::vector<int> v{0, 5};
+  if (same) {
+    v.push_back(1);
+    v.push_back(4);
+    if (!r.g.herm) { v.push_back(2); v.push_back(3); }
+  }
+  if (!r.g.herm && r.g.M <= 64) v.push_back(6);
+  return v;
+}
+}  // namespace
+}  // namespace mpdo
+
+extern "C" {
+
+int mpdo_gemm_tune_record(int on) {
+  std::unique_lock<std::shared_mutex> lk(mpdo::g_tune_mu);
+  const int old = mpdo::g_tune_recording ? 1 : 0;
+  if (on >= 0) {
+    if (on && !mpdo::g_tune_recording) mpdo::g_tune_log.clear();
+    mpdo::g_tune_recording = on != 0;
+  }
+  return old;
+}
+
+int mpdo_gemm_tune_run(int reps, void* stream) {
+  using namespace mpdo;
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<TuneRec> log;
+  {
+    std::unique_lock<std::shared_mutex> lk(g_tune_mu);
+    g_tune_recording = false;
+    log = g_tune_log;
+  }
+  if (reps < 1) reps = 1;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return -1;
+  int tuned = 0;
+  for (const auto& r : log) {
+    const bool a128 = r.key[7] != 0, b128 = r.key[8] != 0;
+    int best = -1;
+    float best_ms = 0;
+    for (int v : tune_variants(r)) {
+      if (launch_variant(r.p, r.g, s, a128, b128, v) != hipSuccess) continue;   // untimed first launch
+      hipEventRecord(e0, s);
+      bool ok = true;
+      for (int i = 0; i < reps && ok; ++i) ok = launch_variant(r.p, r.g, s, a128, b128, v) == hipSuccess;
+      hipEventRecord(e1, s);
+      if (!ok || hipEventSynchronize(e1) != hipSuccess) continue;
+      float ms = 0;
+      hipEventElapsedTime(&ms, e0, e1);
+      if (best < 0 || ms < best_ms) { best = v; best_ms = ms; }
+    }
+    if (best >= 0) {
+      std::unique_lock<std::shared_mutex> lk(g_tune_mu);
+      g_tune_table[std::vector<int>(r.key, r.key + TUNE_FIELDS)] = best;
+      ++tuned;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return tuned;
+}
+
+int mpdo_gemm_tune_get(int* out, int cap) {
+  std::shared_lock<std::shared_mutex> lk(mpdo::g_tune_mu);
+  int n = 0;
+  for (const auto& kv : mpdo::g_tune_table) {
+    if (out && n < cap) {
+      for (int i = 0; i < mpdo::TUNE_FIELDS; ++i) out[(size_t)n * (mpdo::TUNE_FIELDS + 1) + i] = kv.first[i];
+      out[(size_t)n * (mpdo::TUNE_FIELDS + 1) + mpdo::TUNE_FIELDS] = kv.second;
+    }
+    ++n;
+  }
+  return n;
+}
+
+int mpdo_gemm_tune_set(const int* in, int count) {
+  std::unique_lock<std::shared_mutex> lk(mpdo::g_tune_mu);
+  mpdo::g_tune_table.clear();
+  int n = 0;
+  for (int e = 0; e < count && in; ++e) {
+    const int* r = in + (size_t)e * (mpdo::TUNE_FIELDS + 1);
+    const int v = r[mpdo::TUNE_FIELDS];
+    if (v < 0 || v > 6) continue;
+    mpdo::g_tune_table[std::vector<int>(r, r + mpdo::TUNE_FIELDS)] = v;
+    ++n;
+  }
+  return n;
+}
+
+}  // extern "C"
 
 namespace mpdo {
 

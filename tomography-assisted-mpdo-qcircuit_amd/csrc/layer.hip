@@ -285,6 +285,25 @@ hipError_t k_bds64(const double2* M, int A, int R, double2* out, long long bsM, 
   return block_diag_sum64(M, A, R, out, bsM, bsO, batch, s);
 }
 
+// out[b][e] = in[b][e] (conjugated when conj) in complex64, e < n
+__global__ void narrow_c128_kernel(const double2* in, float2* out, long long n, int conj) {
+  const int b = blockIdx.y;
+  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < n; e += (long long)gridDim.x * 256) {
+    const double2 v = in[b * n + e];
+    out[b * n + e] = make_float2((float)v.x, conj ? (float)-v.y : (float)v.y);
+  }
+}
+
+hipError_t k_narrow(const double2* in, float2* out, long long n, int conj, int batch, hipStream_t s, int line) {
+  if (!fits(in, (long long)batch * n, sizeof(double2), "narrow input", line) ||
+      !fits(out, (long long)batch * n, sizeof(float2), "narrow output", line))
+    return hipErrorInvalidValue;
+  if (t_dry) return hipSuccess;
+  const unsigned g = (unsigned)std::max<long long>(1, std::min<long long>((n + 255) / 256, 2048));
+  hipLaunchKernelGGL(narrow_c128_kernel, dim3(g, batch), dim3(256), 0, s, in, out, n, conj);
+  return hipGetLastError();
+}
+
 hipError_t k_widen(const float2* in, double2* out, long long n, long long bsI, long long bsO, int batch, hipStream_t s,
                    int line) {
   if (!fits(in, (batch - 1) * bsI + n, sizeof(float2), "widen input", line) ||
@@ -825,7 +844,7 @@ struct PairStore {
   float2 *Qz, *Rz;              // (Ra (x) I) Um = Qz Rz when the gate-time cut is active
   int mq, pz, nc;               // kept directions (max over chains), rows of Rz, rows of the core
   bool qzid;
-  double2* WR;
+  double2* WR;    // right inverse of R_max (r x r, complex128), kept for the chi sweep
   double* lamR;
   bool has_R, has_P;            // has_P: the gate-time cut drops a direction in some chain
 };
@@ -961,6 +980,9 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
   float2 *Mlast = nullptr, *FLq, *Rcur, *Rnext, *RT, *X1, *X1t, *Q1t, *R1t, *Cs, *Cp, *Cu, *Zb, *Q2P, *FL, *FR, *Tmin, *Wm,
          *Tm, *Pn, *Pn2, *PW, *Pp, *Tchi, *Mp, *Vk, *Kout, *dG, *Tcur2, *dK;
   double2 *G, *GR, *Om64, *Z64, *H64, *Gparts, *Pg, *Hk, *Tw, *Rs64, *Zr, *Nn, *Sparts;
+  double2* LR;          // pivoted Cholesky factor of the right-bond Gram (R_max = L^H)
+  int *pivR, *rankR;
+  char *pwsR, *iwsR;
   float2 *Zm, *Vf, *Vt;
   double* ddisc;
   int* dk;
@@ -1039,6 +1061,11 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     Vf = c.get<float2>(mxV * B);
     Vt = c.get<float2>(mxVt * B);
     GR = c.get<double2>((size_t)mxEig * mxEig * B);
+    LR = c.get<double2>((size_t)mxEigR * mxEigR * B);
+    pivR = c.get<int>((size_t)mxEigR * B);
+    rankR = c.get<int>((size_t)B);
+    pwsR = c.get<char>(pchol_workspace_bytes(mxEigR, B));
+    iwsR = c.get<char>(pchol_inverse_workspace_bytes(mxEigR, B));
     carve_eig(c, mxEig, B, mxKp, eE);
     carve_qr(c, (int)mxQR_rows, (int)mxQR_cols, B, qw);
     dk = c.get<int>((size_t)B * 4);
@@ -1350,12 +1377,24 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
       g.herm = 1;
       LCK(herm_gemm_(__LINE__, C128_F64ACC, g, s));
     }
-    {
-      EigB eR{ps.WR, ps.lamR, eE.ws};
-      LST(eig_b(r, B, GR, eR, s));
+    // R_max with R_max^H R_max = GR: the pivoted Cholesky factor GR = L L^H (pivots down to FLOOR
+    // of the largest diagonal; rows of R_max past the rank zero), R_max = L^H; its right inverse
+    // P Lt^{-H} (Lt the pivot-ordered triangle), kept for the chi sweep, maps the kept directions
+    // back to the bond. Any factor of GR gives the layer the same result (the QR sweep's bond
+    // basis is free); this one needs no eigensolver.
+    if (!fits(GR, (long long)r * r * B, sizeof(double2), "right-bond Gram", __LINE__) ||
+        !fits(LR, (long long)r * r * B, sizeof(double2), "right-bond factor", __LINE__) ||
+        !fits(ps.WR, (long long)r * r * B, sizeof(double2), "right-bond inverse", __LINE__) ||
+        !fits(pivR, (long long)r * B, sizeof(int), "right-bond pivots", __LINE__) ||
+        !fits(pwsR, (long long)pchol_workspace_bytes(r, B), 1, "right-bond Cholesky workspace", __LINE__) ||
+        !fits(iwsR, (long long)pchol_inverse_workspace_bytes(r, B), 1, "right-bond inverse workspace", __LINE__))
+      return layer_fail(MPDO_EINVAL, t_oob.c_str());
+    if (!dry) {
+      LCK(pchol_b(r, B, GR, r, (long long)r * r, FLOOR, LR, pivR, rankR, pwsR, s, false));
+      LCK(pchol_inverse_b(r, B, LR, pivR, rankR, 0, ps.WR, iwsR, s));
     }
-    // R_max = L^{1/2} W^H  (r x r)
-    LCK(K_FACTOR(ps.WR, r, (long long)r * r, ps.lamR, r, r, r, r, 0.5, FLOOR, 1, Rcur, r, (long long)r * r, B, s));
+    // R_max[k][i] = conj(L_ik): L is stored by columns (L[k r + i] = L_ik), so elementwise
+    LCK(k_narrow(LR, Rcur, (long long)r * r, 1, B, s, __LINE__));
     kcur = r;
   }
 
@@ -1521,7 +1560,7 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     // right projector in the original bond: P = W_R L_R^{-1/2} Pn (r x kr), or the identity
     const float2* Qright;   // Q2 P : [p2][a2][kr] = [p2 x c3]
     if (ps.has_R) {
-      LCK(K_FACTOR(ps.WR, r, (long long)r * r, ps.lamR, r, r, r, r, -0.5, FLOOR, 0, PW, r, (long long)r * r, B, s));
+      LCK(k_narrow(ps.WR, PW, (long long)r * r, 0, B, s, __LINE__));   // R_max's right inverse
       LCK(mmb(g_tensor_prec, r, kr, r, PW, r, (long long)r * r, Pn, kP, (long long)r * kP, Pp, kr, (long long)r * kr, B,
               s));
       const float2* Q2p = ps.Q2;

@@ -50,6 +50,10 @@ def _load() -> ctypes.CDLL:
     sig = {
         'mpdo_version': (I, []),
         'mpdo_bind_workspace': (I, [VP, VP, ctypes.c_size_t]),
+        'mpdo_gemm_tune_record': (I, [I]),
+        'mpdo_gemm_tune_run': (I, [I, VP]),
+        'mpdo_gemm_tune_get': (I, [P(I), I]),
+        'mpdo_gemm_tune_set': (I, [P(I), I]),
         'mpdo_workspace_needed': (ctypes.c_size_t, []),
         'mpdo_layer_workspace_bytes': (ctypes.c_size_t, [I, I, SP, I, I, I, I, I]),
         'mpdo_last_error': (ctypes.c_char_p, []),
@@ -112,8 +116,24 @@ def _load() -> ctypes.CDLL:
         f = getattr(lib, name)
         f.restype = res
         f.argtypes = args
+    _load_tile_table(lib)
     _lib = lib
     return lib
+
+
+# The GEMM tile autotuner's table for this architecture (mpdo_amd/autotune.py writes it): loaded
+# into the library when it is loaded; absent, the library's built-in tile rules apply.
+TILE_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'gemm_tiles_gfx950.json')
+
+
+def _load_tile_table(lib) -> int:
+    import json
+    if not os.path.exists(TILE_TABLE):
+        return 0
+    with open(TILE_TABLE) as f:
+        entries = json.load(f)['entries']
+    flat = (ctypes.c_int * max(13 * len(entries), 1))(*[int(v) for e in entries for v in list(e['key']) + [e['tile']]])
+    return int(lib.mpdo_gemm_tune_set(flat, len(entries)))
 
 
 class MPDOError(RuntimeError):

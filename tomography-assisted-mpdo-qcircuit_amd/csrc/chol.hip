@@ -395,8 +395,11 @@ __global__ __launch_bounds__(256) void chol_init_kernel(const double2* A, int n,
 }
 
 // Diagonal block j0 .. j0 + nb - 1 of matrix b (already updated by the earlier panels): Cholesky in
-// LDS, written to L (column k at L[k n + i]); its triangle's inverse to st.Linv.
-__global__ __launch_bounds__(256) void chol_diag_kernel(const double2* A, int n, long long lda, long long bsA, int j0,
+// LDS, written to L (column k at L[k n + i]); its triangle's inverse to st.Linv. Four waves per
+// matrix (one wave alone is slower: its longer per-thread update loops cost more than the
+// barriers they save).
+constexpr int CD_T = 256;
+__global__ __launch_bounds__(CD_T) void chol_diag_kernel(const double2* A, int n, long long lda, long long bsA, int j0,
                                                         double2* L, ChState st) {
   __shared__ double2 S[64][65];
   __shared__ double2 X[64][65];
@@ -404,7 +407,7 @@ __global__ __launch_bounds__(256) void chol_diag_kernel(const double2* A, int n,
   if (st.stop[b]) return;
   const int nb = min(64, n - j0);
   A += (long long)b * bsA + (long long)j0 * lda + j0;
-  for (int e = tid; e < nb * nb; e += 256) {
+  for (int e = tid; e < nb * nb; e += CD_T) {
     const int i = e / nb, k = e % nb;
     S[i][k] = A[(long long)i * lda + k];
   }
@@ -416,12 +419,12 @@ __global__ __launch_bounds__(256) void chol_diag_kernel(const double2* A, int n,
     if (!(d > thr)) { fail = j; break; }   // uniform: every thread read the same value
     const double piv = sqrt(d);
     __syncthreads();
-    for (int i = j + 1 + tid; i < nb; i += 256) S[i][j] = make_double2(S[i][j].x / piv, S[i][j].y / piv);
+    for (int i = j + 1 + tid; i < nb; i += CD_T) S[i][j] = make_double2(S[i][j].x / piv, S[i][j].y / piv);
     if (tid == 0) S[j][j] = make_double2(piv, 0.0);
     __syncthreads();
     // S[i][k] -= S[i][j] conj(S[k][j]) for j < k <= i < nb
     const int m = nb - j - 1;
-    for (int e = tid; e < m * m; e += 256) {
+    for (int e = tid; e < m * m; e += CD_T) {
       const int i = j + 1 + e / m, k = j + 1 + e % m;
       if (k <= i) {
         const double2 x = S[i][j], y = S[k][j];
@@ -434,28 +437,28 @@ __global__ __launch_bounds__(256) void chol_diag_kernel(const double2* A, int n,
   double2* Lb = L + (long long)b * n * n;
   double2* Li = st.Linv + (long long)b * 64 * 64;
   if (fail >= 0) {
-    for (int e = tid; e < 64 * 64; e += 256) Li[e] = make_double2(0.0, 0.0);
+    for (int e = tid; e < 64 * 64; e += CD_T) Li[e] = make_double2(0.0, 0.0);
     if (tid == 0) {
       st.rank[b] = j0 + fail;
       st.stop[b] = 1;
     }
     return;
   }
-  for (int e = tid; e < nb * nb; e += 256) {
+  for (int e = tid; e < nb * nb; e += CD_T) {
     const int i = e / nb, k = e % nb;
     if (k <= i) Lb[(long long)(j0 + k) * n + j0 + i] = S[i][k];
   }
   // inverse of the lower triangle, row by row: X starts as the identity (the right-hand sides);
   // row i is final once divided by S_ii, then its multiples leave the rows below it (one rank-1
   // update per row, spread over the workgroup)
-  for (int e = tid; e < 64 * 64; e += 256) X[e / 64][e % 64] = make_double2((e / 64 == e % 64) ? 1.0 : 0.0, 0.0);
+  for (int e = tid; e < 64 * 64; e += CD_T) X[e / 64][e % 64] = make_double2((e / 64 == e % 64) ? 1.0 : 0.0, 0.0);
   __syncthreads();
   for (int i = 0; i < nb; ++i) {
     const double dii = S[i][i].x;   // real and positive
     if (tid <= i) X[i][tid] = make_double2(X[i][tid].x / dii, X[i][tid].y / dii);
     __syncthreads();
     const int rows = nb - i - 1, cols = i + 1;
-    for (int e = tid; e < rows * cols; e += 256) {   // X[r][t] -= S[r][i] X[i][t], r > i, t <= i
+    for (int e = tid; e < rows * cols; e += CD_T) {   // X[r][t] -= S[r][i] X[i][t], r > i, t <= i
       const int r = i + 1 + e / cols, t = e % cols;
       const double2 a = S[r][i], x = X[i][t];
       X[r][t].x -= a.x * x.x - a.y * x.y;
@@ -463,7 +466,7 @@ __global__ __launch_bounds__(256) void chol_diag_kernel(const double2* A, int n,
     }
     __syncthreads();
   }
-  for (int e = tid; e < 64 * 64; e += 256) Li[e] = X[e / 64][e % 64];
+  for (int e = tid; e < 64 * 64; e += CD_T) Li[e] = X[e / 64][e % 64];
   if (tid == 0) st.rank[b] = j0 + nb;
 }
 
@@ -490,7 +493,7 @@ hipError_t chol_b(int n, int batch, void* A, long long lda, long long bsA, doubl
   if ((err = hipGetLastError()) != hipSuccess) return err;
   double2* Ab = (double2*)A;
   for (int j0 = 0; j0 < n; j0 += 64) {
-    hipLaunchKernelGGL(chol_diag_kernel, dim3(batch), dim3(256), 0, st, (const double2*)A, n, lda, bsA, j0, L, cs);
+    hipLaunchKernelGGL(chol_diag_kernel, dim3(batch), dim3(CD_T), 0, st, (const double2*)A, n, lda, bsA, j0, L, cs);
     if ((err = hipGetLastError()) != hipSuccess) return err;
     const int nb = std::min(64, n - j0), rest = n - j0 - nb;
     if (rest <= 0) break;

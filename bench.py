@@ -47,7 +47,19 @@ for _p in (ROOT, PKG):
         sys.path.insert(0, _p)
 
 METRIC = 'circuit-layers/sec at χ=256,κ=8 (batched vs plain) @ ≤1e-5 DM error'
-PEAK_TFLOPS = {'fp64': 78.6, 'fp32': 157.3}     # MI355X dense peaks
+PEAK_TFLOPS = {'fp64': 78.6, 'fp32': 157.3, 'bf16': 2500.0}     # MI355X dense peaks
+MATH_MODES = {'fp64acc': 0, 'fp32x3m': 2, 'bf16x3': 3, 'bf16x6': 4}
+# GEMM groups by the precision field of a recorded shape: (group, executed flops per counted flop,
+# peak, kernel). A complex product counts 8 M N K flops; 3M executes 6 of them, the split-bf16
+# modes 4 real products x 3 or 6 part products (24 or 48 M N K) on the bf16 matrix cores.
+GEMM_GROUPS = {5: ('3m', 0.75, 'fp32', 'mpdo::gemm_kernel, 3M products on fp32 MFMA (site contractions)'),
+               6: ('bf16x3', 3.0, 'bf16', 'mpdo::gemm_bf16_kernel, 2-part split operands on bf16 MFMA (site contractions)'),
+               7: ('bf16x6', 6.0, 'bf16', 'mpdo::gemm_bf16_kernel, 3-part split operands on bf16 MFMA (site contractions)')}
+FP64_GROUP = ('fp64', 1.0, 'fp64', 'mpdo::gemm_kernel, fp64 MFMA (Grams, factors, fp64-mode contractions)')
+DTYPES = {0: 'c64 data, fp64 accumulate/eigensolve',
+          2: 'c64 data; site contractions 3M fp32-accumulate MFMA, Grams/QR/eigensolves fp64',
+          3: 'c64 data; site contractions bf16x3 (2-part split, fp32 accumulate) MFMA, Grams/QR/eigensolves fp64',
+          4: 'c64 data; site contractions bf16x6 (3-part split, fp32 accumulate) MFMA, Grams/QR/eigensolves fp64'}
 N_QUBITS, CHI, KAPPA, P_NOISE, FIRST_LAYER = 32, 256, 8, 0.002, 10
 # HBM bytes per GEMM launch of the timed step, from the two rocprofv3 --pmc passes (FETCH_SIZE,
 # WRITE_SIZE) of this benchmark, scaled by the copy calibration; the file records the commands
@@ -349,10 +361,11 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-threads', type=int, default=None,
                     help='threads of the CPU oracle sample (default: the cores this process may run on)')
-    ap.add_argument('--math', choices=['fp64acc', 'fp32x3m'], default='fp32x3m',
-                    help='contractions whose results are complex64 site data: fp64 MFMA accumulation, or three real '
-                         'products per complex product accumulated in fp32 (3M; Gram matrices, QR/LQ factors and '
-                         'eigensolves stay fp64)')
+    ap.add_argument('--math', choices=list(MATH_MODES), default='bf16x6',
+                    help='contractions whose results are complex64 site data: fp64 MFMA accumulation, three real '
+                         'products per complex product accumulated in fp32 (3M), or bf16 matrix-core products of '
+                         'operands split into 2 or 3 bf16 parts (fp32 accumulation); Gram matrices, QR/LQ factors '
+                         'and eigensolves stay fp64')
     ap.add_argument('--uneven', type=int, default=64,
                     help='circuits of the 256-circuit uneven batch (workloads.random_brickwork_batch) to time through '
                          'scheduler.evolve_many after the main measurement (0: skip)')
@@ -406,7 +419,7 @@ def main():
         _log(f'copy calibration: 3 launches of torch.neg, {CALIB_BYTES} bytes read and {CALIB_BYTES} bytes '
              'written each')
     lib = _lib.load()
-    math_mode = {'fp64acc': 0, 'fp32x3m': 2}[args.math]
+    math_mode = MATH_MODES[args.math]
     prev_mode = lib.mpdo_set_math_mode(math_mode)
     B = args.copies
     layers = _layers(B, args.warmup + args.steps, dev)
@@ -484,22 +497,21 @@ def main():
              '; '.join(f'{sh}: {c}, {t:.0f}, {f / (t * 1e-3) / 1e12:.1f}' for sh, c, t, f in shapes[:14]))
         total_ms = max(sum(v[1] for v in prof.values()), 1e-9)
         n_l, ms, fl, by = prof['gemm']
-        # the GEMM launches by arithmetic: fp64 MFMA (complex products as four real ones) and the 3M
-        # products on fp32 MFMA (precision mode 5; three real products, 6 M N K executed flops of the
-        # 8 M N K a complex product counts)
-        groups = {}
+        # the GEMM launches by arithmetic (GEMM_GROUPS): fp64 MFMA (complex products as four real
+        # ones), the 3M products on fp32 MFMA and the split-bf16 products, each at its executed flops
+        groups, info = {}, {}
         for sh, c, t, f in shapes:
-            g3 = sh[4] % 10 == 5
-            a = groups.setdefault('3m' if g3 else 'fp64', [0, 0.0, 0.0])
+            key, mult, pk, kern = GEMM_GROUPS.get(sh[4] % 10, FP64_GROUP)
+            info[key] = (pk, kern)
+            a = groups.setdefault(key, [0, 0.0, 0.0])
             a[0] += c
             a[1] += t
-            a[2] += f * (0.75 if g3 else 1.0)
+            a[2] += f * mult
         gemm_groups = {}
         for key, (c, t, f) in groups.items():
-            peak = PEAK_TFLOPS['fp32' if key == '3m' else 'fp64']
+            peak = PEAK_TFLOPS[info[key][0]]
             a = f / (t * 1e-3) / 1e12
-            gemm_groups[key] = {'kernel': ('mpdo::gemm_kernel, 3M products on fp32 MFMA (site contractions)' if key == '3m'
-                                           else 'mpdo::gemm_kernel, fp64 MFMA (Grams, factors, fp64-mode contractions)'),
+            gemm_groups[key] = {'kernel': info[key][1],
                                 'bound': 'mfma', 'achieved': round(a, 3), 'peak': peak, 'unit': 'TFLOP/s',
                                 'frac': round(a / peak, 4), 'launches': c, 'avg_launch_ms': round(t / max(c, 1), 4),
                                 'executed_flops_per_launch': f / max(c, 1), 'share_of_kernel_time': round(t / total_ms, 3),
@@ -610,8 +622,7 @@ def main():
     line = {'metric': METRIC, 'value': round(value, 6), 'unit': 'circuit-layers/s', 'n_gpus': 1,
             'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': round(T / args.steps * 1e3, 3),
             'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
-            'dtype': ('c64 data, fp64 accumulate/eigensolve' if math_mode == 0 else
-                      'c64 data; site contractions 3M fp32-accumulate MFMA, Grams/QR/eigensolves fp64'),
+            'dtype': DTYPES.get(math_mode, DTYPES[0]),
             'data': 'synthetic',
             'config': {'workload': ('32-qubit depth-30 RY/RZ+CNOT brickwork, depolarizing p=0.002 (1q, 2q), '
                                     'chi=256, kappa=8, 32 angle-perturbed copies; step = one circuit-layer of '

@@ -59,8 +59,11 @@ size_t mpdo_workspace_needed(void);
  * matrix-core accumulation (default; complex64 storage), 1 = fp32 matrix-core accumulation of
  * four real products per complex product (twice the fp64 peak rate), 2 = fp32 accumulation of
  * three real products (3M: Re = P1 - P2, Im = P3 - P1 - P2 with P1 = Ar Br, P2 = Ai Bi,
- * P3 = (Ar + Ai)(Br + Bi); 8/3 of the fp64 rate). Gram matrices, eigensolves and the QR/LQ
- * factors are always fp64. Returns the previous mode.
+ * P3 = (Ar + Ai)(Br + Bi); 8/3 of the fp64 rate), 3 = bf16 matrix cores on operands split into
+ * two bf16 parts (x = p0 + p1 + r, |r| <= 2^-16 |x|; three part products per real product, fp32
+ * accumulation: about 2^-16 relative per product), 4 = the same with three parts (six part
+ * products: fp32-level products). Gram matrices, eigensolves and the QR/LQ factors are always
+ * fp64. Returns the previous mode.
  */
 int mpdo_set_math_mode(int mode);
 /*

@@ -15,7 +15,9 @@ extern "C" {
 /* C[b] = alpha op(A[b]) op(B[b]) + beta C[b]; strides in complex elements.
  * prec: 0 = complex64 in / fp32 accumulate / complex64 out, 1 = complex64 in / fp64 / complex128 out,
  *       2 = complex128 in / fp64 / complex128 out, 3 = complex128 in / fp64 / complex64 out,
- *       4 = complex64 in / fp64 / complex64 out;
+ *       4 = complex64 in / fp64 / complex64 out, 5 = complex64 in / fp32 accumulation of three real
+ *       products (3M) / complex64 out, 6 and 7 = complex64 in / bf16 matrix cores on operands split
+ *       into 2 (6) or 3 (7) bf16 parts, fp32 accumulation / complex64 out;
  * prec + 10: the same with a Hermitian output (M == N; the tiles on and below the diagonal are
  *       computed and the strict upper triangle set to the conjugate transpose of the lower). */
 mpdo_status mpdo_debug_gemm(int prec, int M, int N, int K, int batch,

@@ -140,12 +140,13 @@ def _layer10_inputs(cuda, copies):
     return chains, layer
 
 
-@pytest.mark.parametrize('mode', [0, 2])
+@pytest.mark.parametrize('mode', [0, 2, 4])
 def test_layer10_chi256_kappa8_golden(cuda, mode):
     """One chi=256, kappa=8 layer at 10 qubits from the seeded steady state: the fused layer
     (two chains: the fixture's and one from another seed) against the oracle fixture and
-    against the per-gate entry points; the site contractions with fp64 accumulation (mode 0) and
-    as 3M products with fp32 accumulation (mode 2)."""
+    against the per-gate entry points; the site contractions with fp64 accumulation (mode 0), as
+    3M products with fp32 accumulation (mode 2) and as bf16 matrix-core products of operands split
+    into three bf16 parts (mode 4)."""
     from mpdo_amd import _lib
     lib = _lib.load()
     old = lib.mpdo_set_math_mode(mode)
@@ -197,7 +198,19 @@ def test_full_size_n32_chi256_kappa8(cuda):
     layers (both pair parities) from the synthetic steady state the benchmark starts from.
     Copy 0 against the per-gate path on the same input (all 496 <Z_i Z_j>, 32 <Z_i>, Tr rho~,
     shapes, discarded weights of every cut); copies 0 and 17 run alone (batch of one) against
-    their results in the batch of 32."""
+    their results in the batch of 32. The fused layers compute their site contractions as the
+    benchmark does (bf16 matrix cores, three-part split operands: math mode 4), the per-gate
+    reference with fp64 accumulation (mode 0)."""
+    from mpdo_amd import _lib
+    lib = _lib.load()
+    old = lib.mpdo_set_math_mode(0)
+    try:
+        _full_size_check(cuda, lib)
+    finally:
+        lib.mpdo_set_math_mode(old)
+
+
+def _full_size_check(cuda, lib):
     from mpdo_amd import _lib, batched
     from mpdo_amd import workloads as W
     n, B, chi, kappa, first = 32, 32, 256, 8, 10
@@ -215,10 +228,12 @@ def test_full_size_n32_chi256_kappa8(cuda):
     lone = {b: [t[b:b + 1].clone() for t in sites] for b in (0, 17)}
     pg = [t[0].clone() for t in sites]
     disc_all = []
+    lib.mpdo_set_math_mode(4)
     for L in layers:
         sites, disc = _lib.layer_batched(sites, L.kraus1, L.parity, L.gates2, chi, kappa)
         disc_all.append(disc)
         print(f'fused layer parity {L.parity}: kept {_lib.layer_kept()}', flush=True)
+    lib.mpdo_set_math_mode(0)
     codes, _ = _zz_codes(n)
     v_f = np.array(_lib.pauli_expect([t[0].contiguous() for t in sites], codes))
     # per-gate path, copy 0
@@ -231,6 +246,7 @@ def test_full_size_n32_chi256_kappa8(cuda):
     tr = abs(v_g[-1])
     assert np.abs(v_f - v_g).max() <= 1e-5 * tr, np.abs(v_f - v_g).max() / tr
     # batch invariance
+    lib.mpdo_set_math_mode(4)
     for b, ch in lone.items():
         for L in layers:
             k1 = L.kraus1[b:b + 1]

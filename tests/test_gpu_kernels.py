@@ -7,22 +7,30 @@ import torch
 pytestmark = pytest.mark.gpu
 
 
+# Error bars relative to the largest output element. fp32 accumulation: 0 four real products, 5
+# three (the imaginary part P3 - P1 - P2 loses up to a few ulp of |A||B| more), 7 four real products
+# of operands split into three bf16 parts (six part products: fp32-level); 6: two bf16 parts (three
+# part products: each real product within ~3 * 2^-16 of |a||b|).
+_TOL = {0: lambda K: 3e-6 * np.sqrt(K), 1: lambda K: 1e-6, 2: lambda K: 1e-13, 3: lambda K: 1e-6,
+        4: lambda K: 1e-6, 5: lambda K: 6e-6 * np.sqrt(K), 6: lambda K: 1e-4, 7: lambda K: 3e-6 * np.sqrt(K)}
+
+
 def _crand(rng, *shape):
     return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
 
 
-@pytest.mark.parametrize('prec', [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize('prec', [0, 1, 2, 3, 4, 5, 6, 7])
 @pytest.mark.parametrize('M,N,K', [(1, 1, 1), (16, 16, 4), (37, 53, 29), (130, 70, 200), (257, 129, 300)])
 @pytest.mark.parametrize('tA,tB,cA,cB', [(0, 0, 0, 0), (1, 0, 1, 0), (0, 1, 0, 1), (1, 1, 0, 0)])
 def test_gemm(cuda, prec, M, N, K, tA, tB, cA, cB):
     from mpdo_amd import _lib
     rng = np.random.default_rng(M * 1000 + N * 10 + K)
     batch = 2
-    din = np.complex64 if prec in (0, 1, 4, 5) else np.complex128
+    din = np.complex64 if prec in (0, 1, 4, 5, 6, 7) else np.complex128
     A = _crand(rng, batch, *((K, M) if tA else (M, K))).astype(din)
     B = _crand(rng, batch, *((N, K) if tB else (K, N))).astype(din)
     C0 = _crand(rng, batch, M, N)
-    out_t = torch.complex64 if prec in (0, 3, 4, 5) else torch.complex128
+    out_t = torch.complex64 if prec in (0, 3, 4, 5, 6, 7) else torch.complex128
     Cd = torch.from_numpy(C0).to(out_t).to(cuda)
     sa = (1, M, M * K) if tA else (K, 1, M * K)
     sb = (1, K, K * N) if tB else (N, 1, K * N)
@@ -34,16 +42,14 @@ def test_gemm(cuda, prec, M, N, K, tA, tB, cA, cB):
     opA = opA.conj() if cA else opA
     opB = opB.conj() if cB else opB
     ref = alpha * (opA.astype(np.complex128) @ opB.astype(np.complex128)) + beta * C0.astype(
-        np.complex64 if prec in (0, 3, 4, 5) else np.complex128).astype(np.complex128)
+        np.complex64 if prec in (0, 3, 4, 5, 6, 7) else np.complex128).astype(np.complex128)
     got = Cd.cpu().numpy()
     err = np.abs(got - ref).max() / np.abs(ref).max()
-    # fp32 accumulation (0: four real products, 5: three, whose imaginary part P3 - P1 - P2 loses
-    # up to a few ulp of |A||B| more)
-    tol = {0: 3e-6 * np.sqrt(K), 1: 1e-6, 2: 1e-13, 3: 1e-6, 4: 1e-6, 5: 6e-6 * np.sqrt(K)}[prec]
+    tol = _TOL[prec](K)
     assert err < tol, (prec, M, N, K, err)
 
 
-@pytest.mark.parametrize('prec', [1, 2, 3, 4])
+@pytest.mark.parametrize('prec', [1, 2, 3, 4, 6, 7])
 @pytest.mark.parametrize('n,K', [(1, 5), (17, 33), (64, 16), (65, 130), (200, 70), (257, 512)])
 def test_gemm_hermitian(cuda, prec, n, K):
     """Hermitian-output mode (prec + 10): X^H X computed from the tiles on and below the
@@ -51,9 +57,9 @@ def test_gemm_hermitian(cuda, prec, n, K):
     from mpdo_amd import _lib
     rng = np.random.default_rng(n * 7 + K)
     batch = 3
-    din = np.complex64 if prec in (1, 4) else np.complex128
+    din = np.complex64 if prec in (1, 4, 6, 7) else np.complex128
     X = _crand(rng, batch, K, n).astype(din)
-    out_t = torch.complex64 if prec in (3, 4) else torch.complex128
+    out_t = torch.complex64 if prec in (3, 4, 6, 7) else torch.complex128
     Cd = torch.full((batch, n, n), 7.0 + 3.0j, dtype=out_t, device=cuda)
     Xd = torch.from_numpy(X).to(cuda)
     _lib.debug_gemm(prec + 10, Xd, Xd, Cd, n, n, K, (1, n, K * n), (n, 1, K * n), (n, 1, n * n), True, False,
@@ -62,7 +68,7 @@ def test_gemm_hermitian(cuda, prec, n, K):
     ref = np.conj(np.swapaxes(Xw, 1, 2)) @ Xw
     got = Cd.cpu().numpy()
     err = np.abs(got - ref).max() / np.abs(ref).max()
-    assert err < (1e-13 if prec == 2 else 1e-6), (prec, n, K, err)
+    assert err < (_TOL[prec](K) if prec in (6, 7) else 1e-13 if prec == 2 else 1e-6), (prec, n, K, err)
     assert np.array_equal(got, np.conj(np.swapaxes(got, 1, 2)) * np.where(np.eye(n, dtype=bool), 0, 1) +
                           got * np.eye(n)), 'not exactly Hermitian off the diagonal'
 
@@ -100,13 +106,13 @@ def test_gemm_mixed_operands(cuda, prec, inA, inB, M, N, K, tA, tB):
     assert err < (1e-13 if prec in (1, 2) else 1e-6), (prec, inA, inB, M, N, K, err)
 
 
-@pytest.mark.parametrize('prec', [4, 0, 5])
+@pytest.mark.parametrize('prec', [4, 0, 5, 6, 7])
 @pytest.mark.parametrize('M,N,K,tA', [(16, 4096, 256, True), (32, 640, 40, False), (5, 513, 17, True),
                                       (31, 1000, 300, False)])
 def test_gemm_few_rows(cuda, prec, M, N, K, tA):
     """At most 32 output rows and a wide output: the 32 x 128 tile shape, with fp64 accumulation
-    (mode 4) and fp32 accumulation of four (mode 0) and three (mode 5, 3M) real products; vs numpy
-    (fp32 bars as for the general test: 3e-6 and 6e-6 x sqrt(K) of the largest element)."""
+    (mode 4), fp32 accumulation of four (mode 0) and three (mode 5, 3M) real products and the
+    split-bf16 products (modes 6, 7); vs numpy with the bars of the general test."""
     from mpdo_amd import _lib
     rng = np.random.default_rng(M * 11 + N + K)
     batch = 3
@@ -119,8 +125,7 @@ def test_gemm_few_rows(cuda, prec, M, N, K, tA):
     opA = np.swapaxes(A, 1, 2) if tA else A
     ref = opA.astype(np.complex128) @ B.astype(np.complex128)
     got = C.cpu().numpy()
-    tol = {4: 1e-6, 0: 3e-6 * np.sqrt(K), 5: 6e-6 * np.sqrt(K)}[prec]
-    assert np.abs(got - ref).max() / np.abs(ref).max() < tol
+    assert np.abs(got - ref).max() / np.abs(ref).max() < _TOL[prec](K)
 
 
 @pytest.mark.parametrize('tri', [1, 2])

@@ -374,10 +374,16 @@ extern "C" {
 int mpdo_version(void) { return 1; }
 
 int mpdo_set_math_mode(int mode) {
-  const int old = g_tensor_prec == C64_F32ACC ? 1 : g_tensor_prec == C64_F32ACC_3M ? 2 : 0;
+  const int old = g_tensor_prec == C64_F32ACC      ? 1
+                  : g_tensor_prec == C64_F32ACC_3M ? 2
+                  : g_tensor_prec == C64_BF16X3    ? 3
+                  : g_tensor_prec == C64_BF16X6    ? 4
+                                                   : 0;
   if (mode == 0) g_tensor_prec = C64_F64ACC_C64OUT;
   else if (mode == 1) g_tensor_prec = C64_F32ACC;
   else if (mode == 2) g_tensor_prec = C64_F32ACC_3M;
+  else if (mode == 3) g_tensor_prec = C64_BF16X3;
+  else if (mode == 4) g_tensor_prec = C64_BF16X6;
   return old;
 }
 const char* mpdo_last_error(void) { return g_err.c_str(); }
@@ -1056,7 +1062,7 @@ mpdo_status mpdo_debug_gemm_ex(int prec, int herm, int inA, int inB, int tri, in
                                long long csB, long long bsB, int conjB, void* C, long long rsC, long long csC,
                                long long bsC, double alpha_re, double alpha_im, double beta_re, double beta_im,
                                void* stream) {
-  if (prec < 0 || prec > 5) return fail(MPDO_EINVAL, "prec %d", prec);
+  if (prec < 0 || prec > 7) return fail(MPDO_EINVAL, "prec %d", prec);
   if (herm && M != N) return fail(MPDO_EINVAL, "Hermitian output needs M == N (%d, %d)", M, N);
   if (inA < 0 || inA > 2 || inB < 0 || inB > 2) return fail(MPDO_EINVAL, "operand storage %d %d", inA, inB);
   if (tri < 0 || tri > 2 || (tri && herm)) return fail(MPDO_EINVAL, "tri %d", tri);

@@ -12,6 +12,7 @@
 // elements of the reduction per LDS stage, the next stage prefetched into registers while
 // the current one feeds the matrix cores. Operands are read along whichever of their two
 // dimensions is unit-stride, so transposed and conjugated views need no copies.
+#include "gemm_tile.h"
 #include "mpdo_internal.h"
 #include "prof.h"
 
@@ -85,27 +86,8 @@ __global__ __launch_bounds__(256, (gemm_wg_per_cu<TInA, TInB, NBUF, G3>())) void
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * WM, wn = (wave & 1) * WN;
   const int b = blockIdx.z;
-  // tile order: blockIdx.x enumerates (m-tile, n-tile) with n fastest; a Hermitian output
-  // (BM == BN) enumerates the tiles with n-tile <= m-tile only
   int m0, n0;
-  if (g.herm) {
-    const int e = blockIdx.x;
-    int mt = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
-    while (mt * (mt + 1) / 2 > e) --mt;
-    while ((mt + 1) * (mt + 2) / 2 <= e) ++mt;
-    m0 = mt * BM;
-    n0 = (e - mt * (mt + 1) / 2) * BN;
-  } else if (g.tri == 1) {
-    // work grows with the n-tile: m fastest, so that consecutive workgroups (dealt round-robin to
-    // the XCDs) share an n-tile and every XCD gets every n-tile
-    const int ntm = (g.M + BM - 1) / BM;
-    m0 = (blockIdx.x % ntm) * BM;
-    n0 = (blockIdx.x / ntm) * BN;
-  } else {
-    const int ntn = (g.N + BN - 1) / BN;
-    m0 = (blockIdx.x / ntn) * BM;
-    n0 = (blockIdx.x % ntn) * BN;
-  }
+  gemm_tile_origin<BM, BN>(g, blockIdx.x, m0, n0);
   const int ks = blockIdx.y;   // reduction slice (0 without a split)
   const TInA* A = (const TInA*)g.A + (long long)b * g.bsA + ks * g.ksA;
   const TInB* B = (const TInB*)g.B + (long long)b * g.bsB + ks * g.ksB;
@@ -213,8 +195,7 @@ __global__ __launch_bounds__(256, (gemm_wg_per_cu<TInA, TInB, NBUF, G3>())) void
       }
     }
   };
-  // a triangular operand leaves only part of the reduction nonzero for this tile
-  const int kend = g.tri == 1 ? min(g.K, n0 + BN) : g.tri == 2 ? min(g.K, m0 + BM) : g.K;
+  const int kend = gemm_tile_kend<BM, BN>(g, m0, n0);
   load(0);
   if constexpr (NBUF == 1) {
     for (int k0 = 0; k0 < kend; k0 += BK) {
@@ -437,7 +418,8 @@ hipError_t gemm(Prec p, const GemmArgs& g, hipStream_t s) {
   if (g.tri < 0 || g.tri > 2 || (g.tri && (g.ksplit > 1 || g.herm))) return hipErrorInvalidValue;
   const bool in128 = (p == C128_F64ACC || p == C128_IN_C64_OUT);
   const bool a128 = g.inA ? g.inA == 2 : in128, b128 = g.inB ? g.inB == 2 : in128;
-  if ((p == C64_F32ACC || p == C64_F32ACC_3M) && (a128 || b128)) return hipErrorInvalidValue;
+  const bool f32acc = p == C64_F32ACC || p == C64_F32ACC_3M || p == C64_BF16X3 || p == C64_BF16X6;
+  if (f32acc && (a128 || b128)) return hipErrorInvalidValue;
   const double inbA = a128 ? 16 : 8, inbB = b128 ? 16 : 8;
   const double outb = (p == C64_F64ACC || p == C128_F64ACC) ? 16 : 8;
   const double mn = (double)g.M * g.N, nb = g.batch;
@@ -472,6 +454,8 @@ hipError_t gemm(Prec p, const GemmArgs& g, hipStream_t s) {
     case C128_F64ACC: return launch_f64_mixed<double2>(a128, b128, g, s);
     case C128_IN_C64_OUT:
     case C64_F64ACC_C64OUT: return launch_f64_mixed<float2>(a128, b128, g, s);
+    case C64_BF16X3: return gemm_bf16(2, g, s);
+    case C64_BF16X6: return gemm_bf16(3, g, s);
   }
   return hipErrorInvalidValue;
 }

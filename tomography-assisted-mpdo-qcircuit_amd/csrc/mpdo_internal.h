@@ -40,9 +40,14 @@ enum Prec { C64_F32ACC = 0,   // complex64 in, fp32 MFMA accumulate, complex64 o
             C128_F64ACC = 2,  // complex128 in, fp64 MFMA accumulate, complex128 out
             C128_IN_C64_OUT = 3,    // complex128 in, fp64 accumulate, complex64 out
             C64_F64ACC_C64OUT = 4,    // complex64 in, fp64 accumulate, complex64 out
-            C64_F32ACC_3M = 5 };      // complex64 in, fp32 MFMA accumulate of 3 real products, complex64 out
+            C64_F32ACC_3M = 5,        // complex64 in, fp32 MFMA accumulate of 3 real products, complex64 out
+            C64_BF16X3 = 6,           // complex64 in, bf16 MFMA on 2-part split operands (3 part products), fp32 accumulate, complex64 out
+            C64_BF16X6 = 7 };         // the same with 3-part split operands (6 part products)
 
 hipError_t gemm(Prec p, const GemmArgs& g, hipStream_t s);
+// Complex64 GEMM on the bf16 matrix cores with each fp32 component split into nsplit (2 or 3) bf16
+// parts; fp32 accumulation, complex64 output (gemm_bf16.hip)
+hipError_t gemm_bf16(int nsplit, const GemmArgs& g, hipStream_t s);
 // C[b][i][j] = conj(C[b][j][i]) for j > i (n x n row-major, batch stride bs)
 hipError_t herm_mirror_b(void* C, bool c128, int n, long long bs, int batch, hipStream_t s);
 

@@ -349,8 +349,65 @@ __global__ void kappa_core_gram_kernel(const double2* Hg, const float2* Gt, int 
   }
 }
 
+// The same Gram one K x K block (a2, a2') per workgroup, through the intermediate
+// U[s2][oi][K2'] = sum_oi' g[s2][oi'][K2'] conj(Hg[(oi a2), (oi' a2')]) held in LDS:
+// out[(a2 K2), (a2' K2')] = sum_{s2, oi} conj(g[s2][oi][K2]) U[s2][oi][K2'] (a fifth of the
+// arithmetic of the direct sum, every operand read once). K <= 16.
+__global__ __launch_bounds__(256) void kappa_core_gram_blk_kernel(const double2* Hg, const float2* Gt, int K, int A,
+                                                                  double2* out, long long bsH, long long bsO) {
+  __shared__ double2 g[2][8][16];
+  __shared__ double2 H[8][9];
+  __shared__ double2 U[2][8][16];
+  const int a2 = blockIdx.x / A, a2p = blockIdx.x % A, b = blockIdx.y, tid = threadIdx.x;
+  Hg += (long long)b * bsH;
+  out += (long long)b * bsO;
+  const long long ldh = 8LL * A;
+  if (tid < 16 * K) {
+    const int s2 = tid / (8 * K), oi = (tid / K) % 8, k2 = tid % K;
+    const int o1 = oi >> 2, i = oi & 3;
+    const float2 v = Gt[((o1 * 2 + s2) * 4 + i) * K + k2];
+    g[s2][oi][k2] = make_double2(v.x, v.y);
+  }
+  if (tid < 64) {
+    const int oi = tid >> 3, oj = tid & 7;
+    H[oi][oj] = Hg[((long long)oi * A + a2) * ldh + (long long)oj * A + a2p];
+  }
+  __syncthreads();
+  if (tid < 16 * K) {
+    const int s2 = tid / (8 * K), oi = (tid / K) % 8, k2p = tid % K;
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int oj = 0; oj < 8; ++oj) {   // g[s2][oj][k2p] conj(H[oi][oj])
+      const double2 x = g[s2][oj][k2p], h = H[oi][oj];
+      acc.x += x.x * h.x + x.y * h.y;
+      acc.y += x.y * h.x - x.x * h.y;
+    }
+    U[s2][oi][k2p] = acc;
+  }
+  __syncthreads();
+  if (tid < K * K) {
+    const int k2 = tid / K, k2p = tid % K;
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int s2 = 0; s2 < 2; ++s2)
+#pragma unroll
+      for (int oi = 0; oi < 8; ++oi) {   // conj(g[s2][oi][k2]) U[s2][oi][k2p]
+        const double2 x = g[s2][oi][k2], u = U[s2][oi][k2p];
+        acc.x += x.x * u.x + x.y * u.y;
+        acc.y += x.x * u.y - x.y * u.x;
+      }
+    const long long n = (long long)A * K;
+    out[((long long)a2 * K + k2) * n + (long long)a2p * K + k2p] = acc;
+  }
+}
+
 hipError_t kappa_core_gram(const double2* Hg, const float2* Gt, int K, int A, double2* out, int batch, hipStream_t s) {
   const long long n = (long long)A * K;
+  if (K <= 16) {
+    hipLaunchKernelGGL(kappa_core_gram_blk_kernel, dim3(A * A, batch), dim3(256), 0, s, Hg, Gt, K, A, out, 64LL * A * A,
+                       n * n);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(kappa_core_gram_kernel, dim3(grid1(n * n), batch), dim3(256), sizeof(double2) * 16 * K, s, Hg,
                      Gt, K, A, out, 64LL * A * A, n * n);
   return hipGetLastError();

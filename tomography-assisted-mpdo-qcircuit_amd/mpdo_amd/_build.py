@@ -16,7 +16,7 @@ REPO = os.path.dirname(ROOT)
 CSRC = os.path.join(ROOT, 'csrc')
 INCLUDE = os.path.join(REPO, 'include')
 LIB = os.path.join(PKG, 'libmpdo_gfx950.so')
-SOURCES = ['prof.hip', 'gemm.hip', 'eigh.hip', 'chol.hip', 'tailcut.hip', 'site_ops.hip', 'layer.hip', 'sample.hip',
+SOURCES = ['prof.hip', 'gemm.hip', 'gemm_bf16.hip', 'eigh.hip', 'chol.hip', 'tailcut.hip', 'site_ops.hip', 'layer.hip', 'sample.hip',
            'trace.hip', 'api.hip', 'noalloc.hip']
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 FLAGS = ['-O3', '-std=c++17', '--offload-arch=gfx950', '-fPIC',

@@ -248,11 +248,16 @@ def _view(buf: torch.Tensor, s: Site) -> torch.Tensor:
     return buf[:n].view(s.l, 2, s.a, s.r)
 
 
+MATH_MODES = {'fp64acc': 0, 'fp32acc': 1, 'fp32x3m': 2, 'bf16x3': 3, 'bf16x6': 4}
+
+
 def set_math_mode(mode: str) -> str:
-    """'fp64acc' (default) or 'fp32acc': accumulation precision of the tensor contractions."""
-    modes = {'fp64acc': 0, 'fp32acc': 1}
-    old = load().mpdo_set_math_mode(modes[mode])
-    return {0: 'fp64acc', 1: 'fp32acc'}[old]
+    """Arithmetic of the contractions whose results are complex64 site data: 'fp64acc' (default,
+    fp64 matrix cores), 'fp32acc' (fp32 matrix cores, four real products), 'fp32x3m' (three real
+    products), 'bf16x3' / 'bf16x6' (bf16 matrix cores on operands split into two / three bf16
+    parts, fp32 accumulation). Returns the previous mode's name."""
+    old = load().mpdo_set_math_mode(MATH_MODES[mode])
+    return {v: k for k, v in MATH_MODES.items()}[old]
 
 
 # ---------------------------------------------------------------------------------------------

@@ -454,8 +454,12 @@ hipError_t gemm(Prec p, const GemmArgs& g, hipStream_t s) {
     case C128_F64ACC: return launch_f64_mixed<double2>(a128, b128, g, s);
     case C128_IN_C64_OUT:
     case C64_F64ACC_C64OUT: return launch_f64_mixed<float2>(a128, b128, g, s);
-    case C64_BF16X3: return gemm_bf16(2, g, s);
-    case C64_BF16X6: return gemm_bf16(3, g, s);
+    case C64_BF16X3:
+    case C64_BF16X6:
+      // operands too large for the split-bf16 kernel's 32-bit offsets: four real products on the
+      // fp32 matrix cores (the accuracy class of the three-part split)
+      if (!gemm_bf16_fits(g)) return launch<float2, float2, float, float2, 128, 64, 16>(g, s);
+      return gemm_bf16(p == C64_BF16X3 ? 2 : 3, g, s);
   }
   return hipErrorInvalidValue;
 }

@@ -48,6 +48,9 @@ hipError_t gemm(Prec p, const GemmArgs& g, hipStream_t s);
 // Complex64 GEMM on the bf16 matrix cores with each fp32 component split into nsplit (2 or 3) bf16
 // parts; fp32 accumulation, complex64 output (gemm_bf16.hip)
 hipError_t gemm_bf16(int nsplit, const GemmArgs& g, hipStream_t s);
+// whether gemm_bf16 can address the operands (nonnegative strides, one batch entry's elements
+// within 2 GiB of its first)
+bool gemm_bf16_fits(const GemmArgs& g);
 // C[b][i][j] = conj(C[b][j][i]) for j > i (n x n row-major, batch stride bs)
 hipError_t herm_mirror_b(void* C, bool c128, int n, long long bs, int batch, hipStream_t s);
 

@@ -105,6 +105,7 @@ def _load() -> ctypes.CDLL:
         'mpdo_debug_layer_chi_eigs': (I, [P(D), I]),
         'mpdo_debug_layer_gemm_log': (I, [P(I), I]),
         'mpdo_debug_gemm_tiles': (I, [I]),
+        'mpdo_debug_gemm_bf16_tiles': (I, [I]),
         'mpdo_debug_bj_threads': (I, [I]),
         'mpdo_debug_bj_fused_max': (I, [I]),
         'mpdo_debug_prof_eig_log': (I, [P(I), I]),

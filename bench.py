@@ -49,6 +49,7 @@ for _p in (ROOT, PKG):
 METRIC = 'circuit-layers/sec at χ=256,κ=8 (batched vs plain) @ ≤1e-5 DM error'
 PEAK_TFLOPS = {'fp64': 78.6, 'fp32': 157.3, 'bf16': 2500.0}     # MI355X dense peaks
 MATH_MODES = {'fp64acc': 0, 'fp32x3m': 2, 'bf16x3': 3, 'bf16x6': 4}
+DEFAULT_MATH = 'bf16x6'
 # GEMM groups by the precision field of a recorded shape: (group, executed flops per counted flop,
 # peak, kernel). A complex product counts 8 M N K flops; 3M executes 6 of them, the split-bf16
 # modes 4 real products x 3 or 6 part products (24 or 48 M N K) on the bf16 matrix cores.
@@ -123,16 +124,20 @@ def kernel_trace_summary(path, top=24):
             v[1] += t1 - t0
         tot = sum(v[1] for v in agg.values())
         span = win[-1][1] - win[0][0]
-        g = [v for k, v in agg.items() if 'gemm_kernel' in k]
-        gl, gt = sum(v[0] for v in g), sum(v[1] for v in g)
-        g3 = [v for k, v in agg.items() if 'gemm_kernel' in k and k.rstrip().endswith('true>')]
-        g3l, g3t = sum(v[0] for v in g3), sum(v[1] for v in g3)
+        # GEMM groups: fp64 MFMA (gemm_kernel, last template argument false), 3M on fp32 MFMA
+        # (gemm_kernel ... true>), split-bf16 (gemm_bf16_kernel)
+        groups = {'fp64 MFMA': lambda k: 'gemm_kernel' in k and not k.rstrip().endswith('true>'),
+                  '3M fp32 MFMA': lambda k: 'gemm_kernel' in k and k.rstrip().endswith('true>'),
+                  'split-bf16 MFMA': lambda k: 'gemm_bf16_kernel' in k}
+        parts = []
+        for gname, sel in groups.items():
+            g = [v for k, v in agg.items() if sel(k)]
+            gl, gt = sum(v[0] for v in g), sum(v[1] for v in g)
+            if gl:
+                parts.append(f'{gname}: {gl} launches, {gt / 1e6:.1f} ms, average {gt / gl / 1e6:.4f} ms per launch')
         out += [f'### {title}', '',
                 f'{len(win)} kernels, {tot / 1e6:.1f} ms inside kernels, {span / 1e6:.1f} ms from the first start '
-                f'to the last end. All `mpdo::gemm_kernel` instantiations: {gl} launches, {gt / 1e6:.1f} ms, '
-                f'average {gt / max(gl, 1) / 1e6:.4f} ms per launch; the fp64-MFMA ones {gl - g3l} launches, '
-                f'average {(gt - g3t) / max(gl - g3l, 1) / 1e6:.4f} ms; the 3M ones {g3l} launches, average '
-                f'{g3t / max(g3l, 1) / 1e6:.4f} ms.', '',
+                f'to the last end. GEMM groups: ' + '; '.join(parts) + '.', '',
                 '| kernel | launches | total ms | avg ms | % of kernel time |', '|---|---|---|---|---|']
         items = sorted(agg.items(), key=lambda kv: -kv[1][1])
         for k, (c, t) in items[:top]:
@@ -144,6 +149,11 @@ def kernel_trace_summary(path, top=24):
             out.append(f'| {len(rest)} other kernels | {c} | {t / 1e6:.1f} | {t / c / 1e6:.4f} | {100 * t / tot:.1f} |')
         out.append('')
     return '\n'.join(out)
+
+
+GEMM_ROW_FP64 = 'mpdo::gemm_kernel, fp64 MFMA'
+GEMM_ROW_3M = 'mpdo::gemm_kernel, 3M on fp32 MFMA'
+GEMM_ROW_BF16 = 'mpdo::gemm_bf16_kernel, split-bf16 MFMA'
 
 
 def counter_summary(path):
@@ -168,9 +178,10 @@ def counter_summary(path):
         agg = {}
         for d in win:
             name = d['name'].replace('void ', '').replace('(anonymous namespace)::', '').split('(')[0]
-            if 'gemm_kernel' in name:   # the 3M instantiations end in `true>`
-                name = ('mpdo::gemm_kernel, 3M on fp32 MFMA' if name.rstrip().endswith('true>')
-                        else 'mpdo::gemm_kernel, fp64 MFMA')
+            if 'gemm_bf16_kernel' in name:
+                name = GEMM_ROW_BF16
+            elif 'gemm_kernel' in name:   # the 3M instantiations end in `true>`
+                name = GEMM_ROW_3M if name.rstrip().endswith('true>') else GEMM_ROW_FP64
             a = agg.setdefault(name, [0, {}])
             a[0] += 1
             for c, v in d['c'].items():
@@ -179,14 +190,17 @@ def counter_summary(path):
         out += [f'### window {w + 1}: {len(win)} dispatches', '',
                 '| kernel | dispatches | ' + ' | '.join(f'mean {c} per dispatch' for c in counters) + ' |',
                 '|---|---|' + '---|' * len(counters)]
-        for name, (n, tot) in sorted(agg.items(), key=lambda kv: -kv[1][0])[:12]:
+        gemm_rows = [k for k in (GEMM_ROW_FP64, GEMM_ROW_3M, GEMM_ROW_BF16) if k in agg]
+        rest = [k for k, _ in sorted(agg.items(), key=lambda kv: -kv[1][0]) if k not in gemm_rows][:12]
+        for name in gemm_rows + rest:
+            n, tot = agg[name]
             out.append(f'| `{name[:90]}` | {n} | ' + ' | '.join(f'{tot.get(c, 0.0) / n:.1f}' for c in counters) + ' |')
         out.append('')
     return '\n'.join(out)
 
 
 def _traffic_per_launch(group='fp64'):
-    """The measured HBM bytes per GEMM launch of one group (fp64 MFMA, or 3m) recorded in
+    """The measured HBM bytes per GEMM launch of one group (fp64, 3m, bf16x3, bf16x6) recorded in
     TRAFFIC_FILE (line `traffic_bytes_per_launch_<group> = N`), or None when absent."""
     import re
     try:
@@ -361,7 +375,7 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-threads', type=int, default=None,
                     help='threads of the CPU oracle sample (default: the cores this process may run on)')
-    ap.add_argument('--math', choices=list(MATH_MODES), default='bf16x6',
+    ap.add_argument('--math', choices=list(MATH_MODES), default=DEFAULT_MATH,
                     help='contractions whose results are complex64 site data: fp64 MFMA accumulation, three real '
                          'products per complex product accumulated in fp32 (3M), or bf16 matrix-core products of '
                          'operands split into 2 or 3 bf16 parts (fp32 accumulation); Gram matrices, QR/LQ factors '
@@ -386,6 +400,9 @@ def main():
                     help='print a markdown summary of the timed steps and the profiled pass of a rocprofv3 kernel '
                          'trace (CSV) of this benchmark, then exit (no GPU work)')
     args = ap.parse_args()
+    for path in (args.kernel_trace_summary, args.counter_summary):
+        if path is not None and not os.path.isfile(path):
+            sys.exit(f'bench.py: no such file: {path!r}')
     if args.kernel_trace_summary:
         print(kernel_trace_summary(args.kernel_trace_summary))
         return

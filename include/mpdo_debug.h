@@ -66,8 +66,8 @@ mpdo_status mpdo_debug_layer_dry_run(int batch, int n, const mpdo_site* in, int 
  * (an out-of-range v only reads it). */
 int mpdo_debug_gemm_tiles(int v);
 /* Tile shape of the split-bf16 GEMMs (precisions 6 and 7) for tuning sweeps: 0 = the built-in
- * rules (default), 1 = 64x64, 2 = 128x64, 3 = 64x128, 4 = 32x128; a Hermitian output always
- * takes 64x64. Returns the previous choice (an out-of-range v only reads it). */
+ * rules (default), 1 = 64x64, 2 = 128x64, 3 = 64x128, 4 = 32x128, 5 = 128x32; a Hermitian output
+ * always takes 64x64. Returns the previous choice (an out-of-range v only reads it). */
 int mpdo_debug_gemm_bf16_tiles(int v);
 
 /* Threads per workgroup of the block-Jacobi subproblem kernel (256, 512 or 1024); returns the

@@ -24,10 +24,16 @@ def test_traffic_is_the_profile_figure():
     import bench
     with open(bench.TRAFFIC_FILE) as f:
         text = f.read()
+    labels = {'fp64 MFMA': 'fp64', '3M on fp32 MFMA': '3m', 'split-bf16 MFMA, two parts': 'bf16x3',
+              'split-bf16 MFMA, three parts': 'bf16x6'}
     rows = {g: (float(fe), float(wr), float(by)) for g, fe, wr, by in re.findall(
-        r'\| (fp64 MFMA|3M on fp32 MFMA) \| [0-9]+ \| ([0-9.]+) \| ([0-9.]+) \| ([0-9.e+]+) \|', text)}
-    assert set(rows) == {'fp64 MFMA', '3M on fp32 MFMA'}
-    for key, row in (('fp64', 'fp64 MFMA'), ('3m', '3M on fp32 MFMA')):
+        r'\| (' + '|'.join(labels) + r') \| [0-9]+ \| ([0-9.]+) \| ([0-9.]+) \| ([0-9.e+]+) \|', text)}
+    # the fp64 group and the group of the benchmark's default site-contraction arithmetic
+    prec = {0: 4, 2: 5, 3: 6, 4: 7}[bench.MATH_MODES[bench.DEFAULT_MATH]]
+    default = bench.GEMM_GROUPS.get(prec, bench.FP64_GROUP)[0]
+    assert {'fp64', default} <= {labels[r] for r in rows}
+    for row in rows:
+        key = labels[row]
         want = float(re.search(r'traffic_bytes_per_launch_' + key + r' = ([0-9.e+]+)', text).group(1))
         assert bench._traffic_per_launch(key) == want
         fe, wr, by = rows[row]

@@ -128,6 +128,32 @@ def test_gemm_few_rows(cuda, prec, M, N, K, tA):
     assert np.abs(got - ref).max() / np.abs(ref).max() < _TOL[prec](K)
 
 
+@pytest.mark.parametrize('prec', [6, 7])
+@pytest.mark.parametrize('tile', [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize('M,N,K,tA', [(700, 8, 40, False), (600, 32, 70, True), (33, 600, 50, False)])
+def test_gemm_bf16_tiles(cuda, prec, tile, M, N, K, tA):
+    """Every tile shape of the split-bf16 GEMM (the built-in rules, 64x64, 128x64, 64x128, 32x128,
+    128x32) on outputs with few columns, few rows and ragged edges; vs numpy with the bars of the
+    general test."""
+    from mpdo_amd import _lib
+    lib = _lib.load()
+    rng = np.random.default_rng(M + 7 * N + K + tile)
+    batch = 2
+    A = _crand(rng, batch, *((K, M) if tA else (M, K))).astype(np.complex64)
+    B = _crand(rng, batch, K, N).astype(np.complex64)
+    C = torch.zeros(batch, M, N, dtype=torch.complex64, device=cuda)
+    sa = (1, M, M * K) if tA else (K, 1, M * K)
+    old = lib.mpdo_debug_gemm_bf16_tiles(tile)
+    try:
+        _lib.debug_gemm(prec, torch.from_numpy(A).to(cuda), torch.from_numpy(B).to(cuda), C, M, N, K, sa,
+                        (N, 1, K * N), (N, 1, M * N), batch=batch)
+    finally:
+        lib.mpdo_debug_gemm_bf16_tiles(old)
+    opA = np.swapaxes(A, 1, 2) if tA else A
+    ref = opA.astype(np.complex128) @ B.astype(np.complex128)
+    assert np.abs(C.cpu().numpy() - ref).max() / np.abs(ref).max() < _TOL[prec](K)
+
+
 @pytest.mark.parametrize('tri', [1, 2])
 @pytest.mark.parametrize('M,N,K', [(300, 130, 130), (130, 257, 130), (64, 64, 64), (17, 40, 40), (40, 17, 40)])
 def test_gemm_triangular_operand(cuda, tri, M, N, K):

@@ -251,21 +251,23 @@ bool gemm_bf16_fits(const GemmArgs& g) {
 // product, nsplit = 3: six), fp32 accumulation, complex64 output. A Hermitian output needs C
 // row-major and contiguous (rsC = N, csC = 1); the operands must pass gemm_bf16_fits.
 // Tile shape of the split-bf16 GEMMs: 0 = by the rules below, 1 = 64 x 64, 2 = 128 x 64,
-// 3 = 64 x 128, 4 = 32 x 128 (a Hermitian output always takes 64 x 64). Measured at the
-// benchmark's site-contraction shapes (2048 x 8192 x 512, 256 x 16384 x 256, 8192 x 512 x 512,
+// 3 = 64 x 128, 4 = 32 x 128, 5 = 128 x 32 (a Hermitian output always takes 64 x 64). Measured at
+// the benchmark's site-contraction shapes (2048 x 8192 x 512, 256 x 16384 x 256, 8192 x 512 x 512,
 // 1024 x 512 x 1024, 32 batches), three-part split: 64 x 128 ahead of 64 x 64 by 7-16 %,
-// 128 x 64 and 32 x 128 behind it; at most 32 rows (16 x 4096 x 256): 32 x 128.
+// 128 x 64 and 32 x 128 behind it; at most 32 rows (16 x 4096 x 256): 32 x 128; at most 32
+// columns: 128 x 32.
 int g_bf16_tile = 0;
 
 template <int NS>
 static hipError_t launch_bf16(const GemmArgs& g, hipStream_t s) {
   int v = g_bf16_tile;
-  if (v == 0) v = (g.M <= 32 && g.N >= 512 && !g.tri) ? 4 : 3;
+  if (v == 0) v = (g.M <= 32 && g.N >= 512 && !g.tri) ? 4 : (g.N <= 32 && g.M >= 512 && !g.tri) ? 5 : 3;
   if (g.herm) v = 1;
   switch (v) {
     case 2: return launch_bf16_tile<NS, 128, 64>(g, s);
     case 3: return launch_bf16_tile<NS, 64, 128>(g, s);
     case 4: return launch_bf16_tile<NS, 32, 128>(g, s);
+    case 5: return launch_bf16_tile<NS, 128, 32>(g, s);
     default: return launch_bf16_tile<NS, 64, 64>(g, s);
   }
 }
@@ -283,6 +285,6 @@ hipError_t gemm_bf16(int nsplit, const GemmArgs& g, hipStream_t s) {
 
 extern "C" int mpdo_debug_gemm_bf16_tiles(int v) {
   const int old = mpdo::g_bf16_tile;
-  if (v >= 0 && v <= 4) mpdo::g_bf16_tile = v;
+  if (v >= 0 && v <= 5) mpdo::g_bf16_tile = v;
   return old;
 }

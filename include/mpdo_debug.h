@@ -117,6 +117,10 @@ int mpdo_debug_layer_gate_tails(void);
  * pair-bond chi steps of the last fused layer on this thread were decided that way. */
 int mpdo_debug_layer_chi_top(int on);
 int mpdo_debug_layer_chi_tops(void);
+/* Subspace of the chi step's top-eigenpair solver: extra > 0 takes k + extra columns (rounded up
+ * to a multiple of 64; default 128), extra = 0 takes 2k, extra < 0 leaves it; q >= 1 sets the
+ * subspace-iteration steps (default 5). Returns the previous extra. */
+int mpdo_debug_layer_top_shape(int extra, int q);
 /* That top-k solver on `batch` Hermitian positive semidefinite complex128 Gram matrices G (device,
  * n x n each, contiguous; not modified): m subspace columns, q steps. info (host, 2 ints): accepted,
  * the step that declined. When accepted, w (device, n doubles per matrix) holds the m Ritz values

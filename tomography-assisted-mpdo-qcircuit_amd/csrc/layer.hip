@@ -102,11 +102,22 @@ bool g_gate_tail = true;               // mpdo_debug_layer_gate_tail
 // the bottom-of-the-spectrum cut: subspace columns and iteration steps; below order PARTIAL_MIN the
 // full eigensolver costs less than the partial solvers' launches and readbacks
 constexpr int TAIL_M = 128, TAIL_Q = 5, PARTIAL_MIN = 512;
-// the chi step at a pair bond by its top k = chi eigenpairs: 2k subspace columns, iteration steps
-constexpr int TOP_Q = 4;
+// the chi step at a pair bond by its top k = chi eigenpairs: k + g_top_extra subspace columns
+// (rounded up to a multiple of 64; 2k when g_top_extra = 0) and g_top_q iteration steps
+// (mpdo_debug_layer_top_shape). Measured on two steady-state layers of the 32-qubit chi = 256,
+// kappa = 8 benchmark (k = 256 of 1024 and 128 of 512): k + 128 columns and 5 steps certify all
+// 29 chi steps at 8.39 circuit-layers/s; 2k columns and 4 steps 7.61 (2 declined: the wider
+// subspace reaches the part of the spectrum below the Rayleigh-Ritz solver's floor); k + 128 with
+// 4 steps 7.27 (5 declined); k + 192 with 4 or 5 steps 8.01 / 7.86; k + 32 with 6 or 7 steps
+// about 5 (most declined).
+int g_top_extra = 128;
+int g_top_q = 5;
 thread_local int t_chi_tops = 0;       // pair-bond chi steps decided that way (last layer)
 bool g_chi_top = true;                 // mpdo_debug_layer_chi_top
-int top_m(int n, int k) { return (k >= 16 && 2 * k + 64 <= n && n >= PARTIAL_MIN && n <= 1024) ? 2 * k : 0; }
+int top_m(int n, int k) {
+  const int m = g_top_extra > 0 ? (k + g_top_extra + 63) / 64 * 64 : 2 * k;
+  return (k >= 16 && m + 64 <= n && n >= PARTIAL_MIN && n <= 1024) ? m : 0;
+}
 
 void add_span(const void* p, size_t bytes) {
   if (t_spans && p && bytes) t_spans->push_back(Span{(uintptr_t)p, (uintptr_t)p + bytes});
@@ -1619,7 +1630,7 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
     if (mt && !dry && !t_rec_gate && use_gram_eig() && g_chi_top && kp <= P.step_len[i]) {
       if (!fits(eE.ws, (long long)gram_top_workspace_bytes(nC, mt, B), 1, "chi step workspace", __LINE__))
         return layer_fail(MPDO_EINVAL, t_oob.c_str());
-      LCK(gram_top_eig(nC, B, G, kp, mt, TOP_Q, eE.ws, s, eE.w, nC, eE.V, nC, (long long)nC * nC, &top));
+      LCK(gram_top_eig(nC, B, G, kp, mt, g_top_q, eE.ws, s, eE.w, nC, eE.V, nC, (long long)nC * nC, &top));
       if (top.ok) ++t_chi_tops;
     }
     if (!top.ok) LST(eig_b(nC, B, G, eE, s, true));
@@ -1773,6 +1784,13 @@ int mpdo_debug_layer_chi_top(int on) {
 }
 
 int mpdo_debug_layer_chi_tops(void) { return t_chi_tops; }
+
+int mpdo_debug_layer_top_shape(int extra, int q) {
+  const int old = g_top_extra;
+  if (extra >= 0) g_top_extra = extra;
+  if (q >= 1) g_top_q = q;
+  return old;
+}
 
 int mpdo_debug_layer_kept(int* out, int cap) {
   const int n = (int)t_kept.size();

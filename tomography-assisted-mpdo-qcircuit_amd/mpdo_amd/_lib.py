@@ -98,6 +98,7 @@ def _load() -> ctypes.CDLL:
         'mpdo_debug_layer_gate_tails': (I, []),
         'mpdo_debug_layer_chi_top': (I, [I]),
         'mpdo_debug_layer_chi_tops': (I, []),
+        'mpdo_debug_layer_top_shape': (I, [I, I]),
         'mpdo_debug_gram_top_eig': (I, [I, I, VP, I, I, I, VP, VP, P(I), VP]),
         'mpdo_debug_gram_tail_cut': (I, [I, I, VP, D, I, I, P(I), P(D), VP, P(I), VP]),
         'mpdo_debug_layer_kept': (I, [P(I), I]),

@@ -110,6 +110,9 @@ int mpdo_debug_layer_gate_skip(int on);
  * that way. */
 int mpdo_debug_layer_gate_tail(int on);
 int mpdo_debug_layer_gate_tails(void);
+/* Subspace of the gate-time cut from the bottom of the spectrum: m columns (16..256, default 128;
+ * other values leave it) and q >= 1 inverse-iteration steps (default 4). Returns the previous m. */
+int mpdo_debug_layer_tail_shape(int m, int q);
 /* The chi step at a pair bond keeps chi of about 4 chi directions: the fused layer takes its top chi
  * eigenpairs by subspace iteration on 2 chi columns, Rayleigh-Ritz and a positive-definiteness
  * certificate (the full eigensolver only where that declines); on = 0 always runs the eigensolver,

@@ -99,9 +99,13 @@ thread_local int t_gate_skips = 0;     // gate-time cuts decided by the shifted 
 thread_local int t_gate_tails = 0;     // gate-time cuts decided from the bottom of the spectrum (last layer)
 bool g_gate_skip = true;               // mpdo_debug_layer_gate_skip
 bool g_gate_tail = true;               // mpdo_debug_layer_gate_tail
-// the bottom-of-the-spectrum cut: subspace columns and iteration steps; below order PARTIAL_MIN the
-// full eigensolver costs less than the partial solvers' launches and readbacks
-constexpr int TAIL_M = 128, TAIL_Q = 5, PARTIAL_MIN = 512;
+// the bottom-of-the-spectrum cut: subspace columns and iteration steps (mpdo_debug_layer_tail_shape);
+// below order PARTIAL_MIN the full eigensolver costs less than the partial solvers' launches and
+// readbacks. Measured on two steady-state layers of the 32-qubit chi = 256,
+// kappa = 8 benchmark (18 cuts decided this way): 128 columns with 3, 4 or 5 steps and 96 with 4
+// accept every cut (8.39-8.46 circuit-layers/s); 64 columns decline 2-4 at 4-5 steps, 32 most.
+constexpr int PARTIAL_MIN = 512;
+int g_tail_m = 128, g_tail_q = 4;
 // the chi step at a pair bond by its top k = chi eigenpairs: k + g_top_extra subspace columns
 // (rounded up to a multiple of 64; 2k when g_top_extra = 0) and g_top_q iteration steps
 // (mpdo_debug_layer_top_shape). Measured on two steady-state layers of the 32-qubit chi = 256,
@@ -516,7 +520,7 @@ struct EigB {
 // top-k solver for k up to kmax
 size_t eig_ws_bytes(int n, int batch, int kmax) {
   size_t b = eigh_gram_workspace_bytes(n, batch);
-  if (n >= PARTIAL_MIN && 2 * TAIL_M <= n) b = std::max(b, gram_tail_workspace_bytes(n, TAIL_M, batch));
+  if (n >= PARTIAL_MIN && 2 * g_tail_m <= n) b = std::max(b, gram_tail_workspace_bytes(n, g_tail_m, batch));
   const int k = std::min(kmax, (n - 64) / 2);
   if (top_m(n, k)) b = std::max(b, gram_top_workspace_bytes(n, top_m(n, k), batch));
   return b;
@@ -1286,12 +1290,12 @@ mpdo_status run_layer(int batch, const LayerPlan& P, const mpdo_site* in, const 
       // where that declines, from the full eigensolver
       TailResult tail;
       if (!keeps_all && !dry && !t_rec_gate && use_gram_eig() && len_gate >= nT && g_gate_tail && nT >= PARTIAL_MIN &&
-          2 * TAIL_M <= nT) {
-        if (!fits(eE.ws, (long long)gram_tail_workspace_bytes(nT, TAIL_M, B), 1, "gate-time cut workspace", __LINE__))
+          2 * g_tail_m <= nT) {
+        if (!fits(eE.ws, (long long)gram_tail_workspace_bytes(nT, g_tail_m, B), 1, "gate-time cut workspace", __LINE__))
           return layer_fail(MPDO_EINVAL, t_oob.c_str());
         if (!fits(ps.Um, (long long)nT * nT * B, sizeof(float2), "gate-time projector", __LINE__))
           return layer_fail(MPDO_EINVAL, t_oob.c_str());
-        LCK(gram_tail_cut(nT, B, G, GATE_EPS, TAIL_M, TAIL_Q, eE.ws, s, dk, ddisc + (size_t)B * (nsteps + n + i), ps.Um,
+        LCK(gram_tail_cut(nT, B, G, GATE_EPS, g_tail_m, g_tail_q, eE.ws, s, dk, ddisc + (size_t)B * (nsteps + n + i), ps.Um,
                           nT, (long long)nT * nT, &tail));
       }
       if (keeps_all) {
@@ -1784,6 +1788,13 @@ int mpdo_debug_layer_chi_top(int on) {
 }
 
 int mpdo_debug_layer_chi_tops(void) { return t_chi_tops; }
+
+int mpdo_debug_layer_tail_shape(int m, int q) {
+  const int old = g_tail_m;
+  if (m >= 16 && m <= 256) g_tail_m = m;
+  if (q >= 1) g_tail_q = q;
+  return old;
+}
 
 int mpdo_debug_layer_top_shape(int extra, int q) {
   const int old = g_top_extra;

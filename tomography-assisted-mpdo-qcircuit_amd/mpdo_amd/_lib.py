@@ -96,6 +96,7 @@ def _load() -> ctypes.CDLL:
         'mpdo_debug_layer_gate_skips': (I, []),
         'mpdo_debug_layer_gate_tail': (I, [I]),
         'mpdo_debug_layer_gate_tails': (I, []),
+        'mpdo_debug_layer_tail_shape': (I, [I, I]),
         'mpdo_debug_layer_chi_top': (I, [I]),
         'mpdo_debug_layer_chi_tops': (I, []),
         'mpdo_debug_layer_top_shape': (I, [I, I]),

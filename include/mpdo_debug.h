@@ -53,6 +53,13 @@ mpdo_status mpdo_debug_eigh(int n, int batch, void* A, double* w, void* V, int* 
  * *sweeps = Jacobi sweeps spent on the rank-sized problem. */
 mpdo_status mpdo_debug_eigh_gram(int n, int batch, void* A, double* w, void* V, int* sweeps, void* stream);
 
+/* The unpivoted blocked Cholesky the partial eigensolvers' certificates and Cholesky QR use, on
+ * `batch` Hermitian complex128 n x n matrices (row-major, contiguous, the lower triangle read;
+ * destroyed): L (n x n complex128, stored by columns: L[k n + i] = L_ik) with A = L L^H, stopping at
+ * the first pivot at or below tol_rel x max_i A_ii (later columns zero); rank[b] = the columns
+ * produced. */
+mpdo_status mpdo_debug_chol(int n, int batch, void* A, double tol_rel, void* L, int* rank, void* stream);
+
 /* Host-only rehearsal of mpdo_layer_batched: plans the layer and walks every launch with fake
  * device addresses, checking each operand's extent against the buffer it falls in, without
  * touching a GPU. `in[i].data`/`out[i].data` may be any distinct non-overlapping addresses.
@@ -111,7 +118,7 @@ int mpdo_debug_layer_gate_skip(int on);
 int mpdo_debug_layer_gate_tail(int on);
 int mpdo_debug_layer_gate_tails(void);
 /* Subspace of the gate-time cut from the bottom of the spectrum: m columns (16..256, default 128;
- * other values leave it) and q >= 1 inverse-iteration steps (default 4). Returns the previous m. */
+ * other values leave it) and q >= 1 inverse-iteration steps (default 5). Returns the previous m. */
 int mpdo_debug_layer_tail_shape(int m, int q);
 /* The chi step at a pair bond keeps chi of about 4 chi directions: the fused layer takes its top chi
  * eigenpairs by subspace iteration on 2 chi columns, Rayleigh-Ritz and a positive-definiteness

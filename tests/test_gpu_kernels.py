@@ -154,6 +154,44 @@ def test_gemm_bf16_tiles(cuda, prec, tile, M, N, K, tA):
     assert np.abs(C.cpu().numpy() - ref).max() / np.abs(ref).max() < _TOL[prec](K)
 
 
+@pytest.mark.parametrize('n', [1, 37, 64, 100, 200, 513])
+def test_unpivoted_cholesky(cuda, n):
+    """The unpivoted blocked Cholesky (64-column diagonal blocks factored and inverted in one
+    workgroup, the rows below and the trailing matrix as GEMMs): L L^H = A within 1e-12 of max|A| on
+    well-conditioned Hermitian matrices, L lower triangular with a real positive diagonal, rank n;
+    a matrix whose trailing block is singular stops at the first pivot below the threshold."""
+    import ctypes
+    from mpdo_amd import _lib
+    lib = _lib.load()
+    rng = np.random.default_rng(n)
+    batch = 3
+    X = _crand(rng, batch, n, n + 8)
+    A = X @ np.conj(np.swapaxes(X, 1, 2))
+    A = (A + np.conj(np.swapaxes(A, 1, 2))) / 2
+    if n >= 100:   # chain 2: rank n - 30, the deficiency in the last 30 rows
+        Y = _crand(rng, n - 30, n + 8)
+        Y = np.vstack([Y, rng.standard_normal((30, n - 30)) @ Y])
+        A[2] = Y @ Y.conj().T
+    Ad = torch.from_numpy(A.copy()).to(cuda)
+    L = torch.zeros((batch, n, n), dtype=torch.complex128, device=cuda)
+    rank = (ctypes.c_int * batch)()
+    _lib.check(lib.mpdo_debug_chol(n, batch, ctypes.c_void_p(Ad.data_ptr()), 1e-12, ctypes.c_void_p(L.data_ptr()),
+                                   rank, _lib._stream(cuda)), 'mpdo_debug_chol')
+    Lh = np.swapaxes(L.cpu().numpy(), 1, 2)   # stored by columns
+    for b in range(batch):
+        Lb = Lh[b]
+        assert np.abs(np.triu(Lb, 1)).max() == 0
+        if n >= 100 and b == 2:
+            r = rank[b]
+            assert n - 30 - 64 < r <= n - 30, r   # stops inside the singular trailing part
+            assert np.abs(Lb[:, r:]).max() == 0
+            assert np.abs((Lb @ Lb.conj().T - A[b])[:r, :r]).max() <= 1e-10 * np.abs(A[b]).max()
+            continue
+        assert rank[b] == n
+        assert np.all(np.diag(Lb).real > 0) and np.abs(np.diag(Lb).imag).max() == 0
+        assert np.abs(Lb @ Lb.conj().T - A[b]).max() <= 1e-12 * np.abs(A[b]).max()
+
+
 @pytest.mark.parametrize('tri', [1, 2])
 @pytest.mark.parametrize('M,N,K', [(300, 130, 130), (130, 257, 130), (64, 64, 64), (17, 40, 40), (40, 17, 40)])
 def test_gemm_triangular_operand(cuda, tri, M, N, K):

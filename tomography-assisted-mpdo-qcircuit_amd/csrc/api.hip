@@ -1170,6 +1170,24 @@ mpdo_status mpdo_debug_gram_top_eig(int n, int batch, const void* G, int k, int 
   return MPDO_OK;
 }
 
+mpdo_status mpdo_debug_chol(int n, int batch, void* A, double tol_rel, void* L, int* rank, void* stream) {
+  if (n <= 0 || batch <= 0 || !A || !L || !rank) return fail(MPDO_EINVAL, "n=%d batch=%d or a null pointer", n, batch);
+  hipStream_t s = S(stream);
+  Carver c0;
+  c0.get<char>(chol_workspace_bytes(batch));
+  const size_t o_piv = c0.off;
+  c0.get<int>((size_t)n * batch);
+  const size_t o_rank = c0.off;
+  c0.get<int>((size_t)batch);
+  char* base;
+  ST(ws_reserve(c0.off, s, &base));
+  int* dr = (int*)(base + o_rank);
+  CK(chol_b(n, batch, A, n, (long long)n * n, tol_rel, (double2*)L, (int*)(base + o_piv), dr, base, s));
+  CK(hipMemcpyAsync(rank, dr, sizeof(int) * batch, hipMemcpyDeviceToHost, s));
+  CK(hipStreamSynchronize(s));
+  return MPDO_OK;
+}
+
 mpdo_status mpdo_debug_eigh_gram(int n, int batch, void* A, double* w, void* V, int* sweeps, void* stream) {
   if (n <= 0 || batch <= 0) return fail(MPDO_EINVAL, "n=%d batch=%d", n, batch);
   hipStream_t s = S(stream);

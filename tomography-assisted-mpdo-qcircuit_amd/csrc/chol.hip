@@ -394,79 +394,133 @@ __global__ __launch_bounds__(256) void chol_init_kernel(const double2* A, int n,
   }
 }
 
-// Diagonal block j0 .. j0 + nb - 1 of matrix b (already updated by the earlier panels): Cholesky in
-// LDS, written to L (column k at L[k n + i]); its triangle's inverse to st.Linv. Four waves per
-// matrix (one wave alone is slower: its longer per-thread update loops cost more than the
-// barriers they save).
-constexpr int CD_T = 256;
-__global__ __launch_bounds__(CD_T) void chol_diag_kernel(const double2* A, int n, long long lda, long long bsA, int j0,
+// Diagonal block j0 .. j0 + nb - 1 of matrix b (already updated by the earlier panels): Cholesky,
+// written to L (column k at L[k n + i]), and its triangle's inverse to st.Linv. 256 threads as a
+// 16 x 16 grid; thread (ty, tx) keeps the elements (ty + 16 a, tx + 16 c), a, c < 4, in registers.
+// Factor: at column j the owners of column j publish it (unscaled) to LDS, and every thread
+// subtracts S_ij conj(S_kj) / S_jj from its elements with j < k <= i; the columns are scaled by
+// 1 / sqrt(S_jj) at the end. Inverse X = L^{-1}, row by row: the owners of row i divide it by L_ii
+// and publish it, every thread subtracts L_ri X_it from its rows r > i. Both publish into
+// alternating LDS buffers: one barrier per column (row).
+__global__ __launch_bounds__(256) void chol_diag_kernel(const double2* A, int n, long long lda, long long bsA, int j0,
                                                         double2* L, ChState st) {
-  __shared__ double2 S[64][65];
-  __shared__ double2 X[64][65];
-  const int b = blockIdx.x, tid = threadIdx.x;
+  __shared__ double2 vec[2][64];
+  __shared__ double2 Ls[64][65];
+  const int b = blockIdx.x, tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
   if (st.stop[b]) return;
   const int nb = min(64, n - j0);
   A += (long long)b * bsA + (long long)j0 * lda + j0;
-  for (int e = tid; e < nb * nb; e += CD_T) {
-    const int i = e / nb, k = e % nb;
-    S[i][k] = A[(long long)i * lda + k];
-  }
+  double2 v[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int i = ty + 16 * a, k = tx + 16 * c;
+      v[a][c] = (i < nb && k <= i) ? A[(long long)i * lda + k] : make_double2(0.0, 0.0);
+    }
   const double thr = st.thr[b];
+  double dk[4] = {0.0, 0.0, 0.0, 0.0};   // S_kk of the thread's columns k = tx + 16 c
   int fail = -1;
   for (int j = 0; j < nb; ++j) {
+    double2* col = vec[j & 1];
+    // (register arrays indexed by unrolled constants only: the owner tests run over every c)
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (tx + 16 * c == j) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          const int i = ty + 16 * a;
+          if (i >= j && i < nb) col[i] = v[a][c];
+        }
+      }
     __syncthreads();
-    const double d = S[j][j].x;
+    const double d = col[j].x;
     if (!(d > thr)) { fail = j; break; }   // uniform: every thread read the same value
-    const double piv = sqrt(d);
-    __syncthreads();
-    for (int i = j + 1 + tid; i < nb; i += CD_T) S[i][j] = make_double2(S[i][j].x / piv, S[i][j].y / piv);
-    if (tid == 0) S[j][j] = make_double2(piv, 0.0);
-    __syncthreads();
-    // S[i][k] -= S[i][j] conj(S[k][j]) for j < k <= i < nb
-    const int m = nb - j - 1;
-    for (int e = tid; e < m * m; e += CD_T) {
-      const int i = j + 1 + e / m, k = j + 1 + e % m;
-      if (k <= i) {
-        const double2 x = S[i][j], y = S[k][j];
-        S[i][k].x -= x.x * y.x + x.y * y.y;
-        S[i][k].y -= x.y * y.x - x.x * y.y;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (tx + 16 * c == j) dk[c] = d;
+    const double inv = 1.0 / d;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int i = ty + 16 * a;
+      if (i <= j || i >= nb) continue;
+      const double2 x = col[i];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int k = tx + 16 * c;
+        if (k <= j || k > i) continue;
+        const double2 y = col[k];
+        v[a][c].x -= (x.x * y.x + x.y * y.y) * inv;
+        v[a][c].y -= (x.y * y.x - x.x * y.y) * inv;
       }
     }
   }
-  __syncthreads();
   double2* Lb = L + (long long)b * n * n;
   double2* Li = st.Linv + (long long)b * 64 * 64;
+  // the factor's columns k < kend (all nb, or those before a failed pivot): L_ik = S_ik / sqrt(S_kk)
+  // below the diagonal, sqrt(S_kk) on it
+  const int kend = fail >= 0 ? fail : nb;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int k = tx + 16 * c;
+    const double rk = k < kend ? sqrt(dk[c]) : 1.0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int i = ty + 16 * a;
+      double2 l = make_double2(0.0, 0.0);
+      if (i < nb && k < kend && k <= i) {
+        l = (k == i) ? make_double2(rk, 0.0) : make_double2(v[a][c].x / rk, v[a][c].y / rk);
+        Lb[(long long)(j0 + k) * n + j0 + i] = l;
+      }
+      Ls[i][k] = l;
+      // the inverse starts from the identity (the right-hand sides)
+      v[a][c] = make_double2(i == k ? 1.0 : 0.0, 0.0);
+    }
+  }
   if (fail >= 0) {
-    for (int e = tid; e < 64 * 64; e += CD_T) Li[e] = make_double2(0.0, 0.0);
+    for (int e = tid; e < 64 * 64; e += 256) Li[e] = make_double2(0.0, 0.0);
     if (tid == 0) {
       st.rank[b] = j0 + fail;
       st.stop[b] = 1;
     }
     return;
   }
-  for (int e = tid; e < nb * nb; e += CD_T) {
-    const int i = e / nb, k = e % nb;
-    if (k <= i) Lb[(long long)(j0 + k) * n + j0 + i] = S[i][k];
-  }
-  // inverse of the lower triangle, row by row: X starts as the identity (the right-hand sides);
-  // row i is final once divided by S_ii, then its multiples leave the rows below it (one rank-1
-  // update per row, spread over the workgroup)
-  for (int e = tid; e < 64 * 64; e += CD_T) X[e / 64][e % 64] = make_double2((e / 64 == e % 64) ? 1.0 : 0.0, 0.0);
   __syncthreads();
   for (int i = 0; i < nb; ++i) {
-    const double dii = S[i][i].x;   // real and positive
-    if (tid <= i) X[i][tid] = make_double2(X[i][tid].x / dii, X[i][tid].y / dii);
+    double2* row = vec[i & 1];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+      if (ty + 16 * a == i) {
+        const double dii = Ls[i][i].x;   // real and positive
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int t = tx + 16 * c;
+          if (t <= i) {
+            v[a][c] = make_double2(v[a][c].x / dii, v[a][c].y / dii);
+            row[t] = v[a][c];
+          }
+        }
+      }
     __syncthreads();
-    const int rows = nb - i - 1, cols = i + 1;
-    for (int e = tid; e < rows * cols; e += CD_T) {   // X[r][t] -= S[r][i] X[i][t], r > i, t <= i
-      const int r = i + 1 + e / cols, t = e % cols;
-      const double2 a = S[r][i], x = X[i][t];
-      X[r][t].x -= a.x * x.x - a.y * x.y;
-      X[r][t].y -= a.x * x.y + a.y * x.x;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int r = ty + 16 * a;
+      if (r <= i || r >= nb) continue;
+      const double2 l = Ls[r][i];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int t = tx + 16 * c;
+        if (t > i) continue;
+        const double2 x = row[t];
+        v[a][c].x -= l.x * x.x - l.y * x.y;
+        v[a][c].y -= l.x * x.y + l.y * x.x;
+      }
     }
-    __syncthreads();
   }
-  for (int e = tid; e < 64 * 64; e += CD_T) Li[e] = X[e / 64][e % 64];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) Li[(ty + 16 * a) * 64 + tx + 16 * c] = v[a][c];
   if (tid == 0) st.rank[b] = j0 + nb;
 }
 
@@ -493,7 +547,7 @@ hipError_t chol_b(int n, int batch, void* A, long long lda, long long bsA, doubl
   if ((err = hipGetLastError()) != hipSuccess) return err;
   double2* Ab = (double2*)A;
   for (int j0 = 0; j0 < n; j0 += 64) {
-    hipLaunchKernelGGL(chol_diag_kernel, dim3(batch), dim3(CD_T), 0, st, (const double2*)A, n, lda, bsA, j0, L, cs);
+    hipLaunchKernelGGL(chol_diag_kernel, dim3(batch), dim3(256), 0, st, (const double2*)A, n, lda, bsA, j0, L, cs);
     if ((err = hipGetLastError()) != hipSuccess) return err;
     const int nb = std::min(64, n - j0), rest = n - j0 - nb;
     if (rest <= 0) break;

@@ -108,6 +108,7 @@ def _load() -> ctypes.CDLL:
         'mpdo_debug_layer_gemm_log': (I, [P(I), I]),
         'mpdo_debug_gemm_tiles': (I, [I]),
         'mpdo_debug_gemm_bf16_tiles': (I, [I]),
+        'mpdo_debug_chol': (I, [I, I, VP, ctypes.c_double, VP, P(I), VP]),
         'mpdo_debug_bj_threads': (I, [I]),
         'mpdo_debug_bj_fused_max': (I, [I]),
         'mpdo_debug_prof_eig_log': (I, [P(I), I]),

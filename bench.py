@@ -380,6 +380,10 @@ def main():
                          'products per complex product accumulated in fp32 (3M), or bf16 matrix-core products of '
                          'operands split into 2 or 3 bf16 parts (fp32 accumulation); Gram matrices, QR/LQ factors '
                          'and eigensolves stay fp64')
+    ap.add_argument('--streams', type=int, default=3,
+                    help='the copies run as this many sub-batches, each through every warmup and timed layer on a '
+                         'HIP stream of its own (batched.layers_batched_streams); the profiled pass runs one batch '
+                         'on one stream')
     ap.add_argument('--uneven', type=int, default=64,
                     help='circuits of the 256-circuit uneven batch (workloads.random_brickwork_batch) to time through '
                          'scheduler.evolve_many after the main measurement (0: skip)')
@@ -446,24 +450,22 @@ def main():
     torch.cuda.synchronize()
     _log(f'{B} copies, {N_QUBITS} qubits, chi={CHI}, kappa={KAPPA}; warmup {args.warmup}, steps {args.steps}')
 
-    def step(k, sites):
-        L = layers[k]
-        outs, _ = _lib.layer_batched(sites, L.kraus1, L.parity, L.gates2, CHI, KAPPA)
+    n_streams = len(batched._parts(B, args.streams))
+
+    def run(first, count, sites):
+        outs, _ = batched.layers_batched_streams(sites, layers[first:first + count], CHI, KAPPA, streams=args.streams)
         return outs
 
-    for k in range(args.warmup):
-        t = time.perf_counter()
-        sites = step(k, sites)
-        torch.cuda.synchronize()
-        _log(f'warmup step {k}: {time.perf_counter() - t:.2f} s')
+    t = time.perf_counter()
+    sites = run(0, args.warmup, sites)
+    torch.cuda.synchronize()
+    _log(f'warmup: {args.warmup} layers in {time.perf_counter() - t:.2f} s')
 
     saved = None if args.fused_only else [t.clone() for t in sites]   # input of the first timed layer
     _trace_marker()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for k in range(args.steps):
-        sites = step(args.warmup + k, sites)
-        _log(f'timed step {k} issued')
+    sites = run(args.warmup, args.steps, sites)
     torch.cuda.synchronize()
     T = time.perf_counter() - t0
     _trace_marker()
@@ -477,8 +479,7 @@ def main():
         s64 = [t.clone() for t in saved]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for k in range(args.steps):
-            s64 = step(args.warmup + k, s64)
+        s64 = run(args.warmup, args.steps, s64)
         torch.cuda.synchronize()
         T64 = time.perf_counter() - t0
         del s64
@@ -486,6 +487,9 @@ def main():
         fp64 = {'value': round(B * args.steps / T64, 6), 'ms_per_step': round(T64 / args.steps * 1e3, 3),
                 'math': 'every site contraction with fp64 MFMA accumulation (mpdo_set_math_mode(0)), same layers and input'}
         _log(f'fp64 accumulation: {T64:.2f} s, {fp64["value"]:.3f} circuit-layers/s')
+    # the sub-batch streams' workspaces (3.6 GiB per chain) go back before the one-stream passes below
+    batched.release_stream_pools(dev)
+    torch.cuda.empty_cache()
 
     roofline, others, plain, accuracy, cpu = None, None, None, None, None
     if not args.fused_only:
@@ -643,9 +647,10 @@ def main():
             'data': 'synthetic',
             'config': {'workload': ('32-qubit depth-30 RY/RZ+CNOT brickwork, depolarizing p=0.002 (1q, 2q), '
                                     'chi=256, kappa=8, 32 angle-perturbed copies; step = one circuit-layer of '
-                                    'every copy from the steady state (fused batched layer)'),
+                                    'every copy from the steady state (fused batched layers; the copies as '
+                                    f'{n_streams} sub-batches, each on a HIP stream of its own)'),
                        'n_qubits': N_QUBITS, 'chi': CHI, 'kappa': KAPPA, 'copies': B, 'global_batch': B,
-                       'parallelism': 'none (one GPU)'},
+                       'streams': n_streams, 'parallelism': 'none (one GPU)'},
             'roofline': roofline, 'roofline_block_jacobi': others, 'plain': plain,
             'batched_vs_plain': plain['batched_vs_plain'] if plain else None, 'fp64_accumulate': fp64,
             'accuracy': accuracy,

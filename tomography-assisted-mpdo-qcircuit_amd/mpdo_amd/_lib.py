@@ -492,10 +492,11 @@ def layer_dims(dims: Sequence[Tuple[int, int, int]], K1: int, parity: int, K2: i
 
 
 def layer_batched(sites: Sequence[torch.Tensor], kraus1: torch.Tensor, parity: int, gates2: torch.Tensor,
-                  chi: Optional[int], kappa: Optional[int]):
+                  chi: Optional[int], kappa: Optional[int], out: Optional[Sequence[torch.Tensor]] = None):
     """One fused brickwork layer. sites[i]: (B, l, 2, a, r) complex64 on the device;
     kraus1: (B, n, K1, 2, 2); gates2: (npairs, 2, 2, 2, 2, K2). Returns (new sites, discarded
-    (B, 3n) numpy: chi steps by bond, kappa cuts by site, gate-time splits by pair min site)."""
+    (B, 3n) numpy: chi steps by bond, kappa cuts by site, gate-time splits by pair min site).
+    out: optional contiguous complex64 device tensors of the new sites' shapes to write into."""
     lib = load()
     n = len(sites)
     B = sites[0].shape[0]
@@ -511,7 +512,17 @@ def layer_batched(sites: Sequence[torch.Tensor], kraus1: torch.Tensor, parity: i
         if not t.is_contiguous():
             raise ValueError('batched site tensors must be contiguous')
         ins[i] = Site(t.data_ptr(), dims[i][0], dims[i][1], dims[i][2])
-    outs_t = [torch.empty(B, l, 2, a, r, dtype=torch.complex64, device=dev) for (l, a, r) in out_dims]
+    if out is None:
+        outs_t = [torch.empty(B, l, 2, a, r, dtype=torch.complex64, device=dev) for (l, a, r) in out_dims]
+    else:
+        outs_t = list(out)
+        for t, (l, a, r) in zip(outs_t, out_dims):
+            _require_device(t)
+            if tuple(t.shape) != (B, l, 2, a, r) or t.dtype != torch.complex64 or not t.is_contiguous():
+                raise ValueError(f'output tensor {tuple(t.shape)} {t.dtype}: expected contiguous complex64 '
+                                 f'{(B, l, 2, a, r)}')
+        if len(outs_t) != n:
+            raise ValueError(f'{len(outs_t)} output tensors for {n} sites')
     outs = (Site * n)()
     for i, (t, (l, a, r)) in enumerate(zip(outs_t, out_dims)):
         outs[i] = Site(t.data_ptr(), l, a, r)

@@ -380,7 +380,10 @@ def main():
                          'products per complex product accumulated in fp32 (3M), or bf16 matrix-core products of '
                          'operands split into 2 or 3 bf16 parts (fp32 accumulation); Gram matrices, QR/LQ factors '
                          'and eigensolves stay fp64')
-    ap.add_argument('--streams', type=int, default=3,
+    ap.add_argument('--hw-queues', type=int, default=8,
+                    help='GPU_MAX_HW_QUEUES for this process (hardware queues HIP maps its streams onto; HIP\'s '
+                         'default is 4), set before the GPU is initialised; 0 leaves the environment as it is')
+    ap.add_argument('--streams', type=int, default=4,
                     help='the copies run as this many sub-batches, each through every warmup and timed layer on a '
                          'HIP stream of its own (batched.layers_batched_streams); the profiled pass runs one batch '
                          'on one stream')
@@ -418,6 +421,9 @@ def main():
               'Chains are independent, so more GPUs would each take a share of the copies; run with --gpus 1.',
               file=sys.stderr)
         sys.exit(2)
+
+    if args.hw_queues > 0:   # read by the HIP runtime when it initialises (the first GPU call below)
+        os.environ['GPU_MAX_HW_QUEUES'] = str(args.hw_queues)
 
     import numpy as np
     import torch
@@ -650,7 +656,8 @@ def main():
                                     'every copy from the steady state (fused batched layers; the copies as '
                                     f'{n_streams} sub-batches, each on a HIP stream of its own)'),
                        'n_qubits': N_QUBITS, 'chi': CHI, 'kappa': KAPPA, 'copies': B, 'global_batch': B,
-                       'streams': n_streams, 'parallelism': 'none (one GPU)'},
+                       'streams': n_streams, 'hw_queues': os.environ.get('GPU_MAX_HW_QUEUES', '4 (HIP default)'),
+                       'parallelism': 'none (one GPU)'},
             'roofline': roofline, 'roofline_block_jacobi': others, 'plain': plain,
             'batched_vs_plain': plain['batched_vs_plain'] if plain else None, 'fp64_accumulate': fp64,
             'accuracy': accuracy,

@@ -194,9 +194,9 @@ def release_stream_pools(dev: torch.device) -> None:
 def _max_side_streams() -> int:
     """Streams that run concurrently beside the caller's: HIP maps a process's streams onto
     GPU_MAX_HW_QUEUES hardware queues (4 unless set), and streams that share a queue run one after
-    the other. Measured at the 32-qubit chi = 256, kappa = 8 benchmark (32 copies): 2, 3, 4 and 8
-    sub-batch streams beside the caller's gave 8.78, 8.96, 6.99 and 5.72 circuit-layers/s against
-    8.39 on one stream."""
+    the other. Measured at the 32-qubit chi = 256, kappa = 8 benchmark (32 copies): with 4 queues,
+    2, 3, 4 and 8 sub-batch streams beside the caller's gave 8.78, 8.96, 6.99 and 5.72
+    circuit-layers/s against 8.39 on one stream; with 8 queues, 4 and 6 streams 9.28 and 8.43."""
     try:
         q = int(os.environ.get('GPU_MAX_HW_QUEUES', '4'))
     except ValueError:
@@ -211,7 +211,7 @@ def _parts(B: int, streams: int) -> List[Tuple[int, int]]:
 
 
 def layers_batched_streams(sites: Sequence[torch.Tensor], layers: Sequence['FusedLayer'], chi: Optional[int],
-                           kappa: Optional[int], streams: int = 3):
+                           kappa: Optional[int], streams: int = 4):
     """Consecutive fused layers (``_lib.layer_batched`` with each layer's kraus1, parity, gates2)
     over B chains split into up to ``streams`` sub-batches, each advanced through every layer by a
     host thread of its own on a HIP stream of its own, with no synchronisation between the

@@ -110,3 +110,19 @@ def test_gemm_tile_table_round_trip():
             assert sorted(autotune.table(), key=lambda e: e['key']) == got
     finally:
         autotune.set_table(saved)
+
+
+def test_sub_batch_split(monkeypatch):
+    """The sub-batches of layers_batched_streams / evolve_batched(streams=): contiguous, covering
+    every chain once, at most one stream fewer than GPU_MAX_HW_QUEUES (4 when unset)."""
+    from mpdo_amd import batched
+    monkeypatch.delenv('GPU_MAX_HW_QUEUES', raising=False)
+    assert batched._parts(7, 4) == [(0, 3), (3, 6), (6, 7)]
+    assert batched._parts(32, 3) == [(0, 11), (11, 22), (22, 32)]
+    assert batched._parts(1, 4) == [(0, 1)]
+    assert batched._parts(5, 1) == [(0, 5)]
+    monkeypatch.setenv('GPU_MAX_HW_QUEUES', '8')
+    assert batched._parts(32, 4) == [(0, 8), (8, 16), (16, 24), (24, 32)]
+    assert len(batched._parts(32, 16)) == 7
+    monkeypatch.setenv('GPU_MAX_HW_QUEUES', '1')
+    assert batched._parts(32, 4) == [(0, 32)]

@@ -456,7 +456,7 @@ def main():
     torch.cuda.synchronize()
     _log(f'{B} copies, {N_QUBITS} qubits, chi={CHI}, kappa={KAPPA}; warmup {args.warmup}, steps {args.steps}')
 
-    n_streams = len(batched._parts(B, args.streams))
+    n_streams = len(batched.sub_batches(B, args.streams))
 
     def run(first, count, sites):
         outs, _ = batched.layers_batched_streams(sites, layers[first:first + count], CHI, KAPPA, streams=args.streams)

@@ -54,3 +54,45 @@ def test_timed_workload_family():
     for w in ws[1:]:
         d = [abs(a - b) for o, o0 in zip(w.ops, ws[0].ops) for a, b in zip(o[2], o0[2])]
         assert d and max(d) <= 0.01 + 1e-12 and max(d) > 0
+
+
+def _write_csv(path, header, rows):
+    import csv
+    with open(path, 'w', newline='') as f:
+        w = csv.writer(f)
+        w.writerow(header)
+        w.writerows(rows)
+
+
+def test_trace_and_counter_summaries(tmp_path):
+    """The two rocprofv3 summaries on synthetic CSVs: windows between the marker launches, GEMM
+    groups by arithmetic (fp64 gemm_kernel, 3M gemm_kernel ... true>, gemm_bf16_kernel), per-kernel
+    averages, and mean counter values per dispatch."""
+    import bench
+    g64 = 'void mpdo::gemm_kernel<HIP_vector_type<double, 2u>, HIP_vector_type<double, 2u>, double, ' \
+          'HIP_vector_type<double, 2u>, 64, 64, 16, 2, false>(mpdo::GemmArgs)'
+    g3m = 'void mpdo::gemm_kernel<float2, float2, float, float2, 128, 64, 16, 1, true>(mpdo::GemmArgs)'
+    gbf = 'void mpdo::gemm_bf16_kernel<3, 64, 128>(mpdo::GemmArgs)'
+    mark = 'void spin_kernel(long)'
+    # window 1: two fp64 GEMMs (3 and 5 us) and one bf16 (10 us); window 2: one 3M (4 us)
+    trace = [(mark, 0, 1), (g64, 10, 3010), (g64, 4000, 9000), (gbf, 9000, 19000), (mark, 20000, 20001),
+             (mark, 30000, 30001), (g3m, 31000, 35000), (mark, 40000, 40001)]
+    tp = tmp_path / 'bench_kernel_trace.csv'
+    _write_csv(tp, ['Kernel_Name', 'Start_Timestamp', 'End_Timestamp'], trace)
+    text = bench.kernel_trace_summary(str(tp))
+    assert 'fp64 MFMA: 2 launches, 0.0 ms, average 0.0040 ms per launch' in text
+    assert 'split-bf16 MFMA: 1 launches, 0.0 ms, average 0.0100 ms per launch' in text
+    assert '3M fp32 MFMA: 1 launches, 0.0 ms, average 0.0040 ms per launch' in text
+    assert '`mpdo::gemm_bf16_kernel<3, 64, 128>` | 1 |' in text
+    # counters: window 1 (calibration) one neg kernel, window 2 GEMMs of each group
+    neg = 'void at::native::vectorized_elementwise_kernel<4, at::native::neg_kernel_cuda>(int)'
+    disp = [(1, mark, 0), (2, neg, 100.0), (3, mark, 0), (4, mark, 0), (5, g64, 10.0), (6, g64, 30.0),
+            (7, gbf, 50.0), (8, g3m, 7.0), (9, mark, 0)]
+    cp = tmp_path / 'fetch_counter_collection.csv'
+    _write_csv(cp, ['Dispatch_Id', 'Kernel_Name', 'Counter_Name', 'Counter_Value'],
+               [(d, n, 'FETCH_SIZE', v) for d, n, v in disp])
+    text = bench.counter_summary(str(cp))
+    assert '| `mpdo::gemm_kernel, fp64 MFMA` | 2 | 20.0 |' in text
+    assert '| `mpdo::gemm_bf16_kernel, split-bf16 MFMA` | 1 | 50.0 |' in text
+    assert '| `mpdo::gemm_kernel, 3M on fp32 MFMA` | 1 | 7.0 |' in text
+    assert 'neg_kernel_cuda' in text and '| 1 | 100.0 |' in text

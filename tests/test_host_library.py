@@ -117,12 +117,12 @@ def test_sub_batch_split(monkeypatch):
     every chain once, at most one stream fewer than GPU_MAX_HW_QUEUES (4 when unset)."""
     from mpdo_amd import batched
     monkeypatch.delenv('GPU_MAX_HW_QUEUES', raising=False)
-    assert batched._parts(7, 4) == [(0, 3), (3, 6), (6, 7)]
-    assert batched._parts(32, 3) == [(0, 11), (11, 22), (22, 32)]
-    assert batched._parts(1, 4) == [(0, 1)]
-    assert batched._parts(5, 1) == [(0, 5)]
+    assert batched.sub_batches(7, 4) == [(0, 3), (3, 6), (6, 7)]
+    assert batched.sub_batches(32, 3) == [(0, 11), (11, 22), (22, 32)]
+    assert batched.sub_batches(1, 4) == [(0, 1)]
+    assert batched.sub_batches(5, 1) == [(0, 5)]
     monkeypatch.setenv('GPU_MAX_HW_QUEUES', '8')
-    assert batched._parts(32, 4) == [(0, 8), (8, 16), (16, 24), (24, 32)]
-    assert len(batched._parts(32, 16)) == 7
+    assert batched.sub_batches(32, 4) == [(0, 8), (8, 16), (16, 24), (24, 32)]
+    assert len(batched.sub_batches(32, 16)) == 7
     monkeypatch.setenv('GPU_MAX_HW_QUEUES', '1')
-    assert batched._parts(32, 4) == [(0, 32)]
+    assert batched.sub_batches(32, 4) == [(0, 32)]

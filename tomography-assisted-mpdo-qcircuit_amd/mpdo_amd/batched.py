@@ -204,7 +204,10 @@ def _max_side_streams() -> int:
     return max(1, q - 1)
 
 
-def _parts(B: int, streams: int) -> List[Tuple[int, int]]:
+def sub_batches(B: int, streams: int) -> List[Tuple[int, int]]:
+    """The contiguous (lo, hi) chain ranges that ``layers_batched_streams`` and
+    ``evolve_batched(streams=)`` run on separate streams: up to ``streams`` of them, at most one
+    fewer than the hardware queues."""
     k = max(1, min(int(streams), B, _max_side_streams()))
     bs = -(-B // k)
     return [(lo, min(B, lo + bs)) for lo in range(0, B, bs)]
@@ -223,7 +226,7 @@ def layers_batched_streams(sites: Sequence[torch.Tensor], layers: Sequence['Fuse
     batch size). At most one stream fewer than the hardware queues (_max_side_streams).
     Returns (sites after the last layer, [discarded (B, 3n) per layer])."""
     B, n = sites[0].shape[0], len(sites)
-    parts = _parts(B, streams)
+    parts = sub_batches(B, streams)
     if len(parts) == 1:
         disc = []
         for L in layers:
@@ -292,7 +295,7 @@ def evolve_batched(circuits: Sequence[TensorCircuit], states: Sequence[List[Site
     why = _same_structure(cs)
     if why:
         raise ValueError(why)
-    parts = _parts(len(cs), streams)
+    parts = sub_batches(len(cs), streams)
     if len(parts) > 1:
         _evolve_groups(cs, states, parts, fuse_layers, stats)
         return
